@@ -1,0 +1,168 @@
+"""ctypes wrapper of liboracle.so (the C restatement).  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from tcp_allpair_amd.params import NBIN, CParams, TcpParams
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE, "oracle"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ctypes.CDLL(path)
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with {rc}")
+
+
+def ewald_self(p: TcpParams):
+    a, rc2, uo = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _check(lib().tcpo_ewald_self(ctypes.byref(CParams.from_params(p)), ctypes.byref(a),
+                                 ctypes.byref(rc2), ctypes.byref(uo)), "ewald_self")
+    return a.value, rc2.value, uo.value
+
+
+def psi(p: TcpParams):
+    a = ewald_self(p)[0]
+    out = np.zeros(int(np.prod(p.K)))
+    _check(lib().tcpo_psi(ctypes.byref(CParams.from_params(p)), ctypes.c_double(a), _p(out)), "psi")
+    return out.reshape(p.K[2], p.K[1], p.K[0])
+
+
+def force_direct_self(p: TcpParams, x):
+    """Reference loop order; returns (f, Ur, rdf[3,501])."""
+    a, rc2, _ = ewald_self(p)
+    x = _c(x)
+    f = np.zeros_like(x)
+    rdf = np.zeros(3 * NBIN)
+    ur = ctypes.c_double()
+    _check(lib().tcpo_force_direct_self(ctypes.byref(CParams.from_params(p)), ctypes.c_double(a),
+                                        ctypes.c_double(rc2), _p(x), _p(f), ctypes.byref(ur),
+                                        _p(rdf)), "force_direct_self")
+    return f, ur.value, rdf.reshape(3, NBIN)
+
+
+def force_direct_ext(p: TcpParams, ne_o, ni_o, x_own, f_own, ne_g, ni_g, xg):
+    a, rc2, _ = ewald_self(p)
+    x_own, xg = _c(x_own), _c(xg)
+    f_own = _c(f_own).copy()
+    gemit = np.zeros_like(xg)
+    rdf = np.zeros(3 * NBIN)
+    ur = ctypes.c_double()
+    _check(lib().tcpo_force_direct_ext(ctypes.byref(CParams.from_params(p)), ctypes.c_double(a),
+                                       ctypes.c_double(rc2), ne_o, ni_o, _p(x_own), _p(f_own),
+                                       ne_g, ni_g, _p(xg), _p(gemit), ctypes.byref(ur), _p(rdf)),
+           "force_direct_ext")
+    return f_own, gemit, ur.value, rdf.reshape(3, NBIN)
+
+
+def force_allpair(p: TcpParams, x, nthreads=0):
+    """All ordered pairs, OpenMP; returns (f, Ur, rdf[3,501])."""
+    a, rc2, _ = ewald_self(p)
+    x = _c(x)
+    f = np.zeros_like(x)
+    rdf = np.zeros(3 * NBIN)
+    ur = ctypes.c_double()
+    _check(lib().tcpo_force_allpair(ctypes.byref(CParams.from_params(p)), ctypes.c_double(a),
+                                    ctypes.c_double(rc2), _p(x), _p(f), ctypes.byref(ur), _p(rdf),
+                                    int(nthreads)), "force_allpair")
+    return f, ur.value, rdf.reshape(3, NBIN)
+
+
+def spme(p: TcpParams, x):
+    a = ewald_self(p)[0]
+    x = _c(x)
+    f = np.zeros_like(x)
+    um = ctypes.c_double()
+    _check(lib().tcpo_spme(ctypes.byref(CParams.from_params(p)), ctypes.c_double(a), _p(x), _p(f),
+                           ctypes.byref(um)), "spme")
+    return f, um.value
+
+
+def ewald_kspace(p: TcpParams, x):
+    a = ewald_self(p)[0]
+    x = _c(x)
+    f = np.zeros_like(x)
+    um = ctypes.c_double()
+    _check(lib().tcpo_ewald_kspace(ctypes.byref(CParams.from_params(p)), ctypes.c_double(a), _p(x),
+                                   _p(f), ctypes.byref(um)), "ewald_kspace")
+    return f, um.value
+
+
+def bspline(order, u):
+    w = np.zeros(16)
+    dw = np.zeros(16)
+    _check(lib().tcpo_bspline(int(order), ctypes.c_double(u), _p(w), _p(dw)), "bspline")
+    return w[:order].copy(), dw[:order].copy()
+
+
+def bspline_modulus(order, K):
+    b = np.zeros(K)
+    _check(lib().tcpo_bspline_modulus(int(order), int(K), _p(b)), "bspline_modulus")
+    return b
+
+
+def remove_rigidmtn(p: TcpParams, f):
+    f = _c(f).copy()
+    _check(lib().tcpo_remove_rigidmtn(ctypes.byref(CParams.from_params(p)), _p(f)), "remove_rigidmtn")
+    return f
+
+
+def vverlet1(p: TcpParams, thermo, dt, x, v, f):
+    x, v, f = _c(x).copy(), _c(v).copy(), _c(f)
+    _check(lib().tcpo_vverlet1(ctypes.byref(CParams.from_params(p)), int(thermo), ctypes.c_double(dt),
+                               ctypes.c_double(p.fric), _p(x), _p(v), _p(f)), "vverlet1")
+    return x, v
+
+
+def vverlet2(p: TcpParams, thermo, dt, v, f, eta=None):
+    v, f = _c(v).copy(), _c(f).copy()
+    eta = None if eta is None else _c(eta)
+    mv2el, mv2ion = ctypes.c_double(), ctypes.c_double()
+    vdf = np.zeros(2 * NBIN)
+    _check(lib().tcpo_vverlet2(ctypes.byref(CParams.from_params(p)), int(thermo), ctypes.c_double(dt),
+                               ctypes.c_double(p.Te), ctypes.c_double(p.Ti), ctypes.c_double(p.tau),
+                               ctypes.c_double(p.fric), _p(v), _p(f), _p(eta),
+                               ctypes.byref(mv2el), ctypes.byref(mv2ion), _p(vdf)), "vverlet2")
+    return v, f, mv2el.value, mv2ion.value, vdf.reshape(2, NBIN)
+
+
+def step(p: TcpParams, thermo, dt, x, v, f, rdf=None, vdf=None, fast=False, nthreads=0):
+    """One main.f90:87-153 step in place; returns dict of the scalars."""
+    out = np.zeros(6)
+    rdf = np.zeros(3 * NBIN) if rdf is None else rdf
+    vdf = np.zeros(2 * NBIN) if vdf is None else vdf
+    for a in (x, v, f, rdf, vdf):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    _check(lib().tcpo_step(ctypes.byref(CParams.from_params(p)), int(thermo), ctypes.c_double(dt),
+                           ctypes.c_double(p.Te), ctypes.c_double(p.Ti), ctypes.c_double(p.tau),
+                           ctypes.c_double(p.fric), _p(x), _p(v), _p(f), _p(rdf), _p(vdf), _p(out),
+                           int(bool(fast)), int(nthreads)), "step")
+    return dict(Ur=out[0], Uo=out[1], Um=out[2], Epot=out[3], mv2el=out[4], mv2ion=out[5])

@@ -1,0 +1,160 @@
+"""The oracle (C restatement) against the reference's own routines.
+
+The fixtures under tests/golden/ were written by the reference's Fortran
+(force.f90, vverlet.f90) compiled from source; the restatement keeps the
+reference's evaluation order, so most checks are bit-for-bit.
+"""
+import numpy as np
+import pytest
+
+from oracle import cpu
+from tcp_allpair_amd.params import NBIN
+from golden_io import load
+
+
+def rel_force_err(f, ref):
+    """Per-particle relative force error with the floor of 1e-3 * RMS|F_ref|."""
+    nref = np.linalg.norm(ref, axis=1)
+    floor = 1e-3 * np.sqrt(np.mean(nref ** 2))
+    return np.linalg.norm(f - ref, axis=1) / np.maximum(nref, floor)
+
+
+@pytest.mark.parametrize("name", ["force_tiny", "force_c0", "force_bins", "force_aniso"])
+def test_force_direct_self_bitwise(name):
+    p, d = load(name)
+    a, rc2, uo = cpu.ewald_self(p)
+    assert a == d["out_alpha"] and rc2 == d["out_rc2"] and uo == d["out_Uo"]
+    f, ur, rdf = cpu.force_direct_self(p, d["in_x"])
+    np.testing.assert_array_equal(f, d["out_f"])
+    assert ur == d["out_Ur"]
+    np.testing.assert_array_equal(rdf, d["out_rdf"])
+    assert d["out_Epot"] == (ur + uo) * p.eps
+
+
+def test_force_bins_drops_overflow():
+    """RDF bins past Ndist+1 are skipped (force.f90:101): fewer counts than pairs in rc."""
+    p, d = load("force_bins")
+    _, _, rdf = cpu.force_direct_self(p, d["in_x"])
+    x = d["in_x"]
+    dx = x[:, None, :] - x[None, :, :]
+    dx -= np.array(p.box) * np.round(dx / np.array(p.box))
+    r2 = (dx ** 2).sum(-1)
+    inside = (r2 < p.rc ** 2).sum() - p.n
+    assert rdf.sum() < inside
+    assert rdf.sum() == d["out_rdf"].sum()
+
+
+@pytest.mark.parametrize("name", ["force_c0", "force_aniso"])
+def test_force_allpair_matches_reference(name):
+    p, d = load(name)
+    f, ur, rdf = cpu.force_allpair(p, d["in_x"], nthreads=4)
+    assert rel_force_err(f, d["out_f"]).max() < 1e-11
+    assert abs(ur - d["out_Ur"]) <= 1e-12 * abs(d["out_Ur"])
+    np.testing.assert_array_equal(rdf, d["out_rdf"])
+
+
+def test_psi():
+    p, d = load("psi")
+    np.testing.assert_array_equal(cpu.psi(p).ravel(), d["out_psi"].ravel())
+
+
+def test_force_direct_ext_and_sum():
+    p, d = load("ext")
+    ne_o, ni_o = (int(v) for v in d["nown"])
+    x = d["in_x"]
+    own = np.concatenate([x[:ne_o], x[p.ne:p.ne + ni_o]])
+    ext = np.concatenate([x[ne_o:p.ne], x[p.ne + ni_o:]])
+    po = p.replace(ne=ne_o, ni=ni_o)
+    f_self, ur_self, rdf_self = cpu.force_direct_self(po, own)
+    assert ur_self == d["out_Ur_self"]
+    f, gemit, ur_ext, rdf_ext = cpu.force_direct_ext(p, ne_o, ni_o, own, f_self, p.ne - ne_o,
+                                                     p.ni - ni_o, ext)
+    np.testing.assert_array_equal(f, d["out_f"])
+    np.testing.assert_array_equal(gemit, d["out_gemit"])
+    assert ur_ext == d["out_Ur_ext"]
+    np.testing.assert_array_equal(rdf_self + rdf_ext, d["out_rdf"])
+    # Force_direct_sum (force.f90:358-369): ff += grec(i) with grec(i) = i*1e-3 in the harness
+    grec = (np.arange(1, ne_o + ni_o + 1) * 1.0e-3)[:, None] * np.ones(3)
+    np.testing.assert_array_equal(f + grec, d["out_f_after_sum"])
+
+
+def test_force_ext_plus_self_is_full_system():
+    """Own-block self forces + cross block + the other block's reactions = one-block forces."""
+    p, d = load("ext")
+    ne_o, ni_o = (int(v) for v in d["nown"])
+    x = d["in_x"]
+    full, ur_full, _ = cpu.force_allpair(p, x)
+    own_idx = np.r_[0:ne_o, p.ne:p.ne + ni_o]
+    np.testing.assert_allclose(d["out_f"], full[own_idx], rtol=0, atol=1e-9 * np.abs(full).max())
+
+
+def test_remove_rigidmtn():
+    p, d = load("rigid")
+    f = cpu.remove_rigidmtn(p, d["in_f"])
+    np.testing.assert_array_equal(f, d["out_f"])
+    assert np.abs(f.sum(0)).max() < 1e-11
+
+
+@pytest.mark.parametrize("name", ["vv1_notemp", "vv1_stoch"])
+def test_vverlet1(name):
+    p, d = load(name)
+    x, v = cpu.vverlet1(p, int(d["thermo"]), float(d["dt"]), d["in_x"], d["in_v"], d["in_f"])
+    np.testing.assert_array_equal(x, d["out_x"])
+    np.testing.assert_array_equal(v, d["out_v"])
+    np.testing.assert_array_equal(x, d["out_gsend"])
+    assert np.all(np.abs(x) <= 0.5 * np.array(p.box))
+
+
+@pytest.mark.parametrize("name", ["vv2_notemp", "vv2_berend", "vv2_isokin"])
+def test_vverlet2(name):
+    p, d = load(name)
+    v, f, mv2el, mv2ion, vdf = cpu.vverlet2(p, int(d["thermo"]), float(d["dt"]), d["in_v"], d["in_f"])
+    np.testing.assert_array_equal(v, d["out_v"])
+    assert mv2el == d["out_mv2el"] and mv2ion == d["out_mv2ion"]
+    np.testing.assert_array_equal(vdf, d["out_vdf"])
+
+
+def test_isokinetic_rescale_exact():
+    """ISOKIN: after the rescale sum m v^2 = 3 N_s T0 per species (vverlet.f90:331-332)."""
+    p, d = load("vv2_isokin")
+    _, _, mv2el, mv2ion, _ = cpu.vverlet2(p, 2, float(d["dt"]), d["in_v"], d["in_f"])
+    assert mv2el == pytest.approx(3 * p.ne * p.Te, rel=1e-14)
+    assert mv2ion == pytest.approx(3 * p.ni * p.Ti, rel=1e-14)
+
+
+def test_stoch2_closed_form():
+    """VVerletStoch2 with supplied unit noise eta (vverlet.f90:92-97)."""
+    p, d = load("vv1_stoch")
+    dt = float(d["dt"])
+    eta = np.random.Generator(np.random.PCG64(5)).standard_normal(d["in_v"].shape)
+    v, f, mv2el, mv2ion, vdf = cpu.vverlet2(p, 3, dt, d["in_v"], d["in_f"], eta)
+    xm = np.where(np.arange(p.n) < p.ne, p.xm[0], p.xm[1])[:, None]
+    T = np.where(np.arange(p.n) < p.ne, p.Te, p.Ti)[:, None]
+    beta = np.sqrt(2 * p.fric * T / dt)
+    fr = d["in_f"] + eta * beta
+    vr = (d["in_v"] + 0.5 * dt * fr / xm) / (1 + 0.5 * p.fric * dt / xm)
+    np.testing.assert_allclose(f, fr, rtol=1e-15)
+    np.testing.assert_allclose(v, vr, rtol=1e-14)
+    mv2 = (v ** 2).sum(1) * xm[:, 0]
+    dv = np.where(np.arange(p.n) < p.ne, p.dv[0], p.dv[1])
+    assert vdf.sum() == np.sum((mv2 / 3.0 / dv).astype(int) + 1 <= NBIN)
+
+
+@pytest.mark.parametrize("name", ["traj_nve", "traj_isokin"])
+def test_trajectory(name):
+    """main.f90:87-153 without the reciprocal part, step by step against the reference."""
+    p, d = load(name)
+    p = p.replace(recip=0)
+    x, v, f = (d[k].copy() for k in ("in_x", "in_v", "in_f"))
+    rdf = np.zeros(3 * NBIN)
+    vdf = np.zeros(2 * NBIN)
+    tr = d["out_trace"]
+    for s in range(int(d["nsteps"])):
+        r = cpu.step(p, int(d["thermo"]), float(d["dt"]), x, v, f, rdf, vdf)
+        got = np.array([r["Ur"], r["Epot"], r["mv2el"], r["mv2ion"]])
+        np.testing.assert_array_equal(got, tr[s])
+    np.testing.assert_array_equal(x, d["out_x"])
+    np.testing.assert_array_equal(v, d["out_v"])
+    np.testing.assert_array_equal(f, d["out_f"])
+    np.testing.assert_array_equal(rdf.reshape(3, NBIN), d["out_rdf"])
+    np.testing.assert_array_equal(vdf.reshape(2, NBIN), d["out_vdf"])

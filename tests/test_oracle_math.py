@@ -1,0 +1,165 @@
+"""Pins of the oracle's reciprocal part by mathematics.
+
+Force_SPME4 needs FFTE's ZFFT3D, which neither the reference nor the image
+ships, so the SPME restatement is pinned by identities a wrong index, sign or
+dropped factor would break: the cardinal B-spline closed form, partition of
+unity, the spline modulus, convergence to the direct Ewald k-sum, the
+Madelung constant of rock salt, and forces as minus the energy gradient.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import cpu
+from tcp_allpair_amd.inputs import make_plasma
+from tcp_allpair_amd.params import EPS, RECIP_EWALD, RECIP_SPME
+
+
+def cardinal_bspline(n, u):
+    """M_n(u) = 1/(n-1)! sum_k (-1)^k C(n,k) (u-k)_+^(n-1): the closed form, not the
+    recursion, evaluated in exact rational arithmetic (u is taken exactly)."""
+    u = Fraction(u)
+    s = Fraction(0)
+    for k in range(n + 1):
+        if u - k > 0:
+            s += (-1) ** k * math.comb(n, k) * (u - k) ** (n - 1)
+    return float(s / math.factorial(n - 1))
+
+
+@pytest.mark.parametrize("order", [4, 6, 8])
+def test_bspline_closed_form_and_unity(order):
+    rng = np.random.default_rng(order)
+    for u in rng.random(20) * 50:
+        w, dw = cpu.bspline(order, u)
+        f = u - math.floor(u)
+        ref = [cardinal_bspline(order, Fraction(u) - math.floor(u) + order - 1 - i)
+               for i in range(order)]
+        # atol: the reference's order-4 cubics (force.f90:706-738) cancel near the support ends
+        np.testing.assert_allclose(w, ref, rtol=1e-12, atol=1e-14)
+        # M_n'(t) = M_(n-1)(t) - M_(n-1)(t-1), both from the closed form
+        fx = Fraction(u) - math.floor(u)
+        dref = [cardinal_bspline(order - 1, fx + order - 1 - i) -
+                cardinal_bspline(order - 1, fx + order - 2 - i) for i in range(order)]
+        np.testing.assert_allclose(dw, dref, rtol=1e-10, atol=1e-12)
+        assert abs(w.sum() - 1.0) < 1e-14
+        assert abs(dw.sum()) < 1e-13
+        # first moment: the centred spline reproduces linear functions, sum w_i (k_i - u) = 0
+        base = math.floor(u) - order // 2 + 1
+        mom = sum(w[i] * (base + i - u) for i in range(order))
+        assert abs(mom) < 1e-12
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_bspline_modulus(order):
+    K = 24
+    b = cpu.bspline_modulus(order, K)
+    mk = [cardinal_bspline(order, k + 1) for k in range(order - 1)]
+    for m in range(K):
+        s = sum(mk[k] * np.exp(2j * np.pi * m * k / K) for k in range(order - 1))
+        assert b[m] == pytest.approx(1.0 / abs(s) ** 2, rel=1e-12)
+    if order == 4:
+        np.testing.assert_allclose(b, 36.0 / (4 + 2 * np.cos(2 * np.pi * np.arange(K) / K)) ** 2,
+                                   rtol=1e-15)
+
+
+def small_system(seed=3, ne=30, ni=30):
+    p, x, v = make_plasma(ne, ni, seed=seed)
+    return p, x
+
+
+def test_spme_converges_to_ewald_kspace():
+    p, x = small_system()
+    pk = p.replace(recip=RECIP_EWALD, kmax=14)
+    fk, uk = cpu.ewald_kspace(pk, x)
+    errs = []
+    for K, order in [(16, 4), (32, 4), (32, 6), (48, 8)]:
+        fs, us = cpu.spme(p.replace(K=(K, K, K), order=order, recip=RECIP_SPME), x)
+        errs.append((abs(us - uk) / abs(uk), np.sqrt(((fs - fk) ** 2).sum() / (fk ** 2).sum())))
+    assert errs[-1][0] < 1e-9 and errs[-1][1] < 1e-7
+    assert errs[1][0] < errs[0][0] and errs[2][0] < errs[1][0]
+
+
+def test_spme_anisotropic_grid_and_box():
+    p, x = small_system(seed=4, ne=25, ni=35)
+    L = p.box[0]
+    p = p.replace(box=(L, 1.3 * L, 0.9 * L), rc=0.45 * L)
+    x = x * np.array([1.0, 1.3, 0.9])
+    fk, uk = cpu.ewald_kspace(p.replace(kmax=16), x)
+    fs, us = cpu.spme(p.replace(K=(40, 48, 36), order=8), x)
+    assert abs(us - uk) / abs(uk) < 1e-8
+    assert np.sqrt(((fs - fk) ** 2).sum() / (fk ** 2).sum()) < 1e-6
+
+
+def fd_grad(fun, x, i, k, h):
+    xp, xm_ = x.copy(), x.copy()
+    xp[i, k] += h
+    xm_[i, k] -= h
+    return (fun(xp) - fun(xm_)) / (2 * h)
+
+
+@pytest.mark.parametrize("which", ["spme", "kspace", "real"])
+def test_force_is_minus_energy_gradient(which):
+    p, x = small_system(seed=9, ne=12, ni=12)
+    if which == "spme":
+        q = p.replace(K=(20, 20, 20), order=6)
+        fun = lambda y: cpu.spme(q, y)[1]
+        f = cpu.spme(q, x)[0]
+    elif which == "kspace":
+        q = p.replace(kmax=8)
+        fun = lambda y: cpu.ewald_kspace(q, y)[1]
+        f = cpu.ewald_kspace(q, x)[0]
+    else:
+        q = p
+        fun = lambda y: cpu.force_direct_self(q, y)[1] * q.eps
+        f = cpu.force_direct_self(q, x)[0]
+    for i, k in [(0, 0), (5, 1), (13, 2), (20, 0)]:
+        g = fd_grad(fun, x, i, k, 1e-6)
+        assert f[i, k] == pytest.approx(-g, rel=1e-6, abs=1e-6 * np.abs(f).max())
+
+
+def rock_salt(ncell, r0):
+    """ncell^3 conventional cells of NaCl with nearest-neighbour distance r0."""
+    a = 2 * r0
+    pts = []
+    for i in range(2 * ncell):
+        for j in range(2 * ncell):
+            for k in range(2 * ncell):
+                pts.append(((i + j + k) % 2, (i, j, k)))
+    neg = np.array([p[1] for p in pts if p[0] == 0], dtype=float) * r0
+    pos = np.array([p[1] for p in pts if p[0] == 1], dtype=float) * r0
+    L = ncell * a
+    x = np.concatenate([neg, pos]) - L / 2 + 0.25 * r0
+    return L, x, len(neg), len(pos)
+
+
+@pytest.mark.parametrize("recip", ["kspace", "spme"])
+def test_madelung_rock_salt(recip):
+    """E/ion pair = -1.747565 eps/r0 for charges -1/+1 with the e-i term switched off (lambda -> 0)."""
+    r0 = 1.0
+    L, x, ne, ni = rock_salt(2, r0)
+    p, _, _ = make_plasma(ne, ni)
+    # alpha = 5/rc (erfc(5) ~ 1.5e-12) so that the real-space truncation of the
+    # reference's alpha = 3.1/rc (erfc(3.1) ~ 1.2e-5) does not hide a 1e-6 check
+    p = p.replace(box=(L, L, L), rc=L / 2, rs=1e-3, dx=(L / 1000,) * 3, K=(32, 32, 32), order=8,
+                  kmax=16, alpha=10.0 / L)
+    a, rc2, uo = cpu.ewald_self(p)
+    _, ur, _ = cpu.force_allpair(p, x)
+    fr, um = (cpu.ewald_kspace if recip == "kspace" else cpu.spme)(p, x)
+    e = (ur + uo) * p.eps + um
+    madelung = -e / (ne * p.eps / r0)
+    assert madelung == pytest.approx(1.747564594633, rel=1e-7)
+    # every ion sits at an inversion centre: total force vanishes
+    f_real = cpu.force_allpair(p, x)[0]
+    assert np.abs(f_real + fr).max() < 1e-6
+
+
+def test_ewald_self_closed_form():
+    p, _ = small_system()
+    a, rc2, uo = cpu.ewald_self(p)
+    assert a == 3.1 / p.rc and rc2 == p.rc * p.rc
+    assert uo == pytest.approx(-(p.ne * p.q[0] ** 2 + p.ni * p.q[1] ** 2) * a / math.sqrt(math.pi),
+                               rel=1e-15)
+    with pytest.raises(RuntimeError):
+        cpu.ewald_self(p.replace(rc=0.51 * p.box[0]))
