@@ -1,0 +1,134 @@
+/*
+ * tcp_gpu.h -- C ABI of the MI355X (gfx950) force-and-integrate path of the
+ * TCP_allpair two-component plasma.
+ *
+ * Every entry point replaces one external Fortran subroutine that main.f90 and
+ * init.f90 call (file:line of the reference given per function), so an
+ * ISO_C_BINDING shim with the same subroutine names can bind them.  Plain
+ * pointers and sizes only.  All functions return 0 on success and a negative
+ * code on error; tcp_last_error() returns the message.  Calls are synchronous
+ * with respect to the host (results are in host memory on return).  A context
+ * owns all device memory for its lifetime, is used by one host thread at a
+ * time and runs on its own HIP stream.
+ *
+ * Host arrays are AoS double[3*N] (x,y,z per particle) with the electrons first
+ * and the ions after them, the order of qel(:) followed by qion(:)
+ * (main.f90:358-378).  Histograms are REAL8 arrays laid out like
+ * NS%RDF_ee/_ei/_ii and NS%VDF_e/_i (data.f90:99-101), 501 bins each.
+ */
+#ifndef TCP_GPU_H
+#define TCP_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCP_NBIN 501 /* Ndist+1, data.f90:16 */
+
+/* arithmetic of the pair kernel */
+enum { TCP_FP64 = 0, TCP_MIXED = 1 };
+/* param%thermo (main.f90:88-153) */
+enum { TCP_NOTEMP = 0, TCP_BEREND = 1, TCP_ISOKIN = 2, TCP_STOCH = 3 };
+/* reciprocal-space method */
+enum { TCP_RECIP_NONE = 0, TCP_RECIP_SPME = 1, TCP_RECIP_EWALD = 2 };
+
+/* error codes */
+enum {
+    TCP_OK = 0,
+    TCP_ERR_ARG = -1,      /* invalid argument or size */
+    TCP_ERR_CUTOFF = -2,   /* rc > min(box)/2, " *** error at cut-off radius *** " (force.f90:19-22) */
+    TCP_ERR_HIP = -3,      /* a HIP runtime call failed */
+    TCP_ERR_UNSUPPORTED = -4
+};
+
+/* Parameters of one plasma: TYPE(PM) (data.f90:68-74) plus the NM counts and
+ * bin widths the path reads.  Same layout as the oracle's tcpo_params. */
+typedef struct tcp_params {
+    int32_t ne, ni;          /* NS%NTel, NS%NTion */
+    int32_t K[3];            /* param%K: SPME grid (powers of two up to 512) */
+    int32_t order;           /* B-spline order, even, 4..12 (4 = Force_SPME4) */
+    int32_t kmax;            /* direct Ewald k-sum extent for TCP_RECIP_EWALD */
+    int32_t recip;           /* TCP_RECIP_* */
+    int32_t pad_;
+    double box[3];           /* param%box (A) */
+    double q[2];             /* param%q (e) */
+    double xm[2];            /* param%xm (amu) */
+    double rc;               /* param%rc (A) */
+    double rs;               /* param%rs: lambda of the electron-ion term (A) */
+    double alpha;            /* Ewald splitting; <= 0 selects 3.1/rc as force.f90:23 */
+    double eps;              /* e^2/(4 pi eps0), 14.399644154020082 eV*A (data.f90:21) */
+    double dx_ee, dx_ei, dx_ii; /* NS%dx_*: RDF bin widths */
+    double dv_e, dv_i;       /* NS%dv_*: VDF bin widths (main.f90:286-287) */
+} tcp_params;
+
+typedef struct tcp_ctx tcp_ctx;
+
+/* Library setup: a context for one plasma on the current HIP device.  No
+ * counterpart in the reference (its state lives in qel/qion, main.f90:335). */
+int tcp_create(tcp_ctx** out, const tcp_params* p, int precision);
+int tcp_destroy(tcp_ctx* ctx);
+/* Text of the last error of ctx (or of the last failed tcp_create if ctx is NULL). */
+const char* tcp_last_error(const tcp_ctx* ctx);
+
+/* EWALD_SELF (force.f90:4-49): checks rc <= min(box)/2, returns alpha
+ * (3.1/rc unless set), rc2 and the self energy Uo (no eps, as sys%Uo).  If psi
+ * is not NULL it receives PSI(K1,K2,K3) in Fortran order with PSI(1,1,1) = 0. */
+int tcp_ewald_self(tcp_ctx* ctx, double* alpha, double* rc2, double* Uo, double* psi);
+
+/* Copies of qel/qion%xx, %xv, %ff to / from the device (any pointer may be NULL). */
+int tcp_set_state(tcp_ctx* ctx, const double* x, const double* v, const double* f);
+int tcp_get_state(tcp_ctx* ctx, double* x, double* v, double* f);
+
+/* Force_direct_self (force.f90:52-180) on the whole system: f := real-space
+ * Ewald + electron-ion Deutsch forces; *Ur = sys%Ur (no eps); the RDF counts
+ * (+2 per pair within rc) accumulate on the device, see tcp_get_histograms. */
+int tcp_force_direct(tcp_ctx* ctx, double* Ur);
+
+/* Force_direct_ext (force.f90:183-343): the own particles (the context's
+ * state) against an external block xg of ne_g electrons then ni_g ions: f +=
+ * cross forces, gemit[3*(ne_g+ni_g)] := reaction forces, *Ur := cross energy.
+ * RDF counts accumulate as in the reference. */
+int tcp_force_direct_ext(tcp_ctx* ctx, const double* xg, int ne_g, int ni_g, double* gemit,
+                         double* Ur);
+
+/* Force_SPME4 + SPME_SUM (force.f90:652-969): f += reciprocal forces,
+ * *Um = sys%Um (eps included).  With recip = TCP_RECIP_EWALD the direct
+ * k-space sum replaces the mesh. */
+int tcp_force_recip(tcp_ctx* ctx, double* Um);
+
+/* REMOVE_RIGIDMTN (force.f90:974-1007): f -= mean(f) over all particles. */
+int tcp_remove_rigidmtn(tcp_ctx* ctx);
+
+/* VVerletNotemp1 / VVerletStoch1 (vverlet.f90:132-171 / 9-59): half kick,
+ * drift, DNINT wrap.  fric = param%alpha (STOCH only). */
+int tcp_vverlet1(tcp_ctx* ctx, int thermo, double dt, double fric);
+
+/* VVerletNotemp2 / Berend2 / Isokin2 / Stoch2 (vverlet.f90:174,227,301,62):
+ * half kick and thermostat; *mv2el, *mv2ion = sys%mv2el, sys%mv2ion; VDF
+ * counts accumulate on the device.  STOCH draws its Gaussian noise from a
+ * counter-based generator keyed by (seed, step counter, particle, axis). */
+int tcp_vverlet2(tcp_ctx* ctx, int thermo, double dt, double Te, double Ti, double tau,
+                 double fric, uint64_t seed, double* mv2el, double* mv2ion);
+
+/* Adds the device RDF/VDF counts into rdf[3*501] and vdf[2*501] (either may be
+ * NULL) and zeroes them on the device when reset != 0: the NS%RDF_* / NS%VDF_*
+ * accumulation read by DUMP (result.f90:226-268). */
+int tcp_get_histograms(tcp_ctx* ctx, double* rdf, double* vdf, int reset);
+
+/* nsteps of the MD step of main.f90:87-153 entirely on the device: VVerlet*1,
+ * reciprocal part, Force_direct_self, SPME_SUM, Epot = (Ur+Uo)*eps,
+ * REMOVE_RIGIDMTN, VVerlet*2.  trace (may be NULL) receives 6 doubles per step:
+ * Ur, Uo, Um, Epot, mv2el, mv2ion. */
+int tcp_md_steps(tcp_ctx* ctx, int nsteps, int thermo, double dt, double Te, double Ti,
+                 double tau, double fric, uint64_t seed, double* trace);
+
+/* Device time in milliseconds of the pair kernel's last launch and of the
+ * last tcp_md_steps call (HIP events on the context's stream). */
+int tcp_last_timings(tcp_ctx* ctx, double* pair_ms, double* steps_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,460 @@
+// Smooth particle-mesh Ewald reciprocal part (Force_SPME4, force.f90:652-929,
+// generalised to even B-spline order p) and the direct Ewald k-space sum.
+//
+// Pipeline per force evaluation, all FP64:
+//   spread   one thread per particle; centred order-p B-spline weights and
+//            p^3 hardware FP64 atomic adds into the real grid Q (x fastest,
+//            the Fortran order of Q(ix,iy,iz)).
+//   FFT      batched radix-2 Stockham transforms in LDS along x, y, z; each
+//            block holds 1024 complex points (1024/K lines) so the strided
+//            axes load 16*(1024/K)-byte contiguous runs.  The x pass reads Q
+//            as real input, the last forward (z) pass multiplies by
+//            gb = PSI * |b|^2 and writes the energy partial sum gb*|S|^2 of
+//            each block; the inverse x pass writes only the real part C.
+//   gather   one thread per particle; p^3 reads of C, force from the spline
+//            derivatives (force.f90:868-919).
+// With C the convolution of Q with the real, even kernel IDFT(gb):
+//   Um = eps/(2 pi V) sum_m gb(m) |S(m)|^2,  F_i = -eps/(pi V) sum dQ/dx_i C.
+#include "tcp_internal.h"
+
+namespace tcp {
+
+namespace {
+
+constexpr double kPi = 3.1415926535897931;  // data.f90:23
+
+// Centred cardinal B-spline weights: w[i] belongs to grid point
+// floor(u) - P/2 + 1 + i and equals M_P(f + P - 1 - i), f = frac(u);
+// dw[i] is its derivative with respect to u.  Recursion of the cardinal
+// spline: W_k[j] = M_k(f + j) = ((f + j) W_{k-1}[j] + (k - f - j) W_{k-1}[j-1])/(k-1).
+template <int P>
+__device__ inline void bspline(double f, double* w, double* dw)
+{
+    double W[P];
+    W[0] = f;
+    W[1] = 1.0 - f;
+    // W holds M_k(f + j) in W[j] after step k; start at k = 2: M_2(f) = f, M_2(f+1) = 1-f
+#pragma unroll
+    for (int j = 2; j < P; ++j) W[j] = 0.0;
+#pragma unroll
+    for (int k = 3; k <= P; ++k) {
+        if (k == P) {
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const double prev = j > 0 ? W[j - 1] : 0.0;
+                dw[P - 1 - j] = W[j] - prev;
+            }
+        }
+        const double div = 1.0 / (double)(k - 1);
+#pragma unroll
+        for (int j = k - 1; j >= 0; --j) {
+            const double prev = j > 0 ? W[j - 1] : 0.0;
+            W[j] = div * ((f + j) * W[j] + ((double)k - f - j) * prev);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) w[P - 1 - j] = W[j];
+}
+
+__device__ inline int wrap(int i, int K)
+{
+    i %= K;
+    return i < 0 ? i + K : i;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void spread_kernel(RecipArgs a)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.n) return;
+    const double q = a.q[m >= a.ne];
+    const double pos[3] = {a.x[m], a.y[m], a.z[m]};
+    double w[3][P], dw[3][P];
+    int base[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        // u = K (x + L/2)/L, force.f90:700-702
+        const double u = (double)a.K[d] * (pos[d] + a.box[d] * 0.5) / a.box[d];
+        const double fl = floor(u);
+        bspline<P>(u - fl, w[d], dw[d]);
+        base[d] = (int)fl - P / 2 + 1;
+    }
+    const int K1 = a.K[0], K2 = a.K[1], K3 = a.K[2];
+    for (int k = 0; k < P; ++k) {
+        const int iz = wrap(base[2] + k, K3);
+        const double wz = q * w[2][k];
+        for (int j = 0; j < P; ++j) {
+            const int iy = wrap(base[1] + j, K2);
+            const double wyz = wz * w[1][j];
+            double* row = a.Q + ((size_t)iz * K2 + iy) * K1;
+#pragma unroll
+            for (int i = 0; i < P; ++i) unsafeAtomicAdd(row + wrap(base[0] + i, K1), wyz * w[0][i]);
+        }
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void gather_kernel(RecipArgs a)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.n) return;
+    const double q = a.q[m >= a.ne];
+    const double pos[3] = {a.x[m], a.y[m], a.z[m]};
+    double w[3][P], dw[3][P];
+    int base[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double u = (double)a.K[d] * (pos[d] + a.box[d] * 0.5) / a.box[d];
+        const double fl = floor(u);
+        bspline<P>(u - fl, w[d], dw[d]);
+        base[d] = (int)fl - P / 2 + 1;
+    }
+    const int K1 = a.K[0], K2 = a.K[1], K3 = a.K[2];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int k = 0; k < P; ++k) {
+        const int iz = wrap(base[2] + k, K3);
+        for (int j = 0; j < P; ++j) {
+            const int iy = wrap(base[1] + j, K2);
+            const double* row = a.C + ((size_t)iz * K2 + iy) * K1;
+            double cx = 0.0, c0 = 0.0;
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const double c = row[wrap(base[0] + i, K1)];
+                cx += dw[0][i] * c;
+                c0 += w[0][i] * c;
+            }
+            sx += cx * w[1][j] * w[2][k];
+            sy += c0 * dw[1][j] * w[2][k];
+            sz += c0 * w[1][j] * dw[2][k];
+        }
+    }
+    const double V = a.box[0] * a.box[1] * a.box[2];
+    const double pre = -a.eps * q / (kPi * V);
+    a.frx[m] = pre * sx * ((double)K1 / a.box[0]);
+    a.fry[m] = pre * sy * ((double)K2 / a.box[1]);
+    a.frz[m] = pre * sz * ((double)K3 / a.box[2]);
+}
+
+// -------------------------------------------------------------------- FFT ---
+
+constexpr int kFftTile = 1024;   // complex points per block
+constexpr int kFftThreads = 256;
+
+enum { kInComplex = 0, kInReal = 1 };
+enum { kOutComplex = 0, kOutReal = 2, kOutConvolve = 4 };
+
+struct FftPass {
+    const double2* in;
+    const double* in_real;
+    double2* out;
+    double* out_real;
+    const double2* tw;   // exp(+2 pi i m / K), m < K/2
+    const double* gb;
+    double* epart;
+    int K, logK;
+    int lines_per_block;
+    int inner, outer;    // line l -> base = (l / inner) * outer + (l % inner) * lstride
+    int lstride;
+    int estride;         // distance between points of one line
+    int contiguous;      // 1: a block's lines form one contiguous run (x axis)
+    int sign;            // -1 forward, +1 inverse
+};
+
+__device__ inline size_t line_base(const FftPass& f, int l)
+{
+    return (size_t)(l / f.inner) * f.outer + (size_t)(l % f.inner) * f.lstride;
+}
+
+__device__ inline double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
+{
+    __shared__ double2 buf[2][kFftTile];
+    __shared__ double red[kFftThreads];
+    const int K = f.K, B = f.lines_per_block;
+    const int l0 = blockIdx.x * B;
+    const int npts = B * K;
+    // load: tile index t = lb * K + k
+    for (int e = threadIdx.x; e < npts; e += kFftThreads) {
+        int lb, k;
+        if (f.contiguous) { lb = e / K; k = e % K; }
+        else { k = e / B; lb = e % B; }
+        const size_t g = line_base(f, l0 + lb) + (size_t)k * f.estride;
+        double2 v;
+        if (kMode & kInReal) v = make_double2(f.in_real[g], 0.0);
+        else v = f.in[g];
+        buf[0][lb * K + k] = v;
+    }
+    __syncthreads();
+    int src = 0;
+    const int half = K >> 1;
+    for (int s = 0; s < f.logK; ++s) {
+        const int Ns = 1 << s;
+        for (int b = threadIdx.x; b < (npts >> 1); b += kFftThreads) {
+            const int lb = b / half, j = b % half;
+            const int k = j & (Ns - 1);
+            const double2* A = buf[src] + lb * K;
+            double2* Bo = buf[src ^ 1] + lb * K;
+            const double2 a0 = A[j];
+            double2 w = f.tw[k << (f.logK - 1 - s)];
+            if (f.sign < 0) w.y = -w.y;
+            const double2 a1 = cmul(A[j + half], w);
+            const int d = ((j - k) << 1) + k;
+            Bo[d] = make_double2(a0.x + a1.x, a0.y + a1.y);
+            Bo[d + Ns] = make_double2(a0.x - a1.x, a0.y - a1.y);
+        }
+        src ^= 1;
+        __syncthreads();
+    }
+    double esum = 0.0;
+    for (int e = threadIdx.x; e < npts; e += kFftThreads) {
+        int lb, k;
+        if (f.contiguous) { lb = e / K; k = e % K; }
+        else { k = e / B; lb = e % B; }
+        const size_t g = line_base(f, l0 + lb) + (size_t)k * f.estride;
+        double2 v = buf[src][lb * K + k];
+        if (kMode & kOutConvolve) {
+            const double gbv = f.gb[g];
+            esum += gbv * (v.x * v.x + v.y * v.y);
+            v.x *= gbv;
+            v.y *= gbv;
+        }
+        if (kMode & kOutReal) f.out_real[g] = v.x;
+        else f.out[g] = v;
+    }
+    if (kMode & kOutConvolve) {
+        red[threadIdx.x] = esum;
+        __syncthreads();
+        for (int w = kFftThreads / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) f.epart[blockIdx.x] = red[0];
+    }
+}
+
+int ilog2(int k)
+{
+    int l = 0;
+    while ((1 << l) < k) ++l;
+    return l;
+}
+
+// Lines per block: as many as fit 1024 points, and never more than the run of
+// consecutive lines that share one outer index (so a block's loads stay strided
+// by one line).  All K are powers of two, so every count divides.
+int lines_per_block(const int* K, int axis)
+{
+    const int fit = kFftTile / K[axis];
+    const int run = axis == 0 ? K[1] * K[2] : (axis == 1 ? K[0] : K[0] * K[1]);
+    return fit < run ? fit : run;
+}
+
+FftPass make_pass(const RecipArgs& a, int axis, int sign)
+{
+    FftPass f{};
+    const int K1 = a.K[0], K2 = a.K[1], K3 = a.K[2];
+    f.K = a.K[axis];
+    f.logK = ilog2(f.K);
+    f.lines_per_block = lines_per_block(a.K, axis);
+    f.tw = a.tw[axis];
+    f.sign = sign;
+    f.gb = a.gb;
+    f.epart = a.epart;
+    if (axis == 0) {
+        f.inner = 1; f.outer = K1; f.lstride = 0; f.estride = 1; f.contiguous = 1;
+    } else if (axis == 1) {
+        f.inner = K1; f.outer = K1 * K2; f.lstride = 1; f.estride = K1; f.contiguous = 0;
+    } else {
+        f.inner = K1 * K2; f.outer = 0; f.lstride = 1; f.estride = K1 * K2; f.contiguous = 0;
+    }
+    (void)K3;
+    return f;
+}
+
+int nlines(const int* K, int axis)
+{
+    return K[0] * K[1] * K[2] / K[axis];
+}
+
+template <int kMode>
+void run_pass(const FftPass& f, int lines, hipStream_t s)
+{
+    const int blocks = lines / f.lines_per_block;
+    hipLaunchKernelGGL(fft_pass_kernel<kMode>, dim3(blocks), dim3(kFftThreads), 0, s, f);
+}
+
+__global__ void fill_kernel(double* p, size_t n, double v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ------------------------------------------------------ direct k-space sum ---
+
+// S(m) for every m in the (2 kmax + 1)^3 cube: one block per m, a block-wide sum.
+__global__ __launch_bounds__(256) void ewald_sk_kernel(RecipArgs a, int kmax, double alpha,
+                                                       double* sk)
+{
+    __shared__ double rs[256], is[256];
+    const int nk = 2 * kmax + 1;
+    const int mi = blockIdx.x;
+    const int mx = mi % nk - kmax, my = (mi / nk) % nk - kmax, mz = mi / (nk * nk) - kmax;
+    const double kx = mx / a.box[0], ky = my / a.box[1], kz = mz / a.box[2];
+    double sr = 0.0, si = 0.0;
+    for (int j = threadIdx.x; j < a.n; j += blockDim.x) {
+        const double th = 2.0 * kPi * (kx * a.x[j] + ky * a.y[j] + kz * a.z[j]);
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        const double q = a.q[j >= a.ne];
+        sr += q * cs;
+        si += q * sn;
+    }
+    rs[threadIdx.x] = sr;
+    is[threadIdx.x] = si;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            rs[threadIdx.x] += rs[threadIdx.x + w];
+            is[threadIdx.x] += is[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double m2 = kx * kx + ky * ky + kz * kz;
+        const double G = (mx == 0 && my == 0 && mz == 0)
+                             ? 0.0 : exp(-kPi * kPi * m2 / (alpha * alpha)) / m2;
+        sk[3 * mi + 0] = rs[0];
+        sk[3 * mi + 1] = is[0];
+        sk[3 * mi + 2] = G;
+    }
+}
+
+// Forces from S(m): F_j = (2 eps q_j / V) sum_m G m Im(conj(S) e^{2 pi i m.r_j});
+// energy partials G |S|^2 written by particle-thread 0 of block 0 loop.
+__global__ __launch_bounds__(256) void ewald_force_kernel(RecipArgs a, int kmax, const double* sk)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nk = 2 * kmax + 1, nm = nk * nk * nk;
+    if (j < a.n) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const double xj = a.x[j], yj = a.y[j], zj = a.z[j];
+        for (int mi = 0; mi < nm; ++mi) {
+            const double G = sk[3 * mi + 2];
+            if (G == 0.0) continue;
+            const int mx = mi % nk - kmax, my = (mi / nk) % nk - kmax, mz = mi / (nk * nk) - kmax;
+            const double kx = mx / a.box[0], ky = my / a.box[1], kz = mz / a.box[2];
+            double sn, cs;
+            sincos(2.0 * kPi * (kx * xj + ky * yj + kz * zj), &sn, &cs);
+            // Im(conj(S) e) = Sr sin - Si cos
+            const double im = sk[3 * mi] * sn - sk[3 * mi + 1] * cs;
+            fx += G * im * kx;
+            fy += G * im * ky;
+            fz += G * im * kz;
+        }
+        const double V = a.box[0] * a.box[1] * a.box[2];
+        const double pre = 2.0 * a.eps * a.q[j >= a.ne] / V;
+        a.frx[j] = pre * fx;
+        a.fry[j] = pre * fy;
+        a.frz[j] = pre * fz;
+    }
+    if (blockIdx.x == 0) {
+        __shared__ double red[256];
+        double e = 0.0;
+        for (int mi = threadIdx.x; mi < nm; mi += blockDim.x)
+            e += sk[3 * mi + 2] * (sk[3 * mi] * sk[3 * mi] + sk[3 * mi + 1] * sk[3 * mi + 1]);
+        red[threadIdx.x] = e;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) a.epart[0] = red[0];
+    }
+}
+
+}  // namespace
+
+int fft_epart_blocks(const int* K)
+{
+    return nlines(K, 2) / lines_per_block(K, 2);
+}
+
+void launch_fill(double* p, size_t n, double v, hipStream_t s)
+{
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+
+void launch_spme_spread(const RecipArgs& a, hipStream_t s)
+{
+    const size_t G = (size_t)a.K[0] * a.K[1] * a.K[2];
+    launch_fill(a.Q, G, 0.0, s);
+    dim3 grid((a.n + 255) / 256), blk(256);
+    switch (a.order) {
+        case 4: hipLaunchKernelGGL(spread_kernel<4>, grid, blk, 0, s, a); break;
+        case 6: hipLaunchKernelGGL(spread_kernel<6>, grid, blk, 0, s, a); break;
+        case 8: hipLaunchKernelGGL(spread_kernel<8>, grid, blk, 0, s, a); break;
+        case 10: hipLaunchKernelGGL(spread_kernel<10>, grid, blk, 0, s, a); break;
+        default: hipLaunchKernelGGL(spread_kernel<12>, grid, blk, 0, s, a); break;
+    }
+}
+
+void launch_spme_gather(const RecipArgs& a, hipStream_t s)
+{
+    dim3 grid((a.n + 255) / 256), blk(256);
+    switch (a.order) {
+        case 4: hipLaunchKernelGGL(gather_kernel<4>, grid, blk, 0, s, a); break;
+        case 6: hipLaunchKernelGGL(gather_kernel<6>, grid, blk, 0, s, a); break;
+        case 8: hipLaunchKernelGGL(gather_kernel<8>, grid, blk, 0, s, a); break;
+        case 10: hipLaunchKernelGGL(gather_kernel<10>, grid, blk, 0, s, a); break;
+        default: hipLaunchKernelGGL(gather_kernel<12>, grid, blk, 0, s, a); break;
+    }
+}
+
+void launch_fft3d_forward_convolve(const RecipArgs& a, hipStream_t s)
+{
+    FftPass f0 = make_pass(a, 0, -1);
+    f0.in_real = a.Q;
+    f0.out = a.A;
+    run_pass<kInReal | kOutComplex>(f0, nlines(a.K, 0), s);
+    FftPass f1 = make_pass(a, 1, -1);
+    f1.in = a.A;
+    f1.out = a.A;
+    run_pass<kInComplex | kOutComplex>(f1, nlines(a.K, 1), s);
+    FftPass f2 = make_pass(a, 2, -1);
+    f2.in = a.A;
+    f2.out = a.A;
+    run_pass<kInComplex | kOutConvolve>(f2, nlines(a.K, 2), s);
+}
+
+void launch_fft3d_inverse(const RecipArgs& a, hipStream_t s)
+{
+    FftPass f2 = make_pass(a, 2, +1);
+    f2.in = a.A;
+    f2.out = a.A;
+    run_pass<kInComplex | kOutComplex>(f2, nlines(a.K, 2), s);
+    FftPass f1 = make_pass(a, 1, +1);
+    f1.in = a.A;
+    f1.out = a.A;
+    run_pass<kInComplex | kOutComplex>(f1, nlines(a.K, 1), s);
+    FftPass f0 = make_pass(a, 0, +1);
+    f0.in = a.A;
+    f0.out_real = a.C;
+    run_pass<kInComplex | kOutReal>(f0, nlines(a.K, 0), s);
+}
+
+void launch_ewald_kspace(const RecipArgs& a, int kmax, double alpha, double* sk, hipStream_t s)
+{
+    const int nk = 2 * kmax + 1;
+    hipLaunchKernelGGL(ewald_sk_kernel, dim3(nk * nk * nk), dim3(256), 0, s, a, kmax, alpha, sk);
+    hipLaunchKernelGGL(ewald_force_kernel, dim3((a.n + 255) / 256), dim3(256), 0, s, a, kmax,
+                       (const double*)sk);
+}
+
+}  // namespace tcp

@@ -1,0 +1,770 @@
+// C ABI of libtcp_gpu (include/tcp_gpu.h): context, device memory and the
+// orchestration of the kernels in kernels_pair.hip, kernels_spme.hip and
+// kernels_md.hip.  Device layout: positions, velocities and forces as FP64
+// structure of arrays (electrons first), a uint4 fixed-point position mirror
+// for the mixed pair kernel, per-(chunk, particle) FP64 pair partials, the
+// K1*K2*K3 SPME grids, and a small block of device scalars so that a whole MD
+// step runs without a host round trip.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tcp_internal.h"
+
+using namespace tcp;
+
+namespace {
+
+constexpr double kPi = 3.1415926535897931;      // data.f90:23
+constexpr double kSqrtPi = 1.7724538509055159;  // data.f90:24
+constexpr int kEvPool = 64;                     // timed pair launches kept per call
+
+// device scalar slots
+enum {
+    SC_FSUM = 0,   // 3: sum of forces
+    SC_USUM = 3,   // sum_i u_i
+    SC_UM = 4,     // Um
+    SC_MV2 = 5,    // 2: mv2el, mv2ion
+    SC_LAM = 7,    // 2: thermostat lambdas
+    SC_N = 16
+};
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+struct tcp_ctx {
+    tcp_params p{};
+    int precision = TCP_FP64;
+    int N = 0, Ne = 0;
+    int device = 0;
+    double alpha = 0, rc2 = 0, Uo = 0;
+    hipStream_t stream = nullptr;
+    // state
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    uint4* upos = nullptr;
+    // pair partials
+    int nchunk = 1, jchunk = 1;
+    size_t pf_cap = 0;  // doubles
+    double *pf = nullptr, *pu = nullptr;
+    // reciprocal part
+    double *Q = nullptr, *C = nullptr, *gb = nullptr, *epart = nullptr, *sk = nullptr;
+    double2* A = nullptr;
+    double2* tw[3] = {nullptr, nullptr, nullptr};
+    double *frx = nullptr, *fry = nullptr, *frz = nullptr;
+    int nepart = 0;
+    // reductions and histograms
+    double* part = nullptr;
+    double* sc = nullptr;
+    unsigned long long* rdf = nullptr;
+    unsigned long long* vdf = nullptr;
+    double* trace = nullptr;
+    int trace_cap = 0;
+    // external block (Force_direct_ext)
+    int g_cap = 0;
+    double *gx = nullptr, *gy = nullptr, *gz = nullptr;
+    double *gfx = nullptr, *gfy = nullptr, *gfz = nullptr;
+    uint4* gupos = nullptr;
+    double *gpf = nullptr, *gpu = nullptr;
+    size_t gpf_cap = 0;
+    // timing
+    hipEvent_t ev_pair[2][kEvPool];
+    hipEvent_t ev_all[2];
+    bool events = false;
+    double pair_ms = 0, steps_ms = 0;
+    uint64_t counter = 0;
+    std::vector<void*> allocs;
+    std::string err;
+};
+
+namespace {
+
+int fail(tcp_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIPCK(ctx, call)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(ctx, TCP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCHCK(ctx) HIPCK(ctx, hipGetLastError())
+
+template <typename T>
+int dalloc(tcp_ctx* c, T** p, size_t count)
+{
+    void* q = nullptr;
+    HIPCK(c, hipMalloc(&q, count * sizeof(T) + 64));
+    c->allocs.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+#define TRY(expr)                  \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_) return rc_;       \
+    } while (0)
+
+bool pow2(int k) { return k >= 2 && (k & (k - 1)) == 0; }
+
+MdArgs md_args(tcp_ctx* c)
+{
+    MdArgs m{};
+    m.n = c->N;
+    m.ne = c->Ne;
+    m.x = c->x; m.y = c->y; m.z = c->z;
+    m.vx = c->vx; m.vy = c->vy; m.vz = c->vz;
+    m.fx = c->fx; m.fy = c->fy; m.fz = c->fz;
+    m.upos = c->upos;
+    for (int d = 0; d < 3; ++d) m.box[d] = c->p.box[d];
+    m.xm[0] = c->p.xm[0];
+    m.xm[1] = c->p.xm[1];
+    m.dv[0] = c->p.dv_e;
+    m.dv[1] = c->p.dv_i;
+    m.vdf = c->vdf;
+    return m;
+}
+
+RecipArgs recip_args(tcp_ctx* c)
+{
+    RecipArgs a{};
+    a.n = c->N;
+    a.ne = c->Ne;
+    a.x = c->x; a.y = c->y; a.z = c->z;
+    for (int d = 0; d < 3; ++d) {
+        a.K[d] = c->p.K[d];
+        a.box[d] = c->p.box[d];
+        a.tw[d] = c->tw[d];
+    }
+    a.order = c->p.order;
+    a.q[0] = c->p.q[0];
+    a.q[1] = c->p.q[1];
+    a.eps = c->p.eps;
+    a.Q = c->Q; a.A = c->A; a.C = c->C; a.gb = c->gb;
+    a.epart = c->epart;
+    a.nepart = c->nepart;
+    a.frx = c->frx; a.fry = c->fry; a.frz = c->frz;
+    return a;
+}
+
+void choose_chunks(int n_i, int n_j, int* nchunk, int* jchunk)
+{
+    const int iblocks = (n_i + kPairBlock - 1) / kPairBlock;
+    int nc = (2048 + iblocks - 1) / iblocks;             // aim at >= 2048 blocks
+    const int max_by_size = (n_j + 127) / 128;           // at least 128 j per chunk
+    if (nc > max_by_size) nc = max_by_size;
+    if (nc < 1) nc = 1;
+    if (nc > 64) nc = 64;
+    int jc = (n_j + nc - 1) / nc;
+    nc = (n_j + jc - 1) / jc;
+    *nchunk = nc;
+    *jchunk = jc;
+}
+
+PairArgs pair_args(tcp_ctx* c)
+{
+    PairArgs a{};
+    for (int d = 0; d < 3; ++d) {
+        a.box[d] = c->p.box[d];
+        a.inv_box[d] = 1.0 / c->p.box[d];
+    }
+    a.rc2 = c->rc2;
+    a.alpha = c->alpha;
+    a.rs = c->p.rs;
+    a.eps = c->p.eps;
+    a.q[0] = c->p.q[0];
+    a.q[1] = c->p.q[1];
+    const double dx[3] = {c->p.dx_ee, c->p.dx_ei, c->p.dx_ii};
+    for (int t = 0; t < 3; ++t) {
+        a.dx[t] = dx[t];
+        a.inv_dx[t] = 1.0 / dx[t];
+    }
+    a.rdf = c->rdf;
+    return a;
+}
+
+void launch_pair(tcp_ctx* c, const PairArgs& a, int nchunk)
+{
+    if (c->precision == TCP_MIXED) launch_pair_mixed(a, nchunk, c->stream);
+    else launch_pair_f64(a, nchunk, c->stream);
+}
+
+// Real-space all-pair forces of the whole system into f, block partials into part.
+int direct_all(tcp_ctx* c, int rdf_weight, const double* frx, const double* fry, const double* frz,
+               int ev_slot)
+{
+    PairArgs a = pair_args(c);
+    a.pi = PosSoA{c->x, c->y, c->z};
+    a.pj = a.pi;
+    a.ui = c->upos;
+    a.uj = c->upos;
+    a.n_i = a.n_j = c->N;
+    a.ne_i = a.ne_j = c->Ne;
+    a.jchunk = c->jchunk;
+    a.rdf_weight = rdf_weight;
+    a.pf = c->pf;
+    a.pu = c->pu;
+    if (ev_slot >= 0) HIPCK(c, hipEventRecord(c->ev_pair[0][ev_slot], c->stream));
+    launch_pair(c, a, c->nchunk);
+    LAUNCHCK(c);
+    if (ev_slot >= 0) HIPCK(c, hipEventRecord(c->ev_pair[1][ev_slot], c->stream));
+    launch_combine(md_args(c), c->pf, c->pu, c->nchunk, frx, fry, frz, c->part, false, c->stream);
+    LAUNCHCK(c);
+    launch_reduce_parts(c->part, md_blocks(c->N), 4, c->sc + SC_FSUM, c->stream);
+    LAUNCHCK(c);
+    return 0;
+}
+
+// Reciprocal forces into frx/fry/frz and Um into sc[SC_UM].
+int recip(tcp_ctx* c)
+{
+    RecipArgs a = recip_args(c);
+    const double V = c->p.box[0] * c->p.box[1] * c->p.box[2];
+    if (c->p.recip == TCP_RECIP_SPME) {
+        launch_spme_spread(a, c->stream);
+        LAUNCHCK(c);
+        launch_fft3d_forward_convolve(a, c->stream);
+        LAUNCHCK(c);
+        launch_fft3d_inverse(a, c->stream);
+        LAUNCHCK(c);
+        launch_spme_gather(a, c->stream);
+        LAUNCHCK(c);
+        launch_um(c->epart, c->nepart, c->p.eps / (2.0 * kPi * V), c->sc + SC_UM, c->stream);
+    } else {
+        launch_ewald_kspace(a, c->p.kmax, c->alpha, c->sk, c->stream);
+        LAUNCHCK(c);
+        launch_um(c->epart, 1, c->p.eps / (2.0 * kPi * V), c->sc + SC_UM, c->stream);
+    }
+    LAUNCHCK(c);
+    return 0;
+}
+
+int sync(tcp_ctx* c)
+{
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int get_sc(tcp_ctx* c, int slot, int count, double* out)
+{
+    HIPCK(c, hipMemcpyAsync(out, c->sc + slot, count * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    return sync(c);
+}
+
+int vv2(tcp_ctx* c, int thermo, double dt, double Te, double Ti, double tau, double fric,
+        uint64_t seed, const double* fsum)
+{
+    MdArgs m = md_args(c);
+    const int nb = md_blocks(c->N);
+    launch_vv2_kick(m, thermo, dt, Te, Ti, fric, seed, c->counter++, fsum, c->part, c->stream);
+    LAUNCHCK(c);
+    if (thermo == TCP_BEREND || thermo == TCP_ISOKIN) {
+        launch_reduce_parts(c->part, nb, 2, c->sc + SC_MV2, c->stream);
+        launch_thermo_lambda(c->sc + SC_MV2, thermo, c->Ne, c->N - c->Ne, Te, Ti, dt, tau,
+                             c->sc + SC_LAM, c->stream);
+        launch_vv2_scale(m, c->sc + SC_LAM, c->part, c->stream);
+        LAUNCHCK(c);
+    }
+    launch_reduce_parts(c->part, nb, 2, c->sc + SC_MV2, c->stream);
+    LAUNCHCK(c);
+    return 0;
+}
+
+int ensure_ext(tcp_ctx* c, int ng)
+{
+    if (ng <= c->g_cap) return 0;
+    double** arrs[6] = {&c->gx, &c->gy, &c->gz, &c->gfx, &c->gfy, &c->gfz};
+    for (auto a : arrs) TRY(dalloc(c, a, (size_t)ng));
+    TRY(dalloc(c, &c->gupos, (size_t)ng));
+    c->g_cap = ng;
+    return 0;
+}
+
+void aos_to_soa(const double* a, int n, std::vector<double>& s)
+{
+    s.resize(3 * (size_t)n);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) s[(size_t)k * n + i] = a[3 * (size_t)i + k];
+}
+
+void soa_to_aos(const std::vector<double>& s, int n, double* a)
+{
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) a[3 * (size_t)i + k] = s[(size_t)k * n + i];
+}
+
+int upload3(tcp_ctx* c, const double* aos, double* dx, double* dy, double* dz, int n)
+{
+    std::vector<double> s;
+    aos_to_soa(aos, n, s);
+    double* d[3] = {dx, dy, dz};
+    for (int k = 0; k < 3; ++k)
+        HIPCK(c, hipMemcpyAsync(d[k], s.data() + (size_t)k * n, n * sizeof(double),
+                                hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+int download3(tcp_ctx* c, double* aos, const double* dx, const double* dy, const double* dz, int n)
+{
+    std::vector<double> s(3 * (size_t)n);
+    const double* d[3] = {dx, dy, dz};
+    for (int k = 0; k < 3; ++k)
+        HIPCK(c, hipMemcpyAsync(s.data() + (size_t)k * n, d[k], n * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    soa_to_aos(s, n, aos);
+    return 0;
+}
+
+int setup_recip(tcp_ctx* c)
+{
+    const int* K = c->p.K;
+    const size_t G = (size_t)K[0] * K[1] * K[2];
+    TRY(dalloc(c, &c->frx, (size_t)c->N));
+    TRY(dalloc(c, &c->fry, (size_t)c->N));
+    TRY(dalloc(c, &c->frz, (size_t)c->N));
+    if (c->p.recip == TCP_RECIP_EWALD) {
+        const int nk = 2 * c->p.kmax + 1;
+        TRY(dalloc(c, &c->sk, (size_t)3 * nk * nk * nk));
+        TRY(dalloc(c, &c->epart, (size_t)1));
+        c->nepart = 1;
+        return 0;
+    }
+    TRY(dalloc(c, &c->Q, G));
+    TRY(dalloc(c, &c->C, G));
+    TRY(dalloc(c, &c->gb, G));
+    TRY(dalloc(c, &c->A, G));
+    c->nepart = fft_epart_blocks(K);
+    TRY(dalloc(c, &c->epart, (size_t)c->nepart));
+    // influence function PSI (force.f90:27-42, PSI(1,1,1) = 0) times the spline
+    // modulus |b1 b2 b3|^2 (force.f90:834-839 for order 4), built on the host in FP64
+    std::vector<double> bm[3];
+    const int P = c->p.order;
+    for (int d = 0; d < 3; ++d) {
+        bm[d].resize(K[d]);
+        // M_P(k+1), k = 0..P-2, by the cardinal-spline recursion at f = 0
+        std::vector<double> W(P + 1, 0.0);
+        W[0] = 0.0;  // M_2(0)
+        W[1] = 1.0;  // M_2(1)
+        for (int k = 3; k <= P; ++k)
+            for (int j = k - 1; j >= 0; --j)
+                W[j] = (j * W[j] + (k - j) * (j > 0 ? W[j - 1] : 0.0)) / (k - 1);
+        for (int m = 0; m < K[d]; ++m) {
+            double re = 0.0, im = 0.0;
+            for (int k = 0; k <= P - 2; ++k) {
+                const double th = 2.0 * kPi * (double)m * k / K[d];
+                re += W[k + 1] * std::cos(th);
+                im += W[k + 1] * std::sin(th);
+            }
+            bm[d][m] = 1.0 / (re * re + im * im);
+        }
+    }
+    std::vector<double> h(G);
+    const double a2 = c->alpha * c->alpha, pi2 = kPi * kPi;
+    for (int k = 0; k < K[2]; ++k) {
+        const int iz = (k + K[2] / 2) % K[2] - K[2] / 2;
+        for (int j = 0; j < K[1]; ++j) {
+            const int iy = (j + K[1] / 2) % K[1] - K[1] / 2;
+            for (int i = 0; i < K[0]; ++i) {
+                const int ix = (i + K[0] / 2) % K[0] - K[0] / 2;
+                const size_t g = ((size_t)k * K[1] + j) * K[0] + i;
+                if (ix == 0 && iy == 0 && iz == 0) { h[g] = 0.0; continue; }
+                const double m2 = (double)ix * ix / (c->p.box[0] * c->p.box[0]) +
+                                  (double)iy * iy / (c->p.box[1] * c->p.box[1]) +
+                                  (double)iz * iz / (c->p.box[2] * c->p.box[2]);
+                h[g] = std::exp(-(m2 * pi2 / a2)) / m2 * bm[0][i] * bm[1][j] * bm[2][k];
+            }
+        }
+    }
+    HIPCK(c, hipMemcpy(c->gb, h.data(), G * sizeof(double), hipMemcpyHostToDevice));
+    for (int d = 0; d < 3; ++d) {
+        std::vector<double2> t(K[d] / 2);
+        for (int m = 0; m < K[d] / 2; ++m) {
+            // exp(+2 pi i m/K) from long double to round once
+            const long double th = 2.0L * 3.14159265358979323846264338327950288L * m / K[d];
+            t[m] = make_double2((double)cosl(th), (double)sinl(th));
+        }
+        TRY(dalloc(c, &c->tw[d], t.size()));
+        HIPCK(c, hipMemcpy(c->tw[d], t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tcp_last_error(const tcp_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
+{
+    if (!out || !p) return fail(nullptr, TCP_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->ne < 0 || p->ni < 0 || p->ne + p->ni < 2)
+        return fail(nullptr, TCP_ERR_ARG, "need at least two particles");
+    if (precision != TCP_FP64 && precision != TCP_MIXED)
+        return fail(nullptr, TCP_ERR_ARG, "precision must be TCP_FP64 or TCP_MIXED");
+    for (int d = 0; d < 3; ++d)
+        if (!(p->box[d] > 0)) return fail(nullptr, TCP_ERR_ARG, "box lengths must be positive");
+    if (!(p->rc > 0) || !(p->rs > 0) || !(p->dx_ee > 0) || !(p->dx_ei > 0) || !(p->dx_ii > 0))
+        return fail(nullptr, TCP_ERR_ARG, "rc, rs and the RDF bin widths must be positive");
+    const double rc_cut = std::fmin(std::fmin(p->box[0], p->box[1]), p->box[2]) / 2.0;
+    if (p->rc > rc_cut) return fail(nullptr, TCP_ERR_CUTOFF, " *** error at cut-off radius *** ");
+    if (p->recip == TCP_RECIP_SPME) {
+        for (int d = 0; d < 3; ++d)
+            if (!pow2(p->K[d]) || p->K[d] > 1024 || p->K[d] < p->order)
+                return fail(nullptr, TCP_ERR_UNSUPPORTED,
+                            "SPME grid sizes must be powers of two in [order, 1024]");
+        if (p->order < 4 || p->order > 12 || (p->order & 1))
+            return fail(nullptr, TCP_ERR_UNSUPPORTED, "B-spline order must be 4, 6, 8, 10 or 12");
+    } else if (p->recip == TCP_RECIP_EWALD) {
+        if (p->kmax < 1 || p->kmax > 40) return fail(nullptr, TCP_ERR_ARG, "kmax must be in [1, 40]");
+    } else if (p->recip != TCP_RECIP_NONE) {
+        return fail(nullptr, TCP_ERR_ARG, "unknown reciprocal method");
+    }
+    tcp_ctx* c = new tcp_ctx();
+    c->p = *p;
+    c->precision = precision;
+    c->N = p->ne + p->ni;
+    c->Ne = p->ne;
+    c->alpha = p->alpha > 0 ? p->alpha : 3.1 / p->rc;  // force.f90:23
+    c->rc2 = p->rc * p->rc;
+    c->Uo = -((double)p->ne * (p->q[0] * p->q[0])) - (double)p->ni * (p->q[1] * p->q[1]);
+    c->Uo = c->Uo * c->alpha / kSqrtPi;  // force.f90:44-45
+    auto bail = [&](int rc) {
+        g_create_error = c->err;
+        tcp_destroy(c);
+        return rc;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(c, TCP_ERR_HIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(c, TCP_ERR_HIP, "hipStreamCreate failed"));
+    for (int e = 0; e < kEvPool; ++e)
+        for (int k = 0; k < 2; ++k)
+            if (hipEventCreate(&c->ev_pair[k][e]) != hipSuccess)
+                return bail(fail(c, TCP_ERR_HIP, "hipEventCreate failed"));
+    if (hipEventCreate(&c->ev_all[0]) != hipSuccess || hipEventCreate(&c->ev_all[1]) != hipSuccess)
+        return bail(fail(c, TCP_ERR_HIP, "hipEventCreate failed"));
+    c->events = true;
+    const size_t n = c->N;
+    int rc = 0;
+    double** arrs[9] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->fx, &c->fy, &c->fz};
+    for (auto a : arrs)
+        if ((rc = dalloc(c, a, n))) return bail(rc);
+    if ((rc = dalloc(c, &c->upos, n))) return bail(rc);
+    choose_chunks(c->N, c->N, &c->nchunk, &c->jchunk);
+    c->pf_cap = (size_t)c->nchunk * 3 * n;
+    if ((rc = dalloc(c, &c->pf, c->pf_cap))) return bail(rc);
+    if ((rc = dalloc(c, &c->pu, (size_t)c->nchunk * n))) return bail(rc);
+    if ((rc = dalloc(c, &c->part, (size_t)4 * md_blocks(c->N)))) return bail(rc);
+    if ((rc = dalloc(c, &c->sc, (size_t)SC_N))) return bail(rc);
+    if ((rc = dalloc(c, &c->rdf, (size_t)3 * kHistStride))) return bail(rc);
+    if ((rc = dalloc(c, &c->vdf, (size_t)2 * kHistStride))) return bail(rc);
+    if (p->recip != TCP_RECIP_NONE && (rc = setup_recip(c))) return bail(rc);
+    // zero everything the kernels accumulate into (fill kernels, no memset nodes)
+    if (hipMemsetAsync(c->rdf, 0, 3 * kHistStride * sizeof(unsigned long long), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->vdf, 0, 2 * kHistStride * sizeof(unsigned long long), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->sc, 0, SC_N * sizeof(double), c->stream) != hipSuccess)
+        return bail(fail(c, TCP_ERR_HIP, "hipMemsetAsync failed"));
+    double* zero3[9] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz};
+    for (auto a : zero3) launch_fill(a, n, 0.0, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+        return bail(fail(c, TCP_ERR_HIP, "initialisation failed"));
+    *out = c;
+    return 0;
+}
+
+int tcp_destroy(tcp_ctx* c)
+{
+    if (!c) return 0;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* a : c->allocs) hipFree(a);
+    if (c->events) {
+        for (int e = 0; e < kEvPool; ++e)
+            for (int k = 0; k < 2; ++k) hipEventDestroy(c->ev_pair[k][e]);
+        hipEventDestroy(c->ev_all[0]);
+        hipEventDestroy(c->ev_all[1]);
+    }
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int tcp_ewald_self(tcp_ctx* c, double* alpha, double* rc2, double* Uo, double* psi)
+{
+    if (!c) return TCP_ERR_ARG;
+    if (alpha) *alpha = c->alpha;
+    if (rc2) *rc2 = c->rc2;
+    if (Uo) *Uo = c->Uo;
+    if (psi) {
+        const int* K = c->p.K;
+        const double pi2 = kPi * kPi, a2 = c->alpha * c->alpha;
+        for (int k = 0; k < K[2]; ++k) {
+            const int iz = (k + K[2] / 2) % K[2] - K[2] / 2;
+            for (int j = 0; j < K[1]; ++j) {
+                const int iy = (j + K[1] / 2) % K[1] - K[1] / 2;
+                for (int i = 0; i < K[0]; ++i) {
+                    const int ix = (i + K[0] / 2) % K[0] - K[0] / 2;
+                    const size_t g = ((size_t)k * K[1] + j) * K[0] + i;
+                    if (ix == 0 && iy == 0 && iz == 0) { psi[g] = 0.0; continue; }
+                    const double m2 = (double)ix * (double)ix / (c->p.box[0] * c->p.box[0]) +
+                                      (double)iy * (double)iy / (c->p.box[1] * c->p.box[1]) +
+                                      (double)iz * (double)iz / (c->p.box[2] * c->p.box[2]);
+                    psi[g] = std::exp(-(m2 * pi2 / a2)) / m2;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+int tcp_set_state(tcp_ctx* c, const double* x, const double* v, const double* f)
+{
+    if (!c) return TCP_ERR_ARG;
+    if (x) TRY(upload3(c, x, c->x, c->y, c->z, c->N));
+    if (v) TRY(upload3(c, v, c->vx, c->vy, c->vz, c->N));
+    if (f) TRY(upload3(c, f, c->fx, c->fy, c->fz, c->N));
+    if (x) {
+        launch_make_upos(md_args(c), c->stream);
+        LAUNCHCK(c);
+        TRY(sync(c));
+    }
+    return 0;
+}
+
+int tcp_get_state(tcp_ctx* c, double* x, double* v, double* f)
+{
+    if (!c) return TCP_ERR_ARG;
+    if (x) TRY(download3(c, x, c->x, c->y, c->z, c->N));
+    if (v) TRY(download3(c, v, c->vx, c->vy, c->vz, c->N));
+    if (f) TRY(download3(c, f, c->fx, c->fy, c->fz, c->N));
+    return 0;
+}
+
+int tcp_force_direct(tcp_ctx* c, double* Ur)
+{
+    if (!c) return TCP_ERR_ARG;
+    TRY(direct_all(c, 1, nullptr, nullptr, nullptr, 0));
+    double u;
+    TRY(get_sc(c, SC_USUM, 1, &u));
+    float ms = 0;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev_pair[0][0], c->ev_pair[1][0]));
+    c->pair_ms = ms;
+    if (Ur) *Ur = 0.5 * u;
+    return 0;
+}
+
+int tcp_force_direct_ext(tcp_ctx* c, const double* xg, int ne_g, int ni_g, double* gemit,
+                         double* Ur)
+{
+    if (!c || !xg || ne_g < 0 || ni_g < 0) return TCP_ERR_ARG;
+    const int ng = ne_g + ni_g;
+    if (ng == 0) {
+        if (Ur) *Ur = 0.0;
+        return 0;
+    }
+    TRY(ensure_ext(c, ng));
+    TRY(upload3(c, xg, c->gx, c->gy, c->gz, ng));
+    MdArgs mg = md_args(c);
+    mg.n = ng;
+    mg.ne = ne_g;
+    mg.x = c->gx; mg.y = c->gy; mg.z = c->gz;
+    mg.fx = c->gfx; mg.fy = c->gfy; mg.fz = c->gfz;
+    mg.upos = c->gupos;
+    launch_make_upos(mg, c->stream);
+    // own (i) against external (j): f_own += cross forces, RDF +2 per pair
+    int nc, jc;
+    choose_chunks(c->N, ng, &nc, &jc);
+    const size_t need = (size_t)nc * 3 * c->N;
+    if (need > c->pf_cap) {
+        TRY(dalloc(c, &c->pf, need));
+        TRY(dalloc(c, &c->pu, (size_t)nc * c->N));
+        c->pf_cap = need;
+    }
+    PairArgs a = pair_args(c);
+    a.pi = PosSoA{c->x, c->y, c->z};
+    a.pj = PosSoA{c->gx, c->gy, c->gz};
+    a.ui = c->upos;
+    a.uj = c->gupos;
+    a.n_i = c->N; a.ne_i = c->Ne;
+    a.n_j = ng; a.ne_j = ne_g;
+    a.jchunk = jc;
+    a.rdf_weight = 2;
+    a.pf = c->pf;
+    a.pu = c->pu;
+    launch_pair(c, a, nc);
+    LAUNCHCK(c);
+    launch_combine(md_args(c), c->pf, c->pu, nc, nullptr, nullptr, nullptr, c->part, true, c->stream);
+    launch_reduce_parts(c->part, md_blocks(c->N), 4, c->sc + SC_FSUM, c->stream);
+    LAUNCHCK(c);
+    double u;
+    TRY(get_sc(c, SC_USUM, 1, &u));
+    // reaction forces on the external block (gemit, force.f90:237-337)
+    int nc2, jc2;
+    choose_chunks(ng, c->N, &nc2, &jc2);
+    const size_t need2 = (size_t)nc2 * 3 * ng;
+    if (need2 > c->gpf_cap) {
+        TRY(dalloc(c, &c->gpf, need2));
+        TRY(dalloc(c, &c->gpu, (size_t)nc2 * ng));
+        c->gpf_cap = need2;
+    }
+    PairArgs b = a;
+    b.pi = a.pj; b.pj = a.pi;
+    b.ui = a.uj; b.uj = a.ui;
+    b.n_i = ng; b.ne_i = ne_g;
+    b.n_j = c->N; b.ne_j = c->Ne;
+    b.jchunk = jc2;
+    b.rdf_weight = 0;
+    b.pf = c->gpf;
+    b.pu = c->gpu;
+    launch_pair(c, b, nc2);
+    LAUNCHCK(c);
+    // part is large enough for md_blocks(max(N, ng)) only if ng <= N; use a dedicated reduction
+    if (md_blocks(ng) > md_blocks(c->N)) {
+        TRY(dalloc(c, &c->part, (size_t)4 * md_blocks(ng)));
+    }
+    launch_combine(mg, c->gpf, c->gpu, nc2, nullptr, nullptr, nullptr, c->part, false, c->stream);
+    LAUNCHCK(c);
+    if (gemit) TRY(download3(c, gemit, c->gfx, c->gfy, c->gfz, ng));
+    TRY(sync(c));
+    if (Ur) *Ur = u;
+    return 0;
+}
+
+int tcp_force_recip(tcp_ctx* c, double* Um)
+{
+    if (!c) return TCP_ERR_ARG;
+    if (c->p.recip == TCP_RECIP_NONE) {
+        if (Um) *Um = 0.0;
+        return 0;
+    }
+    TRY(recip(c));
+    launch_add_forces(md_args(c), c->frx, c->fry, c->frz, c->stream);
+    LAUNCHCK(c);
+    double um;
+    TRY(get_sc(c, SC_UM, 1, &um));
+    if (Um) *Um = um;
+    return 0;
+}
+
+int tcp_remove_rigidmtn(tcp_ctx* c)
+{
+    if (!c) return TCP_ERR_ARG;
+    MdArgs m = md_args(c);
+    launch_force_sum(m, c->part, c->stream);
+    launch_reduce_parts(c->part, md_blocks(c->N), 3, c->sc + SC_FSUM, c->stream);
+    launch_sub_mean(m, c->sc + SC_FSUM, c->stream);
+    LAUNCHCK(c);
+    return sync(c);
+}
+
+int tcp_vverlet1(tcp_ctx* c, int thermo, double dt, double fric)
+{
+    if (!c || thermo < TCP_NOTEMP || thermo > TCP_STOCH) return TCP_ERR_ARG;
+    launch_vv1(md_args(c), thermo, dt, fric, c->stream);
+    LAUNCHCK(c);
+    return sync(c);
+}
+
+int tcp_vverlet2(tcp_ctx* c, int thermo, double dt, double Te, double Ti, double tau, double fric,
+                 uint64_t seed, double* mv2el, double* mv2ion)
+{
+    if (!c || thermo < TCP_NOTEMP || thermo > TCP_STOCH) return TCP_ERR_ARG;
+    TRY(vv2(c, thermo, dt, Te, Ti, tau, fric, seed, nullptr));
+    double mv2[2];
+    TRY(get_sc(c, SC_MV2, 2, mv2));
+    if (mv2el) *mv2el = mv2[0];
+    if (mv2ion) *mv2ion = mv2[1];
+    return 0;
+}
+
+int tcp_get_histograms(tcp_ctx* c, double* rdf, double* vdf, int reset)
+{
+    if (!c) return TCP_ERR_ARG;
+    std::vector<unsigned long long> h(5 * kHistStride);
+    HIPCK(c, hipMemcpyAsync(h.data(), c->rdf, 3 * kHistStride * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(h.data() + 3 * kHistStride, c->vdf,
+                            2 * kHistStride * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            c->stream));
+    TRY(sync(c));
+    for (int t = 0; t < 3 && rdf; ++t)
+        for (int b = 0; b < kNbin; ++b) rdf[t * kNbin + b] += (double)h[t * kHistStride + b];
+    for (int t = 0; t < 2 && vdf; ++t)
+        for (int b = 0; b < kNbin; ++b) vdf[t * kNbin + b] += (double)h[(3 + t) * kHistStride + b];
+    if (reset) {
+        HIPCK(c, hipMemsetAsync(c->rdf, 0, 3 * kHistStride * sizeof(unsigned long long), c->stream));
+        HIPCK(c, hipMemsetAsync(c->vdf, 0, 2 * kHistStride * sizeof(unsigned long long), c->stream));
+        TRY(sync(c));
+    }
+    return 0;
+}
+
+int tcp_md_steps(tcp_ctx* c, int nsteps, int thermo, double dt, double Te, double Ti, double tau,
+                 double fric, uint64_t seed, double* trace)
+{
+    if (!c || nsteps < 0 || thermo < TCP_NOTEMP || thermo > TCP_STOCH) return TCP_ERR_ARG;
+    if (nsteps == 0) return 0;
+    if (trace && nsteps > c->trace_cap) {
+        TRY(dalloc(c, &c->trace, (size_t)6 * nsteps));
+        c->trace_cap = nsteps;
+    }
+    const bool rec = c->p.recip != TCP_RECIP_NONE;
+    MdArgs m = md_args(c);
+    HIPCK(c, hipEventRecord(c->ev_all[0], c->stream));
+    const int first_timed = nsteps > kEvPool ? nsteps - kEvPool : 0;
+    for (int s = 0; s < nsteps; ++s) {
+        launch_vv1(m, thermo, dt, fric, c->stream);  // main.f90:87-95
+        LAUNCHCK(c);
+        if (rec) TRY(recip(c));                      // main.f90:100
+        const int slot = s >= first_timed ? s - first_timed : -1;
+        TRY(direct_all(c, 1, rec ? c->frx : nullptr, rec ? c->fry : nullptr,
+                       rec ? c->frz : nullptr, slot));  // main.f90:106, 136
+        TRY(vv2(c, thermo, dt, Te, Ti, tau, fric, seed, c->sc + SC_FSUM));  // main.f90:141-153
+        if (trace) {
+            launch_trace(c->trace + 6 * (size_t)s, c->sc + SC_USUM, 0.5, c->Uo,
+                         rec ? c->sc + SC_UM : nullptr, c->sc + SC_MV2, c->p.eps, c->stream);
+            LAUNCHCK(c);
+        }
+    }
+    HIPCK(c, hipEventRecord(c->ev_all[1], c->stream));
+    TRY(sync(c));
+    float ms = 0;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev_all[0], c->ev_all[1]));
+    c->steps_ms = ms;
+    double tot = 0;
+    const int ntimed = nsteps - first_timed;
+    for (int e = 0; e < ntimed; ++e) {
+        HIPCK(c, hipEventElapsedTime(&ms, c->ev_pair[0][e], c->ev_pair[1][e]));
+        tot += ms;
+    }
+    c->pair_ms = tot / ntimed;
+    if (trace) {
+        HIPCK(c, hipMemcpy(trace, c->trace, 6 * (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int tcp_last_timings(tcp_ctx* c, double* pair_ms, double* steps_ms)
+{
+    if (!c) return TCP_ERR_ARG;
+    if (pair_ms) *pair_ms = c->pair_ms;
+    if (steps_ms) *steps_ms = c->steps_ms;
+    return 0;
+}
+
+}  // extern "C"

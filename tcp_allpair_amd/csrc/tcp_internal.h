@@ -1,0 +1,119 @@
+// Internal declarations shared by the HIP translation units of libtcp_gpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tcp_gpu.h"
+
+namespace tcp {
+
+constexpr int kNbin = TCP_NBIN;        // Ndist+1
+constexpr int kHistStride = 512;       // padded bins per histogram in device memory
+constexpr int kPairBlock = 256;        // threads (i particles) per pair-kernel block
+constexpr int kRedBlock = 256;         // threads per block of the O(N) kernels
+
+// Particle positions as structure of arrays, electrons first.
+struct PosSoA {
+    const double* x;
+    const double* y;
+    const double* z;
+};
+
+// Arguments of the all-pair real-space kernels.  The i-set and the j-set are
+// both "electrons first"; i == j pairs are skipped through r2 > 0.
+struct PairArgs {
+    PosSoA pi;                 // i-set (FP64 path)
+    PosSoA pj;                 // j-set (FP64 path)
+    const uint4* ui;           // i-set fixed-point fractional coordinates (mixed path)
+    const uint4* uj;           // j-set fixed-point fractional coordinates (mixed path)
+    int n_i, ne_i;             // i-set size and electron count
+    int n_j, ne_j;             // j-set size and electron count
+    int jchunk;                // j indices per blockIdx.y
+    double box[3], inv_box[3];
+    double rc2, alpha, rs, eps;
+    double q[2];
+    double inv_dx[3];          // RDF bin widths ee, ei, ii (reciprocal, mixed path)
+    double dx[3];              // RDF bin widths (FP64 path divides as the reference)
+    unsigned rdf_weight;       // 0: no RDF, 1: ordered pairs, 2: unordered pairs
+    unsigned long long* rdf;   // [3][kHistStride] device counts
+    double* pf;                // [nchunk][3][n_i] partial forces (eps included)
+    double* pu;                // [nchunk][n_i] partial energies sum_j q q (erfc - y)/r
+};
+
+void launch_pair_f64(const PairArgs& a, int nchunk, hipStream_t s);
+void launch_pair_mixed(const PairArgs& a, int nchunk, hipStream_t s);
+
+// SPME / Ewald reciprocal part
+struct RecipArgs {
+    int n, ne;
+    const double* x;
+    const double* y;
+    const double* z;
+    int K[3];
+    int order;
+    double box[3];
+    double q[2];
+    double eps;
+    double* Q;                 // real charge grid K1*K2*K3 (x fastest)
+    double2* A;                // complex work grid
+    double* C;                 // real convolved grid
+    const double* gb;          // PSI * B modulus, K1*K2*K3
+    const double2* tw[3];      // twiddles exp(+2 pi i m/K_d), m < K_d/2, per axis
+    double* epart;             // energy partials (one per block of the convolution)
+    int nepart;
+    double* frx;               // reciprocal forces (SoA)
+    double* fry;
+    double* frz;
+};
+
+void launch_spme_spread(const RecipArgs& a, hipStream_t s);
+void launch_fft3d_forward_convolve(const RecipArgs& a, hipStream_t s);
+void launch_fft3d_inverse(const RecipArgs& a, hipStream_t s);
+void launch_spme_gather(const RecipArgs& a, hipStream_t s);
+void launch_fill(double* p, size_t n, double v, hipStream_t s);
+int fft_epart_blocks(const int* K);
+
+// Direct Ewald k-space sum (small systems, reference-independent extension)
+void launch_ewald_kspace(const RecipArgs& a, int kmax, double alpha, double* sk_work,
+                         hipStream_t s);  // energy sum into epart[0]
+
+// Integrator / reductions
+struct MdArgs {
+    int n, ne;
+    double* x; double* y; double* z;
+    double* vx; double* vy; double* vz;
+    double* fx; double* fy; double* fz;
+    uint4* upos;               // fixed-point mirror (null on the FP64 path)
+    double box[3];
+    double xm[2];
+    double dv[2];              // VDF bin widths
+    unsigned long long* vdf;   // [2][kHistStride]
+};
+
+int md_blocks(int n);
+void launch_vv1(const MdArgs& m, int thermo, double dt, double fric, hipStream_t s);
+void launch_make_upos(const MdArgs& m, hipStream_t s);
+// f (=|+=) sum over chunks of the pair partials (+ fr when frx != null);
+// block partials (sum f, sum u) into part[4 * md_blocks(n)].
+void launch_combine(const MdArgs& m, const double* pf, const double* pu, int nchunk,
+                    const double* frx, const double* fry, const double* frz, double* part,
+                    bool accumulate, hipStream_t s);
+// sums[c] = sum_b part[b * width + c] in block order (single block)
+void launch_reduce_parts(const double* part, int nblk, int width, double* sums, hipStream_t s);
+void launch_add_forces(const MdArgs& m, const double* frx, const double* fry, const double* frz,
+                       hipStream_t s);
+void launch_force_sum(const MdArgs& m, double* part, hipStream_t s);
+void launch_sub_mean(const MdArgs& m, const double* fsum, hipStream_t s);
+void launch_vv2_kick(const MdArgs& m, int thermo, double dt, double Te, double Ti, double fric,
+                     uint64_t seed, uint64_t counter, const double* fsum, double* part,
+                     hipStream_t s);
+void launch_thermo_lambda(const double* mv2, int thermo, int ne, int ni, double Te, double Ti,
+                          double dt, double tau, double* lam, hipStream_t s);
+void launch_vv2_scale(const MdArgs& m, const double* lam, double* part, hipStream_t s);
+void launch_trace(double* row, const double* usum, double ur_scale, double Uo, const double* um,
+                  const double* mv2, double eps, hipStream_t s);
+void launch_um(const double* epart, int n, double scale, double* um, hipStream_t s);
+
+}  // namespace tcp

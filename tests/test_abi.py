@@ -1,0 +1,87 @@
+"""The C-ABI library without a GPU: it loads, exports every symbol that
+include/tcp_gpu.h declares, and rejects bad parameters before any device call."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from tcp_allpair_amd import gpu
+from tcp_allpair_amd.inputs import make_plasma
+from tcp_allpair_amd.params import CParams, FP64
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tcp_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(tcp_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_bound_functions():
+    names = declared_functions()
+    assert len(names) >= 15
+    assert set(names) == set(gpu.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = gpu.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_params_struct_layout_matches_header(tmp_path):
+    """ctypes' CParams against the C compiler's layout of tcp_params and tcpo_params."""
+    fields = [f[0] for f in CParams._fields_]
+    src = tmp_path / "layout.c"
+    body = "".join(f'  printf("%zu %zu\\n", offsetof(tcp_params, {f}), offsetof(tcpo_params, {f}));\n'
+                   for f in fields)
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"tcp_gpu.h\"\n"
+                   "#include \"tcp_oracle.h\"\nint main(void){\n"
+                   '  printf("%zu %zu\\n", sizeof(tcp_params), sizeof(tcpo_params));\n'
+                   + body + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "oracle"),
+                    str(src), "-o", str(exe)], check=True)
+    rows = [tuple(int(v) for v in line.split())
+            for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split("\n")
+            if line]
+    assert rows[0] == (ctypes.sizeof(CParams), ctypes.sizeof(CParams))
+    for f, (a, b) in zip(fields, rows[1:]):
+        assert a == b == getattr(CParams, f).offset, f
+
+
+def _create(p, precision=FP64):
+    lib = gpu.load()
+    h = ctypes.c_void_p()
+    c = CParams.from_params(p)
+    rc = lib.tcp_create(ctypes.byref(h), ctypes.byref(c), precision)
+    return rc, lib.tcp_last_error(None).decode(), h
+
+
+def test_cutoff_error_before_device_use():
+    p, _, _ = make_plasma(10, 10)
+    rc, msg, h = _create(p.replace(rc=0.51 * p.box[0]))
+    assert rc == -2 and "cut-off radius" in msg and not h.value
+
+
+@pytest.mark.parametrize("change,code", [
+    (dict(K=(24, 32, 32)), -4),      # not a power of two
+    (dict(order=5), -4),             # odd order
+    (dict(order=14), -4),
+    (dict(recip=2, kmax=0), -1),
+    (dict(ne=1, ni=0), -1),
+    (dict(rs=0.0), -1),
+])
+def test_parameter_errors(change, code):
+    p, _, _ = make_plasma(10, 10)
+    rc, msg, h = _create(p.replace(**change))
+    assert rc == code and msg and not h.value
+
+
+def test_bad_precision():
+    p, _, _ = make_plasma(10, 10)
+    rc, msg, _ = _create(p, precision=7)
+    assert rc == -1

@@ -1,0 +1,298 @@
+"""Parity of the HIP path (through the C ABI) with the reference and the oracle.
+
+Tolerances (stated per test):
+  FP64 pair forces vs the reference:  per-particle relative <= 1e-10, Ur relative <= 1e-12,
+                                      RDF counts identical
+  FP64 integrator vs the reference:   bit for bit for the same forces
+  SPME vs the oracle restatement:     per-particle relative <= 1e-10, Um relative <= 1e-12
+  MIXED vs the GPU FP64 path:         RMS relative force error <= 1e-6, Ur relative <= 1e-6
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from golden_io import load
+from oracle import cpu
+from tcp_allpair_amd.gpu import Context
+from tcp_allpair_amd.inputs import make_plasma
+from tcp_allpair_amd.params import (BEREND, FP64, ISOKIN, MIXED, NBIN, NOTEMP, RECIP_EWALD,
+                                    RECIP_NONE, RECIP_SPME, STOCH, reduced_dt)
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(f, ref):
+    nref = np.linalg.norm(ref, axis=1)
+    floor = 1e-3 * np.sqrt(np.mean(nref ** 2))
+    return np.linalg.norm(f - ref, axis=1) / np.maximum(nref, floor)
+
+
+def rms_rel(f, ref):
+    return np.sqrt(((f - ref) ** 2).sum() / (ref ** 2).sum())
+
+
+def direct(p, x, precision=FP64):
+    with Context(p.replace(recip=RECIP_NONE), precision) as c:
+        c.set_state(x=x)
+        ur = c.force_direct()
+        _, _, f = c.get_state()
+        rdf, _ = c.histograms()
+    return f, ur, rdf
+
+
+@pytest.mark.parametrize("name", ["force_tiny", "force_c0", "force_bins", "force_aniso"])
+def test_fp64_direct_vs_reference(name):
+    say("case", name)
+    p, d = load(name)
+    f, ur, rdf = direct(p, d["in_x"])
+    assert rel_err(f, d["out_f"]).max() <= 1e-10
+    assert abs(ur - d["out_Ur"]) <= 1e-12 * abs(d["out_Ur"])
+    np.testing.assert_array_equal(rdf, d["out_rdf"])
+
+
+def test_ewald_self_vs_reference():
+    p, d = load("force_c0")
+    say("case ewald_self")
+    with Context(p, FP64) as c:
+        a, rc2, uo = c.ewald_self()
+    assert a == d["out_alpha"] and rc2 == d["out_rc2"] and uo == d["out_Uo"]
+    p, d = load("psi")
+    with Context(p.replace(recip=RECIP_NONE), FP64) as c:
+        psi = c.ewald_self(with_psi=True)[3]
+    np.testing.assert_allclose(psi.ravel(), d["out_psi"].ravel(), rtol=1e-14, atol=0)
+
+
+def test_mixed_vs_fp64_c0():
+    say("case mixed_c0")
+    p, d = load("force_c0")
+    f64, u64, rdf64 = direct(p, d["in_x"], FP64)
+    f32, u32, rdf32 = direct(p, d["in_x"], MIXED)
+    assert rms_rel(f32, f64) <= 1e-6
+    assert abs(u32 - u64) <= 1e-6 * abs(u64)
+    # r in FP32 may move a pair across a bin edge: counts agree in total and almost per bin
+    assert abs(rdf32.sum() - rdf64.sum()) <= 1e-4 * rdf64.sum()
+    assert np.abs(rdf32 - rdf64).sum() <= 1e-3 * rdf64.sum()
+
+
+def test_direct_ext_vs_reference():
+    say("case direct_ext")
+    p, d = load("ext")
+    ne_o, ni_o = (int(v) for v in d["nown"])
+    x = d["in_x"]
+    own = np.concatenate([x[:ne_o], x[p.ne:p.ne + ni_o]])
+    ext = np.concatenate([x[ne_o:p.ne], x[p.ne + ni_o:]])
+    po = p.replace(ne=ne_o, ni=ni_o, recip=RECIP_NONE)
+    with Context(po, FP64) as c:
+        c.set_state(x=own)
+        ur_self = c.force_direct()
+        gemit, ur_ext = c.force_direct_ext(ext, p.ne - ne_o, p.ni - ni_o)
+        _, _, f = c.get_state()
+        rdf, _ = c.histograms()
+    assert abs(ur_self - d["out_Ur_self"]) <= 1e-12 * abs(d["out_Ur_self"])
+    assert abs(ur_ext - d["out_Ur_ext"]) <= 1e-12 * abs(d["out_Ur_ext"])
+    assert rel_err(f, d["out_f"]).max() <= 1e-10
+    assert rel_err(gemit, d["out_gemit"]).max() <= 1e-10
+    np.testing.assert_array_equal(rdf, d["out_rdf"])
+
+
+def test_remove_rigidmtn_vs_reference():
+    say("case rigid")
+    p, d = load("rigid")
+    with Context(p.replace(recip=RECIP_NONE), FP64) as c:
+        c.set_state(x=np.zeros((p.n, 3)), f=d["in_f"])
+        c.remove_rigidmtn()
+        f = c.get_state()[2]
+    np.testing.assert_allclose(f, d["out_f"], rtol=0, atol=1e-13 * np.abs(d["in_f"]).max())
+    assert np.abs(f.sum(0)).max() < 1e-11
+
+
+@pytest.mark.parametrize("name", ["vv1_notemp", "vv1_stoch"])
+def test_vverlet1_bitwise(name):
+    say("case", name)
+    p, d = load(name)
+    with Context(p.replace(recip=RECIP_NONE), FP64) as c:
+        c.set_state(x=d["in_x"], v=d["in_v"], f=d["in_f"])
+        c.vverlet1(int(d["thermo"]), float(d["dt"]), p.fric)
+        x, v, _ = c.get_state()
+    np.testing.assert_array_equal(x, d["out_x"])
+    np.testing.assert_array_equal(v, d["out_v"])
+
+
+@pytest.mark.parametrize("name", ["vv2_notemp", "vv2_berend", "vv2_isokin"])
+def test_vverlet2_vs_reference(name):
+    say("case", name)
+    p, d = load(name)
+    with Context(p.replace(recip=RECIP_NONE), FP64) as c:
+        c.set_state(x=np.zeros((p.n, 3)), v=d["in_v"], f=d["in_f"])
+        mv2el, mv2ion = c.vverlet2(int(d["thermo"]), float(d["dt"]))
+        v = c.get_state()[1]
+        _, vdf = c.histograms()
+    if int(d["thermo"]) == NOTEMP:
+        np.testing.assert_array_equal(v, d["out_v"])
+    else:  # lambda from a tree-ordered kinetic sum: equal to a few ulp
+        np.testing.assert_allclose(v, d["out_v"], rtol=1e-14, atol=0)
+    assert mv2el == pytest.approx(d["out_mv2el"], rel=1e-14)
+    assert mv2ion == pytest.approx(d["out_mv2ion"], rel=1e-14)
+    np.testing.assert_array_equal(vdf, d["out_vdf"])
+
+
+def test_stoch_noise_statistics():
+    """Langevin kick with zero force and v = 0: v = eta beta dt/2/m/(1 + fric dt/2/m)."""
+    say("case stoch")
+    p, x, _ = make_plasma(20000, 20000, seed=4)
+    p = p.replace(fric=0.3, recip=RECIP_NONE)
+    dt = reduced_dt(0.005)
+    with Context(p, FP64) as c:
+        c.set_state(x=x, v=np.zeros_like(x), f=np.zeros_like(x))
+        c.vverlet2(STOCH, dt, seed=99)
+        v = c.get_state()[1]
+    for s, sl in ((0, slice(0, p.ne)), (1, slice(p.ne, p.n))):
+        T = p.Te if s == 0 else p.Ti
+        m = p.xm[s]
+        beta = np.sqrt(2 * p.fric * T / dt)
+        scale = beta * 0.5 * dt / m / (1 + 0.5 * p.fric * dt / m)
+        eta = v[sl] / scale
+        assert abs(eta.mean()) < 4 / np.sqrt(eta.size)
+        assert abs(eta.var() - 1.0) < 0.03
+
+
+@pytest.mark.parametrize("name", ["traj_nve", "traj_isokin"])
+def test_md_steps_vs_reference_trajectory(name):
+    """main.f90:87-153 without the reciprocal part, on the device, against the reference."""
+    say("case", name)
+    p, d = load(name)
+    p = p.replace(recip=RECIP_NONE)
+    with Context(p, FP64) as c:
+        c.set_state(x=d["in_x"], v=d["in_v"], f=d["in_f"])
+        tr = c.md_steps(int(d["nsteps"]), int(d["thermo"]), float(d["dt"]))
+        x, v, f = c.get_state()
+        rdf, vdf = c.histograms()
+    ref = d["out_trace"]  # Ur, Epot, mv2el, mv2ion
+    got = tr[:, [0, 3, 4, 5]]
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=0)
+    assert rel_err(f, d["out_f"]).max() <= 1e-8
+    np.testing.assert_allclose(x, d["out_x"], rtol=0, atol=1e-10)
+    assert np.abs(rdf - d["out_rdf"]).sum() <= 1e-6 * d["out_rdf"].sum()
+    np.testing.assert_array_equal(vdf, d["out_vdf"])
+
+
+@pytest.mark.parametrize("order,K", [(4, 32), (6, 32), (8, 16)])
+def test_spme_vs_oracle(order, K):
+    say("case spme", order, K)
+    p, x, _ = make_plasma(1000, 1000, seed=31, K=(K, K, K), order=order)
+    fo, umo = cpu.spme(p, x)
+    with Context(p, FP64) as c:
+        c.set_state(x=x, f=np.zeros_like(x))
+        um = c.force_recip()
+        f = c.get_state()[2]
+    assert abs(um - umo) <= 1e-12 * abs(umo)
+    assert rel_err(f, fo).max() <= 1e-10
+
+
+def test_spme_anisotropic_vs_oracle():
+    say("case spme aniso")
+    p, x, _ = make_plasma(300, 500, seed=32, K=(32, 64, 16), order=6)
+    L = p.box[0]
+    p = p.replace(box=(L, 1.5 * L, 0.8 * L), rc=0.4 * L)
+    x = x * np.array([1.0, 1.5, 0.8])
+    fo, umo = cpu.spme(p, x)
+    with Context(p, FP64) as c:
+        c.set_state(x=x, f=np.zeros_like(x))
+        um = c.force_recip()
+        f = c.get_state()[2]
+    assert abs(um - umo) <= 1e-12 * abs(umo)
+    assert rel_err(f, fo).max() <= 1e-10
+
+
+def test_ewald_kspace_vs_oracle():
+    say("case ewald kspace")
+    p, x, _ = make_plasma(200, 200, seed=33, recip=RECIP_EWALD, kmax=8)
+    fo, umo = cpu.ewald_kspace(p, x)
+    with Context(p, FP64) as c:
+        c.set_state(x=x, f=np.zeros_like(x))
+        um = c.force_recip()
+        f = c.get_state()[2]
+    assert abs(um - umo) <= 1e-12 * abs(umo)
+    assert rel_err(f, fo).max() <= 1e-10
+
+
+def test_md_steps_with_spme_vs_oracle():
+    """The full step with SPME (order 4, K = 32), 10 steps NVE, against the oracle's step."""
+    say("case md spme")
+    p, x, v = make_plasma(500, 500, seed=41, K=(32, 32, 32), order=4)
+    dt = reduced_dt(0.005)
+    xo, vo = x.copy(), v.copy()
+    fo = np.zeros_like(x)
+    # initial forces as a restart file would hold them: one oracle force pass
+    f0, _, _ = cpu.force_allpair(p, x)
+    fr0, _ = cpu.spme(p, x)
+    f0 = cpu.remove_rigidmtn(p, f0 + fr0)
+    fo[:] = f0
+    rows = []
+    for _ in range(10):
+        r = cpu.step(p, NOTEMP, dt, xo, vo, fo, fast=True)
+        rows.append([r["Ur"], r["Um"], r["Epot"], r["mv2el"], r["mv2ion"]])
+    with Context(p, FP64) as c:
+        c.set_state(x=x, v=v, f=f0)
+        tr = c.md_steps(10, NOTEMP, dt)
+        xg, vg, fg = c.get_state()
+    np.testing.assert_allclose(tr[:, [0, 2, 3, 4, 5]], np.array(rows), rtol=1e-10)
+    assert rel_err(fg, fo).max() <= 1e-8
+
+
+def test_fp64_deterministic():
+    say("case deterministic")
+    p, x, v = make_plasma(3000, 3000, seed=5, K=(32, 32, 32))
+    outs = []
+    for _ in range(2):
+        with Context(p.replace(recip=RECIP_NONE), FP64) as c:
+            c.set_state(x=x, v=v, f=np.zeros_like(x))
+            tr = c.md_steps(3, NOTEMP, reduced_dt(0.005))
+            outs.append((tr, c.get_state()[2]))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+def test_mixed_vs_fp64_16k():
+    """16384 + 16384 particles (the 64^3 SPME workload), mixed against FP64 pair forces."""
+    say("case mixed 32k")
+    p, x, _ = make_plasma(16384, 16384, seed=1, K=(64, 64, 64))
+    f64, u64, _ = direct(p, x, FP64)
+    f32, u32, _ = direct(p, x, MIXED)
+    err = rms_rel(f32, f64)
+    say("rms rel force err", err, "max", rel_err(f32, f64).max())
+    assert err <= 1e-6
+    assert abs(u32 - u64) <= 1e-6 * abs(u64)
+    # Newton's third law: the real-space forces sum to zero
+    assert np.abs(f64.sum(0)).max() <= 1e-9 * np.abs(f64).sum()
+
+
+def test_fullsize_mixed_step_properties():
+    """131072 particles, SPME 128^3 order 6, mixed: after a step the mean force
+    is removed exactly and the energies are finite; two calls of one step each
+    equal one call of two steps (the state carries over on the device)."""
+    say("case fullsize")
+    p, x, v = make_plasma(65536, 65536, seed=7, K=(128, 128, 128), order=6)
+    dt = reduced_dt(0.002)
+    with Context(p, MIXED) as c:
+        c.set_state(x=x, v=v, f=np.zeros_like(x))
+        t1 = c.md_steps(1, NOTEMP, dt)
+        t2 = c.md_steps(1, NOTEMP, dt)
+        s1 = c.get_state()
+    with Context(p, MIXED) as c:
+        c.set_state(x=x, v=v, f=np.zeros_like(x))
+        t12 = c.md_steps(2, NOTEMP, dt)
+        s2 = c.get_state()
+    np.testing.assert_array_equal(np.vstack([t1, t2]), t12)
+    for a, b in zip(s1, s2):
+        np.testing.assert_array_equal(a, b)
+    f = s1[2]
+    assert np.all(np.isfinite(t12)) and np.all(np.isfinite(f))
+    assert np.abs(f.sum(0)).max() <= 1e-10 * np.abs(f).sum()
+    assert np.all(np.abs(s1[0]) <= 0.5 * p.box[0])
