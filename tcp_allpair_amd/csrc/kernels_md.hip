@@ -54,11 +54,24 @@ __device__ inline void vdf_count(unsigned* h, int s, double mv2, double dv)
     if (t < (double)kNbin) atomicAdd(&h[s * kHistStride + (int)t], 1u);
 }
 
-__device__ inline uint32_t to_fixed(double x, double inv_box)
+// fractional coordinate in [0, 1] -> 32-bit fixed point (modulo 2^32) and the
+// FP32 residual of the unwrapped value
+__device__ inline void to_fixed(double x, double box, uint32_t* u, float* r)
 {
-    // fractional coordinate in [0, 1] -> 32-bit fixed point, modulo 2^32
-    const double s = x * inv_box + 0.5;
-    return (uint32_t)(uint64_t)(int64_t)rint(s * 4294967296.0);
+    const double s = rint((x / box + 0.5) * 4294967296.0);
+    *u = (uint32_t)(uint64_t)(int64_t)s;
+    *r = (float)(x - (s * 0x1p-32 - 0.5) * box);
+}
+
+__device__ inline FixPos make_fixpos(double x, double y, double z, const double* box)
+{
+    FixPos f;
+    to_fixed(x, box[0], &f.ux, &f.rx);
+    to_fixed(y, box[1], &f.uy, &f.ry);
+    to_fixed(z, box[2], &f.uz, &f.rz);
+    f.pad0 = 0u;
+    f.pad1 = 0.f;
+    return f;
 }
 
 __global__ __launch_bounds__(kRedBlock) void vv1_kernel(MdArgs m, int thermo, double dt, double fric)
@@ -83,17 +96,14 @@ __global__ __launch_bounds__(kRedBlock) void vv1_kernel(MdArgs m, int thermo, do
         vs[k][i] = v;
         xs[k][i] = xn[k];
     }
-    if (m.upos)
-        m.upos[i] = make_uint4(to_fixed(xn[0], 1.0 / m.box[0]), to_fixed(xn[1], 1.0 / m.box[1]),
-                               to_fixed(xn[2], 1.0 / m.box[2]), 0u);
+    if (m.upos) m.upos[i] = make_fixpos(xn[0], xn[1], xn[2], m.box);
 }
 
 __global__ __launch_bounds__(kRedBlock) void upos_kernel(MdArgs m)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.n) return;
-    m.upos[i] = make_uint4(to_fixed(m.x[i], 1.0 / m.box[0]), to_fixed(m.y[i], 1.0 / m.box[1]),
-                           to_fixed(m.z[i], 1.0 / m.box[2]), 0u);
+    m.upos[i] = make_fixpos(m.x[i], m.y[i], m.z[i], m.box);
 }
 
 // f = sum over chunks of the pair partials (+ reciprocal forces), block

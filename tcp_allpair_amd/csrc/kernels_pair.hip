@@ -120,30 +120,78 @@ __global__ __launch_bounds__(kPairBlock) void pair_f64_kernel(PairArgs a)
 
 // ----------------------------------------------------------------- MIXED ---
 //
-// Positions are 32-bit fixed-point fractions of the box (u = (x/L + 1/2) 2^32
-// mod 2^32); the wrapped 32-bit difference of two of them is the minimum-image
-// separation exactly, so the FP32 arithmetic starts from a separation with no
-// cancellation error and no rint.  erfc(x) = exp(-x^2) P(t), t = 1/(1+0.4x),
-// with P a degree-8 polynomial fitted here (weighted least squares plus Lawson
-// iterations against exp(x^2) erfc(x), relative error 4.6e-9 on [0, 4.5],
-// about 3e-7 once evaluated in FP32).  exp(-x^2) is shared with the
-// 2x exp(-x^2)/sqrt(pi) term of the force.  Per-lane sums run in FP32 over
-// tiles of 32 j and are added to FP64 accumulators after each tile.
+// Separations come from FixPos: the wrapped 32-bit difference of the fixed-point
+// coordinates is the minimum-image separation exactly, and the FP32 residuals
+// restore the FP64 position to FP32 rounding, dx = float(int(u_i - u_j)) s +
+// (r_i - r_j): no rint and no cancellation even for pairs 0.01 A apart in a
+// 50 A box.  erfc(x) = exp(-x^2) P(w), w = (x/2)/(1 + x/2), with P of degree 9
+// and P(0) = 1 fitted here (weighted least squares plus Lawson iterations
+// against exp(x^2) erfc(x) on [0, 4.5]; relative error 5.6e-9 in exact
+// arithmetic, 3e-8 near x = 0 and below 7.2e-7 up to x = 3.2 once evaluated in
+// FP32).  exp(-x^2) is shared with the 2x exp(-x^2)/sqrt(pi) force term.
+// Electron-ion pairs closer than 2 lambda take a rare, divergent branch
+// (close_ei) where the Deutsch and Ewald parts are evaluated without their
+// mutual cancellation.
+// Per-lane sums run in FP32 over tiles of 32 j and go to FP64 after each tile.
 
-constexpr float kLog2e = 1.4426950408889634f;
-
-__device__ inline float erfcx_poly(float t)
+__device__ inline float erfcx_poly(float w)
 {
-    float p = 8.698318719e-02f;
-    p = fmaf(p, t, -4.607766800e-01f);
-    p = fmaf(p, t, 8.327167932e-01f);
-    p = fmaf(p, t, -5.598961666e-01f);
-    p = fmaf(p, t, 5.512020669e-01f);
-    p = fmaf(p, t, 7.153115576e-02f);
-    p = fmaf(p, t, 2.562803493e-01f);
-    p = fmaf(p, t, 2.217379002e-01f);
-    p = fmaf(p, t, 2.213986858e-04f);
+    float p = 2.674967051e-02f;
+    p = fmaf(p, w, -1.345596015e-01f);
+    p = fmaf(p, w, 1.725731939e-01f);
+    p = fmaf(p, w, 4.970539361e-03f);
+    p = fmaf(p, w, 3.443385288e-02f);
+    p = fmaf(p, w, -3.166950941e-01f);
+    p = fmaf(p, w, -2.740879953e-01f);
+    p = fmaf(p, w, 1.743201137e+00f);
+    p = fmaf(p, w, -2.256757498e+00f);
+    p = fmaf(p, w, 1.0f);
     return p;
+}
+
+// Close electron-ion pair (r < 2 lambda), where erfc + g and e^-s (1 + s) are
+// both near 1 and cancel: num = B(s) - A(x), en = B(s) + s e^-s - erf(x) with
+//   B(s)/s^2 by a degree-8 polynomial fitted here on s in [0, 2] (relative error
+//            1.5e-9 in exact arithmetic, 3.3e-7 evaluated in FP32),
+//   A(x)/x^3 = (2/sqrt(pi)) sum_n (-1)^(n+1) 2n/(n!(2n+1)) x^(2n-2), n = 1..8,
+//   erf(x)/x = (2/sqrt(pi)) sum_n (-1)^n x^(2n)/(n!(2n+1)),        n = 0..7,
+// the two Taylor series truncated for x < 0.6; larger x falls back to erfc.
+__device__ inline void close_ei(float r, float s, float x, float y, float ec, float g,
+                                float* num, float* en)
+{
+    float b = 9.974618998e-07f;
+    b = fmaf(b, s, -1.720568391e-05f);
+    b = fmaf(b, s, 1.642914247e-04f);
+    b = fmaf(b, s, -1.179569318e-03f);
+    b = fmaf(b, s, 6.936748278e-03f);
+    b = fmaf(b, s, -3.333021883e-02f);
+    b = fmaf(b, s, 1.249993556e-01f);
+    b = fmaf(b, s, -3.333332808e-01f);
+    b = fmaf(b, s, 4.999999993e-01f);
+    const float B = b * s * s;
+    const float z = x * x;
+    float pa = -2.633938299e-05f;
+    pa = fmaf(pa, z, 2.089591050e-04f);
+    pa = fmaf(pa, z, -1.446639958e-03f);
+    pa = fmaf(pa, z, 8.548327023e-03f);
+    pa = fmaf(pa, z, -4.179182100e-02f);
+    pa = fmaf(pa, z, 1.611970239e-01f);
+    pa = fmaf(pa, z, -4.513516668e-01f);
+    pa = fmaf(pa, z, 7.522527781e-01f);
+    float pe = -1.492565036e-05f;
+    pe = fmaf(pe, z, 1.205533298e-04f);
+    pe = fmaf(pe, z, -8.548327023e-04f);
+    pe = fmaf(pe, z, 5.223977625e-03f);
+    pe = fmaf(pe, z, -2.686617065e-02f);
+    pe = fmaf(pe, z, 1.128379167e-01f);
+    pe = fmaf(pe, z, -3.761263890e-01f);
+    pe = fmaf(pe, z, 1.128379167e+00f);
+    const bool series = x < 0.6f;
+    const float A = series ? pa * z * x : 1.0f - ec - g;
+    const float erfx = series ? pe * x : 1.0f - ec;
+    *num = B - A;
+    *en = fmaf(s, y, B) - erfx;
+    (void)r;
 }
 
 constexpr int kMixTile = 32;
@@ -159,7 +207,7 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
     const int i = blockIdx.x * kPairBlock + threadIdx.x;
     const bool live = i < a.n_i;
     const int ii = live ? i : a.n_i - 1;
-    const uint4 ui = a.ui[ii];
+    const FixPos pi = a.ui[ii];
     const int si = ii >= a.ne_i;
     const int j0 = blockIdx.y * a.jchunk;
     const int j1 = min(a.n_j, j0 + a.jchunk);
@@ -167,11 +215,12 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
     const float sy = (float)(a.box[1] * 0x1p-32);
     const float sz = (float)(a.box[2] * 0x1p-32);
     const float rc2 = (float)a.rc2;
-    const float alpha = (float)a.alpha;
+    const float half_alpha = (float)(0.5 * a.alpha);
     const float na2l = (float)(-a.alpha * a.alpha * 1.4426950408889634);
     const float two_a_sqrtpi = (float)(2.0 * a.alpha / kSqrtPi);
     const float inv_rs = (float)(1.0 / a.rs);
     const float nl2rs = (float)(-1.4426950408889634 / a.rs);
+    const float rclose = (float)(2.0 * a.rs);
     double fxd = 0.0, fyd = 0.0, fzd = 0.0, ud = 0.0;
     for (int sj = 0; sj < 2; ++sj) {
         const int lo = sj ? max(j0, a.ne_j) : j0;
@@ -186,24 +235,27 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
             float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
 #pragma unroll 4
             for (int j = jt; j < je; ++j) {
-                const uint4 uj = a.uj[j];
-                const float dx = (float)(int)(ui.x - uj.x) * sx;
-                const float dy = (float)(int)(ui.y - uj.y) * sy;
-                const float dz = (float)(int)(ui.z - uj.z) * sz;
+                const FixPos pj = a.uj[j];
+                const float dx = fmaf((float)(int)(pi.ux - pj.ux), sx, pi.rx - pj.rx);
+                const float dy = fmaf((float)(int)(pi.uy - pj.uy), sy, pi.ry - pj.ry);
+                const float dz = fmaf((float)(int)(pi.uz - pj.uz), sz, pi.rz - pj.rz);
                 const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 const bool in = (r2 < rc2) & (r2 > 0.f);
                 const float rinv = __builtin_amdgcn_rsqf(r2);
                 const float r = r2 * rinv;
-                const float x = alpha * r;
                 const float e2 = __builtin_amdgcn_exp2f(na2l * r2);
-                const float t = __builtin_amdgcn_rcpf(fmaf(0.4f, x, 1.0f));
-                const float ec = e2 * erfcx_poly(t);
+                const float px = half_alpha * r;
+                const float t = __builtin_amdgcn_rcpf(px + 1.0f);
+                const float ec = e2 * erfcx_poly(px * t);
                 float num = fmaf(two_a_sqrtpi * r, e2, ec);
                 float en = ec;
                 if (ei) {
                     const float y = __builtin_amdgcn_exp2f(nl2rs * r);
-                    num = fmaf(-y, fmaf(r, inv_rs, 1.0f), num);
+                    const float sr = r * inv_rs;
+                    num = fmaf(-y, sr + 1.0f, num);
                     en -= y;
+                    if (in && r < rclose)
+                        close_ei(r, sr, 2.0f * px, y, ec, two_a_sqrtpi * r * e2, &num, &en);
                 }
                 const float rinv2 = rinv * rinv;
                 const float df = in ? eqq * num * rinv2 * rinv : 0.f;

@@ -1,7 +1,7 @@
 // C ABI of libtcp_gpu (include/tcp_gpu.h): context, device memory and the
 // orchestration of the kernels in kernels_pair.hip, kernels_spme.hip and
 // kernels_md.hip.  Device layout: positions, velocities and forces as FP64
-// structure of arrays (electrons first), a uint4 fixed-point position mirror
+// structure of arrays (electrons first), a fixed-point position mirror (FixPos)
 // for the mixed pair kernel, per-(chunk, particle) FP64 pair partials, the
 // K1*K2*K3 SPME grids, and a small block of device scalars so that a whole MD
 // step runs without a host round trip.
@@ -46,7 +46,7 @@ struct tcp_ctx {
     double *x = nullptr, *y = nullptr, *z = nullptr;
     double *vx = nullptr, *vy = nullptr, *vz = nullptr;
     double *fx = nullptr, *fy = nullptr, *fz = nullptr;
-    uint4* upos = nullptr;
+    FixPos* upos = nullptr;
     // pair partials
     int nchunk = 1, jchunk = 1;
     size_t pf_cap = 0;  // doubles
@@ -68,7 +68,7 @@ struct tcp_ctx {
     int g_cap = 0;
     double *gx = nullptr, *gy = nullptr, *gz = nullptr;
     double *gfx = nullptr, *gfy = nullptr, *gfz = nullptr;
-    uint4* gupos = nullptr;
+    FixPos* gupos = nullptr;
     double *gpf = nullptr, *gpu = nullptr;
     size_t gpf_cap = 0;
     // timing

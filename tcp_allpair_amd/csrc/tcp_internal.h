@@ -14,6 +14,16 @@ constexpr int kHistStride = 512;       // padded bins per histogram in device me
 constexpr int kPairBlock = 256;        // threads (i particles) per pair-kernel block
 constexpr int kRedBlock = 256;         // threads per block of the O(N) kernels
 
+// Fixed-point position for the mixed pair kernel: u = round((x/L + 1/2) 2^32)
+// modulo 2^32 per axis and the FP32 residual r = x - (u 2^-32 - 1/2) L of the
+// unwrapped u.  The wrapped 32-bit difference of two u is the minimum-image
+// separation in units of L 2^-32 exactly, and adding the residual difference
+// restores the FP64 separation to FP32 rounding: dx = float(int(u_i - u_j)) s + (r_i - r_j).
+struct __align__(32) FixPos {
+    uint32_t ux, uy, uz, pad0;
+    float rx, ry, rz, pad1;
+};
+
 // Particle positions as structure of arrays, electrons first.
 struct PosSoA {
     const double* x;
@@ -26,8 +36,8 @@ struct PosSoA {
 struct PairArgs {
     PosSoA pi;                 // i-set (FP64 path)
     PosSoA pj;                 // j-set (FP64 path)
-    const uint4* ui;           // i-set fixed-point fractional coordinates (mixed path)
-    const uint4* uj;           // j-set fixed-point fractional coordinates (mixed path)
+    const FixPos* ui;          // i-set fixed-point coordinates (mixed path)
+    const FixPos* uj;          // j-set fixed-point coordinates (mixed path)
     int n_i, ne_i;             // i-set size and electron count
     int n_j, ne_j;             // j-set size and electron count
     int jchunk;                // j indices per blockIdx.y
@@ -85,7 +95,7 @@ struct MdArgs {
     double* x; double* y; double* z;
     double* vx; double* vy; double* vz;
     double* fx; double* fy; double* fz;
-    uint4* upos;               // fixed-point mirror (null on the FP64 path)
+    FixPos* upos;              // fixed-point mirror of x, y, z
     double box[3];
     double xm[2];
     double dv[2];              // VDF bin widths
