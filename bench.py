@@ -41,7 +41,7 @@ from tcp_allpair_amd.params import FP64, MIXED, NOTEMP, reduced_dt  # noqa: E402
 
 FLOP_PER_ORDERED_PAIR = 62.5       # FMA = 2; exp, rsqrt, rcp = 1 each; e-i/e-e mix
 FP32_VECTOR_PEAK_TFLOPS = 157.3
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pair_mixed_n131072_traffic.json")
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pair_mixed_n131072_pmc.md")
 
 
 def workload():
@@ -141,8 +141,10 @@ def main():
     flops = N * (N - 1) * FLOP_PER_ORDERED_PAIR
     achieved = flops / (mx["pair_ms"] * 1e-3) / 1e12
     traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
-        traffic = json.load(open(PROFILE_TRAFFIC)).get("hbm_bytes_per_launch")
+    if os.path.exists(PROFILE_TRAFFIC):   # measured by rocprofv3 --pmc, see that file
+        for line in open(PROFILE_TRAFFIC):
+            if line.startswith("hbm_bytes_per_launch:"):
+                traffic = float(line.split(":")[1])
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (reference on host cores)", file=sys.stderr, flush=True)

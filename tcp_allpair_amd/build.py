@@ -28,6 +28,7 @@ FLAGS = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
 SOURCES = ["kernels_pair.hip", "kernels_spme.hip", "kernels_md.hip", "tcp_gpu.hip"]
+PROBE = os.path.join(HERE, "copy_probe")   # HBM-counter calibration program
 
 
 def _compile(src: str) -> str:
@@ -49,8 +50,12 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    cmd = [HIPCC, *FLAGS, os.path.join(CSRC, "copy_probe.hip"), "-o", PROBE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for copy_probe.hip:\n{r.stderr}")
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", PROBE)
     return LIB
 
 
