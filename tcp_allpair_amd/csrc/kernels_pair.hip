@@ -196,6 +196,75 @@ __device__ inline void close_ei(float r, float s, float x, float y, float ec, fl
 
 constexpr int kMixTile = 32;
 
+// Loop-invariant FP32 constants of the mixed pair arithmetic.
+struct MixConst {
+    float sx, sy, sz;          // L_d 2^-32
+    float rc2;
+    float half_alpha;          // alpha/2
+    float na2l;                // -alpha^2 log2(e)
+    float two_a_sqrtpi;        // 2 alpha/sqrt(pi)
+    float inv_rs;              // 1/lambda
+    float nl2rs;               // -log2(e)/lambda
+    float rclose;              // 2 lambda
+    float inv_dx[3];           // RDF bin widths (reciprocal)
+};
+
+__device__ inline MixConst mix_const(const PairArgs& a)
+{
+    MixConst k;
+    k.sx = (float)(a.box[0] * 0x1p-32);
+    k.sy = (float)(a.box[1] * 0x1p-32);
+    k.sz = (float)(a.box[2] * 0x1p-32);
+    k.rc2 = (float)a.rc2;
+    k.half_alpha = (float)(0.5 * a.alpha);
+    k.na2l = (float)(-a.alpha * a.alpha * 1.4426950408889634);
+    k.two_a_sqrtpi = (float)(2.0 * a.alpha / kSqrtPi);
+    k.inv_rs = (float)(1.0 / a.rs);
+    k.nl2rs = (float)(-1.4426950408889634 / a.rs);
+    k.rclose = (float)(2.0 * a.rs);
+    for (int t = 0; t < 3; ++t) k.inv_dx[t] = (float)a.inv_dx[t];
+    return k;
+}
+
+// One pair in FP32: separation d, scalar force df (eps q q included, 0 when
+// the pair is masked out) and energy term du = q q (erfc - y)/r.
+struct MixPair {
+    float dx, dy, dz, r, df, du;
+    bool in;
+};
+
+__device__ inline MixPair mix_pair(const MixConst& k, const FixPos& pi, const FixPos& pj,
+                                   float qq, float eqq, bool ei, bool live)
+{
+    MixPair o;
+    o.dx = fmaf((float)(int)(pi.ux - pj.ux), k.sx, pi.rx - pj.rx);
+    o.dy = fmaf((float)(int)(pi.uy - pj.uy), k.sy, pi.ry - pj.ry);
+    o.dz = fmaf((float)(int)(pi.uz - pj.uz), k.sz, pi.rz - pj.rz);
+    const float r2 = fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz));
+    o.in = (r2 < k.rc2) & (r2 > 0.f) & live;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    const float e2 = __builtin_amdgcn_exp2f(k.na2l * r2);
+    const float px = k.half_alpha * r;
+    const float t = __builtin_amdgcn_rcpf(px + 1.0f);
+    const float ec = e2 * erfcx_poly(px * t);
+    float num = fmaf(k.two_a_sqrtpi * r, e2, ec);
+    float en = ec;
+    if (ei) {
+        const float y = __builtin_amdgcn_exp2f(k.nl2rs * r);
+        const float sr = r * k.inv_rs;
+        num = fmaf(-y, sr + 1.0f, num);
+        en -= y;
+        if (o.in && r < k.rclose)
+            close_ei(r, sr, 2.0f * px, y, ec, k.two_a_sqrtpi * r * e2, &num, &en);
+    }
+    const float rinv2 = rinv * rinv;
+    o.df = o.in ? eqq * num * rinv2 * rinv : 0.f;
+    o.du = o.in ? qq * en * rinv : 0.f;
+    o.r = r;
+    return o;
+}
+
 template <bool kRdf>
 __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
 {
@@ -204,6 +273,7 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
         hist_clear(hist);
         __syncthreads();
     }
+    const MixConst k = mix_const(a);
     const int i = blockIdx.x * kPairBlock + threadIdx.x;
     const bool live = i < a.n_i;
     const int ii = live ? i : a.n_i - 1;
@@ -211,16 +281,6 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
     const int si = ii >= a.ne_i;
     const int j0 = blockIdx.y * a.jchunk;
     const int j1 = min(a.n_j, j0 + a.jchunk);
-    const float sx = (float)(a.box[0] * 0x1p-32);
-    const float sy = (float)(a.box[1] * 0x1p-32);
-    const float sz = (float)(a.box[2] * 0x1p-32);
-    const float rc2 = (float)a.rc2;
-    const float half_alpha = (float)(0.5 * a.alpha);
-    const float na2l = (float)(-a.alpha * a.alpha * 1.4426950408889634);
-    const float two_a_sqrtpi = (float)(2.0 * a.alpha / kSqrtPi);
-    const float inv_rs = (float)(1.0 / a.rs);
-    const float nl2rs = (float)(-1.4426950408889634 / a.rs);
-    const float rclose = (float)(2.0 * a.rs);
     double fxd = 0.0, fyd = 0.0, fzd = 0.0, ud = 0.0;
     for (int sj = 0; sj < 2; ++sj) {
         const int lo = sj ? max(j0, a.ne_j) : j0;
@@ -229,43 +289,18 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
         const float eqq = (float)(a.eps * a.q[si] * a.q[sj]);
         const bool ei = si != sj;
         const int ht = si + sj;
-        const float inv_dx = (float)a.inv_dx[ht];
         for (int jt = lo; jt < hi; jt += kMixTile) {
             const int je = min(hi, jt + kMixTile);
             float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
 #pragma unroll 4
             for (int j = jt; j < je; ++j) {
-                const FixPos pj = a.uj[j];
-                const float dx = fmaf((float)(int)(pi.ux - pj.ux), sx, pi.rx - pj.rx);
-                const float dy = fmaf((float)(int)(pi.uy - pj.uy), sy, pi.ry - pj.ry);
-                const float dz = fmaf((float)(int)(pi.uz - pj.uz), sz, pi.rz - pj.rz);
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const bool in = (r2 < rc2) & (r2 > 0.f);
-                const float rinv = __builtin_amdgcn_rsqf(r2);
-                const float r = r2 * rinv;
-                const float e2 = __builtin_amdgcn_exp2f(na2l * r2);
-                const float px = half_alpha * r;
-                const float t = __builtin_amdgcn_rcpf(px + 1.0f);
-                const float ec = e2 * erfcx_poly(px * t);
-                float num = fmaf(two_a_sqrtpi * r, e2, ec);
-                float en = ec;
-                if (ei) {
-                    const float y = __builtin_amdgcn_exp2f(nl2rs * r);
-                    const float sr = r * inv_rs;
-                    num = fmaf(-y, sr + 1.0f, num);
-                    en -= y;
-                    if (in && r < rclose)
-                        close_ei(r, sr, 2.0f * px, y, ec, two_a_sqrtpi * r * e2, &num, &en);
-                }
-                const float rinv2 = rinv * rinv;
-                const float df = in ? eqq * num * rinv2 * rinv : 0.f;
-                const float du = in ? qq * en * rinv : 0.f;
-                fx = fmaf(df, dx, fx);
-                fy = fmaf(df, dy, fy);
-                fz = fmaf(df, dz, fz);
-                u += du;
-                if (kRdf && in && live) {
-                    const int b = (int)(r * inv_dx);
+                const MixPair o = mix_pair(k, pi, a.uj[j], qq, eqq, ei, live);
+                fx = fmaf(o.df, o.dx, fx);
+                fy = fmaf(o.df, o.dy, fy);
+                fz = fmaf(o.df, o.dz, fz);
+                u += o.du;
+                if (kRdf && o.in) {
+                    const int b = (int)(o.r * k.inv_dx[ht]);
                     if (b < kNbin) atomicAdd(&hist[ht * kHistStride + b], 1u);
                 }
             }
@@ -289,6 +324,177 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
     }
 }
 
+// ------------------------------------------- MIXED, Newton's third law ---
+//
+// Self interaction of one particle set, each unordered pair once.  Particles
+// form tiles of 64 (one wavefront); a block owns a 16 x 16 square of tile
+// pairs (I, J) of the upper triangle (blockmap) with 8 wavefronts, each taking
+// two I tiles.  For a tile pair the wavefront holds the 64 i in registers and
+// the 64 j (FixPos, species and live flags) plus their reaction sums in
+// registers as well, and rotates the j data and reactions one lane per step
+// with DPP (wave_ror:1): after 64 steps every i has met every j once and each
+// reaction sum is back in the lane of its j.  i forces add over a J tile in
+// FP32 and across tiles in FP64; reactions go to an LDS FP64 accumulator per
+// J particle.  Diagonal tile pairs (I == J) evaluate every ordered pair for the
+// i side only, with half the energy and RDF weight 1 instead of 2.
+// Partials: particle p of block-row b receives one (f, u) row per block-row
+// slot s: from block (b, s) as an i when s > b, from block (s, b) as a j when
+// s < b, and from block (b, b) for s == b, so the sum over slots in a fixed
+// order is the full force and energy.
+
+constexpr int kTile = 64;
+constexpr int kBlkTiles = 16;
+constexpr int kN3Waves = 8;
+
+// wave_ror:1 -- every lane takes the value of the lane below it (lane 0 that of lane 63)
+__device__ inline uint32_t rot_u(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x13C, 0xF, 0xF, false);
+}
+
+__device__ inline float rot_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x13C, 0xF, 0xF, false));
+}
+
+template <bool kRdf, bool kDiag>
+__device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* hist,
+                               const FixPos& pi, int si, bool ilive, FixPos pj,
+                               float& fxo, float& fyo, float& fzo, float& uo,
+                               float& rx, float& ry, float& rz)
+{
+    const float qi = (float)a.q[si];
+    const float eqi = (float)(a.eps * a.q[si]);
+    // RDF bin widths for a j electron / a j ion, seen from this i
+    const float idx0 = si ? k.inv_dx[1] : k.inv_dx[0];
+    const float idx1 = si ? k.inv_dx[2] : k.inv_dx[1];
+    float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
+    rx = ry = rz = 0.f;
+    for (int s = 0; s < kTile; ++s) {
+        const int sj = (int)(pj.pad0 & 1u);
+        const bool jlive = (pj.pad0 & 2u) != 0u;
+        const float qj = pj.pad1;  // charge of j rides along with its coordinates
+        const MixPair o = mix_pair(k, pi, pj, qi * qj, eqi * qj, si != sj, ilive & jlive);
+        const float gx = o.df * o.dx, gy = o.df * o.dy, gz = o.df * o.dz;
+        fx += gx;
+        fy += gy;
+        fz += gz;
+        if (kDiag) {
+            u = fmaf(0.5f, o.du, u);
+        } else {
+            u += o.du;
+            rx -= gx;
+            ry -= gy;
+            rz -= gz;
+        }
+        if (kRdf && o.in) {
+            const int ht = si + sj;
+            const int b = (int)(o.r * (sj ? idx1 : idx0));
+            if (b < kNbin) atomicAdd(&hist[ht * kHistStride + b], kDiag ? 1u : 2u);
+        }
+        pj.ux = rot_u(pj.ux);
+        pj.uy = rot_u(pj.uy);
+        pj.uz = rot_u(pj.uz);
+        pj.pad0 = rot_u(pj.pad0);
+        pj.rx = rot_f(pj.rx);
+        pj.ry = rot_f(pj.ry);
+        pj.rz = rot_f(pj.rz);
+        pj.pad1 = rot_f(pj.pad1);
+        if (!kDiag) {
+            rx = rot_f(rx);
+            ry = rot_f(ry);
+            rz = rot_f(rz);
+        }
+    }
+    fxo = fx;
+    fyo = fy;
+    fzo = fz;
+    uo = u;
+}
+
+template <bool kRdf>
+__global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArgs a,
+                                                                         const int2* blkmap)
+{
+    constexpr int kBP = kBlkTiles * kTile;  // particles per block side
+    __shared__ FixPos jpos[kBP];
+    __shared__ double racc[4][kBP];
+    __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
+    const int2 rc = blkmap[blockIdx.x];
+    const int br = rc.x, bc = rc.y;
+    const bool dblk = br == bc;
+    const int n = a.n_i, ne = a.ne_i;
+    for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
+        const int jp = bc * kBP + p;
+        FixPos f = a.ui[min(jp, n - 1)];
+        f.pad0 = (jp >= ne ? 1u : 0u) | (jp < n ? 2u : 0u);
+        f.pad1 = (float)a.q[jp >= ne];
+        jpos[p] = f;
+        racc[0][p] = racc[1][p] = racc[2][p] = racc[3][p] = 0.0;
+    }
+    if (kRdf) hist_clear(hist);
+    __syncthreads();
+    const MixConst k = mix_const(a);
+    const int wave = threadIdx.x / kTile, lane = threadIdx.x % kTile;
+    const int ntiles = (n + kTile - 1) / kTile;
+    for (int h = 0; h < kBlkTiles / kN3Waves; ++h) {
+        const int ti = wave + h * kN3Waves;
+        const int I = br * kBlkTiles + ti;
+        if (I >= ntiles) continue;
+        const int ip = I * kTile + lane;
+        const bool ilive = ip < n;
+        const FixPos pi = a.ui[min(ip, n - 1)];
+        const int si = ip >= ne;
+        double Fx = 0.0, Fy = 0.0, Fz = 0.0, U = 0.0;
+        for (int tj = dblk ? ti : 0; tj < kBlkTiles; ++tj) {
+            const int J = bc * kBlkTiles + tj;
+            if (J >= ntiles) break;
+            float fx, fy, fz, u, rx, ry, rz;
+            const FixPos pj = jpos[tj * kTile + lane];
+            if (dblk && tj == ti) {
+                n3_tile<kRdf, true>(k, a, hist, pi, si, ilive, pj, fx, fy, fz, u, rx, ry, rz);
+            } else {
+                n3_tile<kRdf, false>(k, a, hist, pi, si, ilive, pj, fx, fy, fz, u, rx, ry, rz);
+                const int q = tj * kTile + lane;
+                atomicAdd(&racc[0][q], (double)rx);
+                atomicAdd(&racc[1][q], (double)ry);
+                atomicAdd(&racc[2][q], (double)rz);
+            }
+            Fx += (double)fx;
+            Fy += (double)fy;
+            Fz += (double)fz;
+            U += (double)u;
+        }
+        if (dblk) {
+            const int q = ti * kTile + lane;
+            atomicAdd(&racc[0][q], Fx);
+            atomicAdd(&racc[1][q], Fy);
+            atomicAdd(&racc[2][q], Fz);
+            atomicAdd(&racc[3][q], U);
+        } else if (ilive) {
+            double* pf = a.pf + (size_t)bc * 3 * n;
+            pf[ip] = Fx;
+            pf[n + ip] = Fy;
+            pf[2 * n + ip] = Fz;
+            a.pu[(size_t)bc * n + ip] = U;
+        }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
+        const int jp = bc * kBP + p;
+        if (jp >= n) continue;
+        double* pf = a.pf + (size_t)br * 3 * n;
+        pf[jp] = racc[0][p];
+        pf[n + jp] = racc[1][p];
+        pf[2 * n + jp] = racc[2][p];
+        a.pu[(size_t)br * n + jp] = racc[3][p];
+    }
+    if (kRdf) {
+        __syncthreads();
+        hist_flush(hist, a.rdf, 1u);
+    }
+}
+
 }  // namespace
 
 void launch_pair_f64(const PairArgs& a, int nchunk, hipStream_t s)
@@ -298,6 +504,22 @@ void launch_pair_f64(const PairArgs& a, int nchunk, hipStream_t s)
         hipLaunchKernelGGL(pair_f64_kernel<true>, grid, dim3(kPairBlock), 0, s, a);
     else
         hipLaunchKernelGGL(pair_f64_kernel<false>, grid, dim3(kPairBlock), 0, s, a);
+}
+
+int n3_block_rows(int n)
+{
+    const int ntiles = (n + kTile - 1) / kTile;
+    return (ntiles + kBlkTiles - 1) / kBlkTiles;
+}
+
+void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, hipStream_t s)
+{
+    if (a.rdf_weight)
+        hipLaunchKernelGGL(pair_mixed_n3_kernel<true>, dim3(nblk), dim3(kN3Waves * kTile), 0, s, a,
+                           blkmap);
+    else
+        hipLaunchKernelGGL(pair_mixed_n3_kernel<false>, dim3(nblk), dim3(kN3Waves * kTile), 0, s, a,
+                           blkmap);
 }
 
 void launch_pair_mixed(const PairArgs& a, int nchunk, hipStream_t s)

@@ -49,6 +49,8 @@ struct tcp_ctx {
     FixPos* upos = nullptr;
     // pair partials
     int nchunk = 1, jchunk = 1;
+    int n3_rows = 0, n3_blocks = 0;  // Newton's-third-law kernel: block-row slots and blocks
+    int2* blkmap = nullptr;
     size_t pf_cap = 0;  // doubles
     double *pf = nullptr, *pu = nullptr;
     // reciprocal part
@@ -76,6 +78,7 @@ struct tcp_ctx {
     hipEvent_t ev_all[2];
     bool events = false;
     double pair_ms = 0, steps_ms = 0;
+    double ur_scale = 0.5;
     uint64_t counter = 0;
     std::vector<void*> allocs;
     std::string err;
@@ -215,10 +218,18 @@ int direct_all(tcp_ctx* c, int rdf_weight, const double* frx, const double* fry,
     a.pf = c->pf;
     a.pu = c->pu;
     if (ev_slot >= 0) HIPCK(c, hipEventRecord(c->ev_pair[0][ev_slot], c->stream));
-    launch_pair(c, a, c->nchunk);
+    int slots = c->nchunk;
+    if (c->precision == TCP_MIXED) {
+        launch_pair_mixed_n3(a, c->blkmap, c->n3_blocks, c->stream);
+        slots = c->n3_rows;
+    } else {
+        launch_pair_f64(a, c->nchunk, c->stream);
+    }
     LAUNCHCK(c);
     if (ev_slot >= 0) HIPCK(c, hipEventRecord(c->ev_pair[1][ev_slot], c->stream));
-    launch_combine(md_args(c), c->pf, c->pu, c->nchunk, frx, fry, frz, c->part, false, c->stream);
+    // Ur = sum u / 2 for the ordered-pair FP64 kernel, sum u for the Newton kernel
+    c->ur_scale = c->precision == TCP_MIXED ? 1.0 : 0.5;
+    launch_combine(md_args(c), c->pf, c->pu, slots, frx, fry, frz, c->part, false, c->stream);
     LAUNCHCK(c);
     launch_reduce_parts(c->part, md_blocks(c->N), 4, c->sc + SC_FSUM, c->stream);
     LAUNCHCK(c);
@@ -467,9 +478,23 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
         if ((rc = dalloc(c, a, n))) return bail(rc);
     if ((rc = dalloc(c, &c->upos, n))) return bail(rc);
     choose_chunks(c->N, c->N, &c->nchunk, &c->jchunk);
-    c->pf_cap = (size_t)c->nchunk * 3 * n;
+    int slots = c->nchunk;
+    if (precision == TCP_MIXED) {
+        // upper triangle of block pairs, off-diagonal (full) blocks first
+        c->n3_rows = n3_block_rows(c->N);
+        std::vector<int2> map;
+        for (int r = 0; r < c->n3_rows; ++r)
+            for (int q = r + 1; q < c->n3_rows; ++q) map.push_back(make_int2(r, q));
+        for (int r = 0; r < c->n3_rows; ++r) map.push_back(make_int2(r, r));
+        c->n3_blocks = (int)map.size();
+        if ((rc = dalloc(c, &c->blkmap, map.size()))) return bail(rc);
+        if (hipMemcpy(c->blkmap, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(c, TCP_ERR_HIP, "hipMemcpy of the block map failed"));
+        if (c->n3_rows > slots) slots = c->n3_rows;
+    }
+    c->pf_cap = (size_t)slots * 3 * n;
     if ((rc = dalloc(c, &c->pf, c->pf_cap))) return bail(rc);
-    if ((rc = dalloc(c, &c->pu, (size_t)c->nchunk * n))) return bail(rc);
+    if ((rc = dalloc(c, &c->pu, (size_t)slots * n))) return bail(rc);
     if ((rc = dalloc(c, &c->part, (size_t)4 * md_blocks(c->N)))) return bail(rc);
     if ((rc = dalloc(c, &c->sc, (size_t)SC_N))) return bail(rc);
     if ((rc = dalloc(c, &c->rdf, (size_t)3 * kHistStride))) return bail(rc);
@@ -564,7 +589,7 @@ int tcp_force_direct(tcp_ctx* c, double* Ur)
     float ms = 0;
     HIPCK(c, hipEventElapsedTime(&ms, c->ev_pair[0][0], c->ev_pair[1][0]));
     c->pair_ms = ms;
-    if (Ur) *Ur = 0.5 * u;
+    if (Ur) *Ur = c->ur_scale * u;
     return 0;
 }
 
@@ -736,7 +761,7 @@ int tcp_md_steps(tcp_ctx* c, int nsteps, int thermo, double dt, double Te, doubl
                        rec ? c->frz : nullptr, slot));  // main.f90:106, 136
         TRY(vv2(c, thermo, dt, Te, Ti, tau, fric, seed, c->sc + SC_FSUM));  // main.f90:141-153
         if (trace) {
-            launch_trace(c->trace + 6 * (size_t)s, c->sc + SC_USUM, 0.5, c->Uo,
+            launch_trace(c->trace + 6 * (size_t)s, c->sc + SC_USUM, c->ur_scale, c->Uo,
                          rec ? c->sc + SC_UM : nullptr, c->sc + SC_MV2, c->p.eps, c->stream);
             LAUNCHCK(c);
         }
