@@ -14,9 +14,10 @@ the step bracketed by a device synchronisation on both sides.
 Prints one JSON line.  value = pair-interactions/s of the mixed-precision path
 (N(N-1)/2 unordered pairs per step x steps/s); the FP64 path, their ratio and
 the RMS relative force error of the mixed path against FP64 are reported
-beside it.  roofline: the pair kernel's algorithmic FP32 flops (62.5 per
-ordered pair) over its average launch time from HIP events on its stream,
-against the 157.3 TFLOP/s FP32 vector peak.  cpu_baseline: the reference's own
+beside it.  roofline: the pair kernel's algorithmic FP32 flops (68.5 per
+unordered pair: 62.5 per pair plus 6 for the Newton's-third-law reaction) over
+its average launch time from HIP events on its stream, against the
+157.3 TFLOP/s FP32 vector peak.  cpu_baseline: the reference's own
 Force_direct_ext (compiled from source, oracle/_ref) timed on the host cores on
 a bounded row slice of the same workload.
 """
@@ -39,7 +40,10 @@ from tcp_allpair_amd.gpu import Context  # noqa: E402
 from tcp_allpair_amd.inputs import make_plasma  # noqa: E402
 from tcp_allpair_amd.params import FP64, MIXED, NOTEMP, reduced_dt  # noqa: E402
 
-FLOP_PER_ORDERED_PAIR = 62.5       # FMA = 2; exp, rsqrt, rcp = 1 each; e-i/e-e mix
+# Algorithmic FP32 flops: 62.5 per ordered pair (FMA = 2; exp, rsqrt, rcp, rint = 1;
+# the e-i / e-e mix), and 62.5 + 6 per unordered pair for a kernel that uses
+# Newton's third law (the reaction accumulate), which the mixed kernel does.
+FLOP_PER_UNORDERED_PAIR_NEWTON = 62.5 + 6.0
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pair_mixed_n131072_pmc.md")
 
@@ -138,7 +142,7 @@ def main():
     sps64 = args.fp64_steps / fp["wall_s"]
     tr = mx["trace"]
     etot = 0.5 * (tr[:, 4] + tr[:, 5]) + tr[:, 3] + tr[:, 2]
-    flops = N * (N - 1) * FLOP_PER_ORDERED_PAIR
+    flops = N * (N - 1) / 2 * FLOP_PER_UNORDERED_PAIR_NEWTON
     achieved = flops / (mx["pair_ms"] * 1e-3) / 1e12
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):   # measured by rocprofv3 --pmc, see that file
@@ -178,7 +182,7 @@ def main():
         if len(etot) > 1 else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
-                     "traffic": traffic, "kernel": "pair_mixed_kernel<true>",
+                     "traffic": traffic, "kernel": "pair_mixed_n3_kernel<true>",
                      "kernel_ms": mx["pair_ms"], "flop_per_launch": flops},
         "cpu_baseline": cpu,
     }

@@ -357,9 +357,13 @@ __device__ inline float rot_f(float v)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x13C, 0xF, 0xF, false));
 }
 
-template <bool kRdf, bool kDiag>
+// One tile pair (64 i x 64 j) by 64 rotation steps.  kUni: both tiles hold
+// one species and only live particles (all but the tiles at the electron/ion
+// boundary and at the end), so species, charges and liveness are
+// wave-uniform and the flags do not travel with j.
+template <bool kRdf, bool kDiag, bool kUni>
 __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* hist,
-                               const FixPos& pi, int si, bool ilive, FixPos pj,
+                               const FixPos& pi, int si, bool ilive, int sj_uni, FixPos pj,
                                float& fxo, float& fyo, float& fzo, float& uo,
                                float& rx, float& ry, float& rz)
 {
@@ -368,13 +372,26 @@ __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* h
     // RDF bin widths for a j electron / a j ion, seen from this i
     const float idx0 = si ? k.inv_dx[1] : k.inv_dx[0];
     const float idx1 = si ? k.inv_dx[2] : k.inv_dx[1];
+    const float qj_uni = (float)a.q[sj_uni];
+    const bool ei_uni = si != sj_uni;
     float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
     rx = ry = rz = 0.f;
     for (int s = 0; s < kTile; ++s) {
-        const int sj = (int)(pj.pad0 & 1u);
-        const bool jlive = (pj.pad0 & 2u) != 0u;
-        const float qj = pj.pad1;  // charge of j rides along with its coordinates
-        const MixPair o = mix_pair(k, pi, pj, qi * qj, eqi * qj, si != sj, ilive & jlive);
+        int sj;
+        float qj;
+        bool live, ei;
+        if (kUni) {
+            sj = sj_uni;
+            qj = qj_uni;
+            live = true;
+            ei = ei_uni;
+        } else {
+            sj = (int)(pj.pad0 & 1u);
+            qj = pj.pad1;  // the charge of j rides along with its coordinates
+            live = ilive & ((pj.pad0 & 2u) != 0u);
+            ei = si != sj;
+        }
+        const MixPair o = mix_pair(k, pi, pj, qi * qj, eqi * qj, ei, live);
         const float gx = o.df * o.dx, gy = o.df * o.dy, gz = o.df * o.dz;
         fx += gx;
         fy += gy;
@@ -388,18 +405,20 @@ __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* h
             rz -= gz;
         }
         if (kRdf && o.in) {
-            const int ht = si + sj;
-            const int b = (int)(o.r * (sj ? idx1 : idx0));
-            if (b < kNbin) atomicAdd(&hist[ht * kHistStride + b], kDiag ? 1u : 2u);
+            // bins past Ndist+1 land in the padding bins 501..511, never read
+            const int b = min((int)(o.r * (sj ? idx1 : idx0)), kHistStride - 1);
+            atomicAdd(&hist[(si + sj) * kHistStride + b], kDiag ? 1u : 2u);
         }
         pj.ux = rot_u(pj.ux);
         pj.uy = rot_u(pj.uy);
         pj.uz = rot_u(pj.uz);
-        pj.pad0 = rot_u(pj.pad0);
         pj.rx = rot_f(pj.rx);
         pj.ry = rot_f(pj.ry);
         pj.rz = rot_f(pj.rz);
-        pj.pad1 = rot_f(pj.pad1);
+        if (!kUni) {
+            pj.pad0 = rot_u(pj.pad0);
+            pj.pad1 = rot_f(pj.pad1);
+        }
         if (!kDiag) {
             rx = rot_f(rx);
             ry = rot_f(ry);
@@ -445,16 +464,27 @@ __global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArg
         const bool ilive = ip < n;
         const FixPos pi = a.ui[min(ip, n - 1)];
         const int si = ip >= ne;
+        // the i tile holds one species and only live particles
+        const bool iuni = (I * kTile >= ne || I * kTile + kTile - 1 < ne) && I * kTile + kTile - 1 < n;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0, U = 0.0;
         for (int tj = dblk ? ti : 0; tj < kBlkTiles; ++tj) {
             const int J = bc * kBlkTiles + tj;
             if (J >= ntiles) break;
             float fx, fy, fz, u, rx, ry, rz;
             const FixPos pj = jpos[tj * kTile + lane];
+            const int sj_uni = J * kTile >= ne;
+            const bool uni = iuni && (J * kTile >= ne || J * kTile + kTile - 1 < ne) &&
+                             J * kTile + kTile - 1 < n;
             if (dblk && tj == ti) {
-                n3_tile<kRdf, true>(k, a, hist, pi, si, ilive, pj, fx, fy, fz, u, rx, ry, rz);
+                if (uni)
+                    n3_tile<kRdf, true, true>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                else
+                    n3_tile<kRdf, true, false>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
             } else {
-                n3_tile<kRdf, false>(k, a, hist, pi, si, ilive, pj, fx, fy, fz, u, rx, ry, rz);
+                if (uni)
+                    n3_tile<kRdf, false, true>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                else
+                    n3_tile<kRdf, false, false>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
                 const int q = tj * kTile + lane;
                 atomicAdd(&racc[0][q], (double)rx);
                 atomicAdd(&racc[1][q], (double)ry);

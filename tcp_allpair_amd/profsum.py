@@ -1,0 +1,88 @@
+"""Summaries of rocprofv3 CSV output for profiles/ (text, a few KB).
+
+    python -m tcp_allpair_amd.profsum stats <kernel_stats.csv> <out.md> [note]
+    python -m tcp_allpair_amd.profsum pmc <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv> <out.md>
+
+`stats` tabulates rocprofv3 --stats per kernel.  `pmc` scales FETCH_SIZE and
+WRITE_SIZE (KiB per dispatch) by what the same counters read for copy_probe's
+copy_kernel of known size (512 MiB in, 512 MiB out) and writes the HBM bytes
+per launch of each kernel of the step; the line `hbm_bytes_per_launch:` holds
+the pair kernel's figure that bench.py reports as roofline.traffic.
+"""
+from __future__ import annotations
+
+import collections
+import csv
+import sys
+
+PAIR_KERNEL = "pair_mixed_n3_kernel<true>"
+STATS_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_stats -o bench "
+             "-- python3 bench.py --steps 5 --warmup 1 --fp64-steps 2 --no-cpu-baseline")
+PMC_CMDS = [
+    "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o bench -- python3 bench.py "
+    "--steps 2 --warmup 0 --fp64-steps 1 --no-cpu-baseline",
+    "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o bench -- python3 bench.py "
+    "--steps 2 --warmup 0 --fp64-steps 1 --no-cpu-baseline",
+    "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_copy_fetch -o copy -- "
+    "tcp_allpair_amd/copy_probe 512 3",
+    "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_copy_write -o copy -- "
+    "tcp_allpair_amd/copy_probe 512 3",
+]
+
+
+def short(name: str) -> str:
+    name = name.replace("tcp::(anonymous namespace)::", "").replace("void ", "")
+    return name.split("(")[0]
+
+
+def stats(src: str, out: str, note: str = "") -> None:
+    rows = list(csv.DictReader(open(src)))
+    lines = ["# Kernel time summary: N = 131072 step, mixed and FP64 paths\n\n",
+             "Workload: 65536 e + 65536 i, L = 50.797 A, SPME 128^3 order 6, RDF/VDF on; bench.py with "
+             "1 warm-up and 5 timed mixed steps, 2 FP64 steps, and one initial force pass per precision.\n\n",
+             "Command (repository root on the GPU box, TMPDIR=/tmp):\n\n    " + STATS_CMD + "\n\n",
+             "| kernel | calls | average ms | total ms | % |\n|---|---|---|---|---|\n"]
+    for r in rows[:18]:
+        lines.append("| %s | %s | %.4f | %.2f | %.2f |\n" % (
+            short(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e6,
+            float(r["TotalDurationNs"]) / 1e6, float(r["Percentage"])))
+    if note:
+        lines.append("\n" + note + "\n")
+    open(out, "w").write("".join(lines))
+
+
+def _agg(path: str) -> dict:
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        d[short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in d.items()}
+
+
+def pmc(fetch: str, write: str, cfetch: str, cwrite: str, out: str) -> None:
+    fe, wr, cf, cw = _agg(fetch), _agg(write), _agg(cfetch), _agg(cwrite)
+    known = 512 * 1024  # KiB moved each way by copy_kernel
+    ffac, wfac = known / cf["copy_kernel"], known / cw["copy_kernel"]
+    lines = ["# HBM traffic per launch (rocprofv3 --pmc), N = 131072 step\n\n",
+             "Commands (repository root on the GPU box, one counter per run):\n\n"]
+    lines += ["    " + c + "\n" for c in PMC_CMDS]
+    lines.append("\nCalibration: copy_kernel moves %d KiB in and %d KiB out per launch; FETCH_SIZE reads "
+                 "%.1f KiB (scale x%.3f), WRITE_SIZE reads %.1f KiB (scale x%.3f).\n\n"
+                 % (known, known, cf["copy_kernel"], ffac, cw["copy_kernel"], wfac))
+    lines.append("| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | scaled HBM bytes per launch |\n"
+                 "|---|---|---|---|\n")
+    res = {}
+    for k in sorted(fe):
+        if any(s in k for s in ("pair_", "spread", "gather", "fft_pass", "combine", "bin_", "grid_")):
+            tot = (fe[k] * ffac + wr.get(k, 0.0) * wfac) * 1024
+            res[k] = tot
+            lines.append("| %s | %.1f | %.1f | %.3e |\n" % (k, fe[k], wr.get(k, 0.0), tot))
+    lines.append("\nhbm_bytes_per_launch: %.6e\n(%s; bench.py reports this line as roofline.traffic)\n"
+                 % (res[PAIR_KERNEL], PAIR_KERNEL))
+    open(out, "w").write("".join(lines))
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "stats":
+        stats(sys.argv[2], sys.argv[3], sys.argv[4] if len(sys.argv) > 4 else "")
+    else:
+        pmc(*sys.argv[2:7])
