@@ -96,8 +96,10 @@ __global__ __launch_bounds__(256) void spread_kernel(RecipArgs a)
 template <int P>
 __global__ __launch_bounds__(256) void gather_kernel(RecipArgs a)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= a.n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    // in brick order neighbouring threads read neighbouring grid rows
+    const int m = a.nbrick[0] ? a.sorted[t] : t;
     const double q = a.q[m >= a.ne];
     const double pos[3] = {a.x[m], a.y[m], a.z[m]};
     double w[3][P], dw[3][P];
@@ -133,6 +135,155 @@ __global__ __launch_bounds__(256) void gather_kernel(RecipArgs a)
     a.frx[m] = pre * sx * ((double)K1 / a.box[0]);
     a.fry[m] = pre * sy * ((double)K2 / a.box[1]);
     a.frz[m] = pre * sz * ((double)K3 / a.box[2]);
+}
+
+// ------------------------------------------------- brick-binned spreading ---
+//
+// Device-scope FP64 atomics resolve beyond the per-XCD L2, so spreading p^3
+// atomics per particle straight into the grid moves ~32 B of memory traffic
+// per update.  Instead particles are counting-sorted by the brick (kBrick^3
+// grid points) that holds their first grid point; one block per brick spreads
+// its particles into an LDS tile of (kBrick + p - 1)^3 points (the brick plus
+// the halo its splines reach), writes the tile out, and a gather over the
+// tiles that cover each grid point (up to 3 per axis) forms Q in a fixed
+// order.
+
+__device__ inline int brick_axis(double u, int P, int K)
+{
+    const int b0 = wrap((int)floor(u) - P / 2 + 1, K);
+    return b0 / kBrick;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void bin_count_kernel(RecipArgs a)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.n) return;
+    const double pos[3] = {a.x[m], a.y[m], a.z[m]};
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double u = (double)a.K[d] * (pos[d] + a.box[d] * 0.5) / a.box[d];
+        b[d] = brick_axis(u, P, a.K[d]);
+    }
+    const int id = (b[2] * a.nbrick[1] + b[1]) * a.nbrick[0] + b[0];
+    a.brick_of[m] = id;
+    atomicAdd(a.brick_count + id, 1);
+}
+
+// exclusive scan of the brick counts (one block), cursors reset
+__global__ __launch_bounds__(1024) void bin_scan_kernel(RecipArgs a, int nb)
+{
+    __shared__ int part[1024];
+    const int per = (nb + 1023) / 1024;
+    const int lo = threadIdx.x * per, hi = min(nb, lo + per);
+    int s = 0;
+    for (int b = lo; b < hi; ++b) s += a.brick_count[b];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (int b = lo; b < hi; ++b) {
+        a.brick_start[b] = run;
+        run += a.brick_count[b];
+        a.brick_cursor[b] = 0;
+    }
+    if (threadIdx.x == 1023) a.brick_start[nb] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void bin_scatter_kernel(RecipArgs a)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.n) return;
+    const int id = a.brick_of[m];
+    a.sorted[a.brick_start[id] + atomicAdd(a.brick_cursor + id, 1)] = m;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void spread_brick_kernel(RecipArgs a)
+{
+    constexpr int E = kBrick + P - 1;      // tile edge
+    constexpr int kBatch = 64;             // particles staged per pass
+    __shared__ double tile[E * E * E];
+    __shared__ double ws[kBatch][3][P];
+    __shared__ int org[kBatch][3];
+    const int bid = blockIdx.x;
+    const int bx = bid % a.nbrick[0], by = (bid / a.nbrick[0]) % a.nbrick[1],
+              bz = bid / (a.nbrick[0] * a.nbrick[1]);
+    for (int t = threadIdx.x; t < E * E * E; t += blockDim.x) tile[t] = 0.0;
+    const int p0 = a.brick_start[bid], p1 = a.brick_start[bid + 1];
+    const int borg[3] = {bx * kBrick, by * kBrick, bz * kBrick};
+    for (int q0 = p0; q0 < p1; q0 += kBatch) {
+        const int nq = min(kBatch, p1 - q0);
+        __syncthreads();
+        if ((int)threadIdx.x < nq) {
+            const int m = a.sorted[q0 + threadIdx.x];
+            const double q = a.q[m >= a.ne];
+            const double pos[3] = {a.x[m], a.y[m], a.z[m]};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double u = (double)a.K[d] * (pos[d] + a.box[d] * 0.5) / a.box[d];
+                const double fl = floor(u);
+                double w[P], dw[P];
+                bspline<P>(u - fl, w, dw);
+                // offset of the first grid point inside the tile, in [0, kBrick)
+                org[threadIdx.x][d] = wrap((int)fl - P / 2 + 1, a.K[d]) - borg[d];
+#pragma unroll
+                for (int i = 0; i < P; ++i) ws[threadIdx.x][d][i] = d == 0 ? q * w[i] : w[i];
+            }
+        }
+        __syncthreads();
+        // one work item = one particle and one (j, k) row of its support
+        for (int it = threadIdx.x; it < nq * P * P; it += blockDim.x) {
+            const int qq = it / (P * P), jk = it % (P * P), j = jk % P, k = jk / P;
+            const double wyz = ws[qq][1][j] * ws[qq][2][k];
+            double* row = tile + ((org[qq][2] + k) * E + org[qq][1] + j) * E + org[qq][0];
+#pragma unroll
+            for (int i = 0; i < P; ++i) atomicAdd(row + i, ws[qq][0][i] * wyz);
+        }
+    }
+    __syncthreads();
+    double* out = a.tiles + (size_t)bid * (E * E * E);
+    for (int t = threadIdx.x; t < E * E * E; t += blockDim.x) out[t] = tile[t];
+}
+
+// Q(g) = sum over the tiles covering g, in a fixed order (x fastest grid index)
+template <int P>
+__global__ __launch_bounds__(256) void grid_from_tiles_kernel(RecipArgs a)
+{
+    constexpr int E = kBrick + P - 1;
+    const size_t G = (size_t)a.K[0] * a.K[1] * a.K[2];
+    const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const int gx = (int)(g % a.K[0]), gy = (int)((g / a.K[0]) % a.K[1]),
+              gz = (int)(g / ((size_t)a.K[0] * a.K[1]));
+    const int gg[3] = {gx, gy, gz};
+    // tile (b', l) covers g when b' kBrick + l = g (mod K) and l < E: b' = b - c
+    // for c = 0, 1, .. as long as o + c kBrick < E (up to 3 for order 12)
+    int nb[3], bs[3][3], ls[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int b = gg[d] / kBrick, o = gg[d] % kBrick;
+        nb[d] = 0;
+        for (int c = 0; c < 3 && o + c * kBrick < E; ++c) {
+            bs[d][c] = ((b - c) % a.nbrick[d] + a.nbrick[d]) % a.nbrick[d];
+            ls[d][c] = o + c * kBrick;
+            nb[d] = c + 1;
+        }
+    }
+    double s = 0.0;
+    for (int cz = 0; cz < nb[2]; ++cz)
+        for (int cy = 0; cy < nb[1]; ++cy)
+            for (int cx = 0; cx < nb[0]; ++cx) {
+                const int bid = (bs[2][cz] * a.nbrick[1] + bs[1][cy]) * a.nbrick[0] + bs[0][cx];
+                s += a.tiles[(size_t)bid * (E * E * E) + (ls[2][cz] * E + ls[1][cy]) * E + ls[0][cx]];
+            }
+    a.Q[g] = s;
 }
 
 // -------------------------------------------------------------------- FFT ---
@@ -288,6 +439,12 @@ void run_pass(const FftPass& f, int lines, hipStream_t s)
     hipLaunchKernelGGL(fft_pass_kernel<kMode>, dim3(blocks), dim3(kFftThreads), 0, s, f);
 }
 
+__global__ void fill_int_kernel(int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 __global__ void fill_kernel(double* p, size_t n, double v)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -391,8 +548,30 @@ void launch_fill(double* p, size_t n, double v, hipStream_t s)
     hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, s, p, n, v);
 }
 
+template <int P>
+void spread_binned(const RecipArgs& a, hipStream_t s)
+{
+    const int nb = a.nbrick[0] * a.nbrick[1] * a.nbrick[2];
+    const size_t G = (size_t)a.K[0] * a.K[1] * a.K[2];
+    hipLaunchKernelGGL(fill_int_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, a.brick_count, nb);
+    hipLaunchKernelGGL(bin_count_kernel<P>, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, s, a, nb);
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(spread_brick_kernel<P>, dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(grid_from_tiles_kernel<P>, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, a);
+}
+
 void launch_spme_spread(const RecipArgs& a, hipStream_t s)
 {
+    if (a.nbrick[0]) {
+        switch (a.order) {
+            case 4: spread_binned<4>(a, s); return;
+            case 6: spread_binned<6>(a, s); return;
+            case 8: spread_binned<8>(a, s); return;
+            case 10: spread_binned<10>(a, s); return;
+            default: spread_binned<12>(a, s); return;
+        }
+    }
     const size_t G = (size_t)a.K[0] * a.K[1] * a.K[2];
     launch_fill(a.Q, G, 0.0, s);
     dim3 grid((a.n + 255) / 256), blk(256);

@@ -59,6 +59,10 @@ struct tcp_ctx {
     double2* tw[3] = {nullptr, nullptr, nullptr};
     double *frx = nullptr, *fry = nullptr, *frz = nullptr;
     int nepart = 0;
+    int nbrick[3] = {0, 0, 0};
+    int *brick_of = nullptr, *brick_count = nullptr, *brick_start = nullptr;
+    int *brick_cursor = nullptr, *sorted = nullptr;
+    double* tiles = nullptr;
     // reductions and histograms
     double* part = nullptr;
     double* sc = nullptr;
@@ -157,6 +161,13 @@ RecipArgs recip_args(tcp_ctx* c)
     a.epart = c->epart;
     a.nepart = c->nepart;
     a.frx = c->frx; a.fry = c->fry; a.frz = c->frz;
+    for (int d = 0; d < 3; ++d) a.nbrick[d] = c->nbrick[d];
+    a.brick_of = c->brick_of;
+    a.brick_count = c->brick_count;
+    a.brick_start = c->brick_start;
+    a.brick_cursor = c->brick_cursor;
+    a.sorted = c->sorted;
+    a.tiles = c->tiles;
     return a;
 }
 
@@ -358,6 +369,17 @@ int setup_recip(tcp_ctx* c)
     TRY(dalloc(c, &c->A, G));
     c->nepart = fft_epart_blocks(K);
     TRY(dalloc(c, &c->epart, (size_t)c->nepart));
+    if (K[0] % kBrick == 0 && K[1] % kBrick == 0 && K[2] % kBrick == 0) {
+        for (int d = 0; d < 3; ++d) c->nbrick[d] = K[d] / kBrick;
+        const int nb = c->nbrick[0] * c->nbrick[1] * c->nbrick[2];
+        const size_t E = kBrick + c->p.order - 1;
+        TRY(dalloc(c, &c->brick_of, (size_t)c->N));
+        TRY(dalloc(c, &c->sorted, (size_t)c->N));
+        TRY(dalloc(c, &c->brick_count, (size_t)nb));
+        TRY(dalloc(c, &c->brick_start, (size_t)nb + 1));
+        TRY(dalloc(c, &c->brick_cursor, (size_t)nb));
+        TRY(dalloc(c, &c->tiles, (size_t)nb * E * E * E));
+    }
     // influence function PSI (force.f90:27-42, PSI(1,1,1) = 0) times the spline
     // modulus |b1 b2 b3|^2 (force.f90:834-839 for order 4), built on the host in FP64
     std::vector<double> bm[3];

@@ -81,7 +81,17 @@ struct RecipArgs {
     double* frx;               // reciprocal forces (SoA)
     double* fry;
     double* frz;
+    // brick-binned spreading (used when every K_d is a multiple of kBrick)
+    int nbrick[3];             // bricks per axis (0: unbinned atomic spreading)
+    int* brick_of;             // [n] brick of each particle's first grid point
+    int* brick_count;          // [nbrick]
+    int* brick_start;          // [nbrick + 1]
+    int* brick_cursor;         // [nbrick]
+    int* sorted;               // [n] particle indices grouped by brick
+    double* tiles;             // [nbrick][(kBrick + P - 1)^3] per-brick grid tiles
 };
+
+constexpr int kBrick = 8;      // grid points per brick edge
 
 void launch_spme_spread(const RecipArgs& a, hipStream_t s);
 void launch_fft3d_forward_convolve(const RecipArgs& a, hipStream_t s);
