@@ -33,6 +33,11 @@ enum { TCP_FP64 = 0, TCP_MIXED = 1 };
 enum { TCP_NOTEMP = 0, TCP_BEREND = 1, TCP_ISOKIN = 2, TCP_STOCH = 3 };
 /* reciprocal-space method */
 enum { TCP_RECIP_NONE = 0, TCP_RECIP_SPME = 1, TCP_RECIP_EWALD = 2 };
+/* short-range regularisation (tcp_params.flags).  The reference adds
+ * -exp(-r/lambda)/r to electron-ion pairs only (Deutsch form, force.f90:136-139).
+ * TCP_KELBG replaces it by the Kelbg form -exp(-r^2/lambda^2)/r + sqrt(pi) erfc(r/lambda)/lambda;
+ * TCP_REG_EE applies the regularisation to electron-electron pairs too. */
+enum { TCP_KELBG = 1, TCP_REG_EE = 2 };
 
 /* error codes */
 enum {
@@ -51,7 +56,7 @@ typedef struct tcp_params {
     int32_t order;           /* B-spline order, even, 4..12 (4 = Force_SPME4) */
     int32_t kmax;            /* direct Ewald k-sum extent for TCP_RECIP_EWALD */
     int32_t recip;           /* TCP_RECIP_* */
-    int32_t pad_;
+    int32_t flags;           /* TCP_KELBG | TCP_REG_EE; 0 = the reference's model */
     double box[3];           /* param%box (A) */
     double q[2];             /* param%q (e) */
     double xm[2];            /* param%xm (amu) */

@@ -91,8 +91,11 @@ static inline double min_image(const double* box, const double* xi, const double
 }
 
 /* One pair inside the cut-off, force.f90:99-110 (bare) and 130-144 (Deutsch e-i).
- * Returns df (eps included) and adds the energy term q1 q2 (erfc - y)/r to *U. */
-static inline double pair_term(const tcpo_params* p, double a, double q1, double q2, int ei,
+ * Returns df (eps included) and adds the energy term q1 q2 (erfc + R)/r to *U.
+ * reg selects the regularised form: R = -exp(-r/lambda) (Deutsch, the reference)
+ * or, with TCPO_KELBG, R = -exp(-s^2) + sqrt(pi) s erfc(s), s = r/lambda, for
+ * which -d/dr[(erfc + R)/r] = (g + erfc - exp(-s^2))/r^2 since dR/dr = sqrt(pi) erfc(s)/lambda. */
+static inline double pair_term(const tcpo_params* p, double a, double q1, double q2, int reg,
                                double r2, double* U, double* rdf_bin, double dx)
 {
     double r = sqrt(r2);
@@ -100,9 +103,15 @@ static inline double pair_term(const tcpo_params* p, double a, double q1, double
     if (idx <= NBIN) rdf_bin[idx - 1] += 2.0;
     double x = 1.0 - erf(a * r);
     double g = 2.0 * exp(-(a * a * r2)) * a * r / SQRTPI;
-    if (!ei) {
+    if (!reg) {
         *U = *U + q1 * q2 * x / r;
         return p->eps * q1 * q2 * (g + x) / r2 / r;
+    }
+    if (p->flags & TCPO_KELBG) {
+        double s = r / p->rs;
+        double y2 = exp(-(s * s));
+        *U = *U + q1 * q2 * (x - y2 + SQRTPI * s * erfc(s)) / r;
+        return p->eps * q1 * q2 * (g + x - y2) / r2 / r;
     }
     double y = exp(-r / p->rs);
     *U = *U + q1 * q2 * (x - y) / r;
@@ -127,7 +136,8 @@ int tcpo_force_direct_self(const tcpo_params* p, double alpha, double rc2,
         for (int j = i + 1; j < ne; ++j) {
             double r2 = min_image(p->box, xe + 3 * i, xe + 3 * j, xr);
             if (r2 < rc2) {
-                double df = pair_term(p, alpha, p->q[0], p->q[0], 0, r2, &U, h, p->dx_ee);
+                double df = pair_term(p, alpha, p->q[0], p->q[0], (p->flags & TCPO_REG_EE) != 0, r2,
+                                      &U, h, p->dx_ee);
                 for (int k = 0; k < 3; ++k) {
                     fe[3 * i + k] = fe[3 * i + k] + df * xr[k];
                     fe[3 * j + k] = fe[3 * j + k] - df * xr[k];
@@ -181,7 +191,7 @@ int tcpo_force_direct_ext(const tcpo_params* p, double alpha, double rc2,
     memset(gemit, 0, sizeof(double) * 3 * (size_t)(ne_g + ni_g));
     double U = 0.0, xr[3];
     struct { const double *a, *b; double *fa, *fb; int na, nb, sa, sb, ei, hist; double dx; } blk[4] = {
-        {xe, ge, fe, me, ne_o, ne_g, 0, 0, 0, 0, p->dx_ee}, /* e-e_g :213-241 */
+        {xe, ge, fe, me, ne_o, ne_g, 0, 0, (p->flags & TCPO_REG_EE) != 0, 0, p->dx_ee}, /* e-e_g :213-241 */
         {xe, gi, fe, mi, ne_o, ni_g, 0, 1, 1, 1, p->dx_ei}, /* e-i_g :244-275 */
         {xi, ge, fi, me, ni_o, ne_g, 1, 0, 1, 1, p->dx_ei}, /* i-e_g :278-309 */
         {xi, gi, fi, mi, ni_o, ni_g, 1, 1, 0, 2, p->dx_ii}, /* i-i_g :312-341 */
@@ -254,9 +264,15 @@ int tcpo_force_allpair(const tcpo_params* p, double alpha, double rc2,
                 double xx = 1.0 - erf(alpha * r);
                 double g = 2.0 * exp(-(alpha * alpha * r2)) * alpha * r / SQRTPI;
                 double q1 = p->q[si], q2 = p->q[sj], df;
-                if (si == sj) {
+                const int reg = si != sj || (si == 0 && sj == 0 && (p->flags & TCPO_REG_EE));
+                if (!reg) {
                     U = q1 * q2 * xx / r;
                     df = p->eps * q1 * q2 * (g + xx) / r2 / r;
+                } else if (p->flags & TCPO_KELBG) {
+                    double s = r / p->rs;
+                    double y2 = exp(-(s * s));
+                    U = q1 * q2 * (xx - y2 + SQRTPI * s * erfc(s)) / r;
+                    df = p->eps * q1 * q2 * (g + xx - y2) / r2 / r;
                 } else {
                     double y = exp(-r / p->rs);
                     U = q1 * q2 * (xx - y) / r;

@@ -21,6 +21,9 @@ extern "C" {
 
 enum { TCPO_NOTEMP = 0, TCPO_BEREND = 1, TCPO_ISOKIN = 2, TCPO_STOCH = 3 };
 enum { TCPO_RECIP_NONE = 0, TCPO_RECIP_SPME = 1, TCPO_RECIP_EWALD = 2 };
+/* regularisation options with no reference counterpart: Kelbg instead of the
+ * Deutsch form, and the regularisation applied to electron-electron pairs */
+enum { TCPO_KELBG = 1, TCPO_REG_EE = 2 };
 
 typedef struct tcpo_params {
     int ne, ni;          /* NS%NTel, NS%NTion */
@@ -28,7 +31,7 @@ typedef struct tcpo_params {
     int order;           /* B-spline order (4 = the reference's Force_SPME4) */
     int kmax;            /* direct Ewald k-sum extent for TCPO_RECIP_EWALD */
     int recip;           /* TCPO_RECIP_* */
-    int pad_;
+    int flags;           /* TCPO_KELBG | TCPO_REG_EE, 0 = the reference's model */
     double box[3];       /* param%box */
     double q[2];         /* param%q */
     double xm[2];        /* param%xm */

@@ -19,6 +19,7 @@ namespace tcp {
 namespace {
 
 constexpr double kSqrtPi = 1.7724538509055159;  // data.f90:24
+constexpr float kSqrtPiF = 1.7724538509055159f;
 
 __device__ inline void hist_clear(unsigned* h)
 {
@@ -61,7 +62,9 @@ __global__ __launch_bounds__(kPairBlock) void pair_f64_kernel(PairArgs a)
         const int hi = sj ? j1 : min(j1, a.ne_j);
         const double qq = a.q[si] * a.q[sj];
         const double eqq = a.eps * qq;
-        const bool ei = si != sj;
+        // regularised pair: electron-ion, and electron-electron with TCP_REG_EE
+        const bool reg = si != sj || (si == 0 && sj == 0 && (a.flags & TCP_REG_EE));
+        const bool kelbg = (a.flags & TCP_KELBG) != 0;
         const int ht = si + sj;  // 0 ee, 1 ei, 2 ii
         const double dxb = a.dx[ht];
 #pragma unroll 2
@@ -91,10 +94,17 @@ __global__ __launch_bounds__(kPairBlock) void pair_f64_kernel(PairArgs a)
                 const double ec = erfc(alpha * r);
                 const double g = two_a_sqrtpi * r * exp(-(a2 * r2));
                 double num = g + ec, en = ec;
-                if (ei) {
-                    const double y = exp(-r * inv_rs);
-                    num -= y * (1.0 + r * inv_rs);
-                    en -= y;
+                if (reg) {
+                    const double s = r * inv_rs;
+                    if (kelbg) {  // R = -e^{-s^2} + sqrt(pi) s erfc(s), r dR/dr = sqrt(pi) s erfc(s)
+                        const double y2 = exp(-(s * s));
+                        num -= y2;
+                        en += kSqrtPi * s * erfc(s) - y2;
+                    } else {      // R = -e^{-s} (Deutsch, force.f90:136-139)
+                        const double y = exp(-s);
+                        num -= y * (1.0 + s);
+                        en -= y;
+                    }
                 }
                 const double df = eqq * num * inv_r2 * inv_r;
                 u += qq * en * inv_r;
@@ -194,6 +204,48 @@ __device__ inline void close_ei(float r, float s, float x, float y, float ec, fl
     (void)r;
 }
 
+// Close pair with the Kelbg form (s = r/lambda < 1): num = B(s^2) - A(x) with
+// B(z) = 1 - e^-z = z (1 - z/2 + z^2/6 - ...) (Taylor to z^9, relative error
+// < 3e-8 for z < 1) and A as in close_ei; en = B - erf(x) + sqrt(pi) s erfc(s).
+__device__ inline void close_kelbg(float s, float x, float erfc_s, float g, float ec, float* num,
+                                   float* en)
+{
+    const float zs = s * s;
+    float b = -2.755731922e-07f;                  // -1/10!
+    b = fmaf(b, zs, 2.755731922e-06f);            // 1/9!
+    b = fmaf(b, zs, -2.480158730e-05f);
+    b = fmaf(b, zs, 1.984126984e-04f);
+    b = fmaf(b, zs, -1.388888889e-03f);
+    b = fmaf(b, zs, 8.333333333e-03f);
+    b = fmaf(b, zs, -4.166666667e-02f);
+    b = fmaf(b, zs, 1.666666667e-01f);
+    b = fmaf(b, zs, -5.000000000e-01f);
+    b = fmaf(b, zs, 1.0f);
+    const float B = b * zs;
+    const float z = x * x;
+    float pa = -2.633938299e-05f;
+    pa = fmaf(pa, z, 2.089591050e-04f);
+    pa = fmaf(pa, z, -1.446639958e-03f);
+    pa = fmaf(pa, z, 8.548327023e-03f);
+    pa = fmaf(pa, z, -4.179182100e-02f);
+    pa = fmaf(pa, z, 1.611970239e-01f);
+    pa = fmaf(pa, z, -4.513516668e-01f);
+    pa = fmaf(pa, z, 7.522527781e-01f);
+    float pe = -1.492565036e-05f;
+    pe = fmaf(pe, z, 1.205533298e-04f);
+    pe = fmaf(pe, z, -8.548327023e-04f);
+    pe = fmaf(pe, z, 5.223977625e-03f);
+    pe = fmaf(pe, z, -2.686617065e-02f);
+    pe = fmaf(pe, z, 1.128379167e-01f);
+    pe = fmaf(pe, z, -3.761263890e-01f);
+    pe = fmaf(pe, z, 1.128379167e+00f);
+    const bool series = x < 0.6f;
+    const float A = series ? pa * z * x : 1.0f - ec - g;
+    const float erfx = series ? pe * x : 1.0f - ec;
+    *num = B - A;
+    *en = fmaf(kSqrtPiF * s, erfc_s, B - erfx);
+}
+
 constexpr int kMixTile = 32;
 
 // Loop-invariant FP32 constants of the mixed pair arithmetic.
@@ -205,8 +257,9 @@ struct MixConst {
     float two_a_sqrtpi;        // 2 alpha/sqrt(pi)
     float inv_rs;              // 1/lambda
     float nl2rs;               // -log2(e)/lambda
-    float rclose;              // 2 lambda
+    float rclose;              // 2 lambda (Deutsch) or lambda (Kelbg)
     float inv_dx[3];           // RDF bin widths (reciprocal)
+    bool kelbg;                // Kelbg instead of Deutsch regularisation
 };
 
 __device__ inline MixConst mix_const(const PairArgs& a)
@@ -221,7 +274,8 @@ __device__ inline MixConst mix_const(const PairArgs& a)
     k.two_a_sqrtpi = (float)(2.0 * a.alpha / kSqrtPi);
     k.inv_rs = (float)(1.0 / a.rs);
     k.nl2rs = (float)(-1.4426950408889634 / a.rs);
-    k.rclose = (float)(2.0 * a.rs);
+    k.kelbg = (a.flags & TCP_KELBG) != 0;
+    k.rclose = (float)((k.kelbg ? 1.0 : 2.0) * a.rs);
     for (int t = 0; t < 3; ++t) k.inv_dx[t] = (float)a.inv_dx[t];
     return k;
 }
@@ -233,8 +287,9 @@ struct MixPair {
     bool in;
 };
 
+template <bool kKelbg>
 __device__ inline MixPair mix_pair(const MixConst& k, const FixPos& pi, const FixPos& pj,
-                                   float qq, float eqq, bool ei, bool live)
+                                   float qq, float eqq, bool reg, bool live)
 {
     MixPair o;
     o.dx = fmaf((float)(int)(pi.ux - pj.ux), k.sx, pi.rx - pj.rx);
@@ -250,13 +305,23 @@ __device__ inline MixPair mix_pair(const MixConst& k, const FixPos& pi, const Fi
     const float ec = e2 * erfcx_poly(px * t);
     float num = fmaf(k.two_a_sqrtpi * r, e2, ec);
     float en = ec;
-    if (ei) {
-        const float y = __builtin_amdgcn_exp2f(k.nl2rs * r);
+    if (reg) {
         const float sr = r * k.inv_rs;
-        num = fmaf(-y, sr + 1.0f, num);
-        en -= y;
-        if (o.in && r < k.rclose)
-            close_ei(r, sr, 2.0f * px, y, ec, k.two_a_sqrtpi * r * e2, &num, &en);
+        if (kKelbg) {
+            const float y2 = __builtin_amdgcn_exp2f(-1.4426950408889634f * sr * sr);
+            const float hs = 0.5f * sr;
+            const float ecs = y2 * erfcx_poly(hs * __builtin_amdgcn_rcpf(hs + 1.0f));  // erfc(s)
+            num -= y2;
+            en = fmaf(kSqrtPiF * sr, ecs, en - y2);
+            if (o.in && r < k.rclose)
+                close_kelbg(sr, 2.0f * px, ecs, k.two_a_sqrtpi * r * e2, ec, &num, &en);
+        } else {
+            const float y = __builtin_amdgcn_exp2f(k.nl2rs * r);
+            num = fmaf(-y, sr + 1.0f, num);
+            en -= y;
+            if (o.in && r < k.rclose)
+                close_ei(r, sr, 2.0f * px, y, ec, k.two_a_sqrtpi * r * e2, &num, &en);
+        }
     }
     const float rinv2 = rinv * rinv;
     o.df = o.in ? eqq * num * rinv2 * rinv : 0.f;
@@ -265,7 +330,7 @@ __device__ inline MixPair mix_pair(const MixConst& k, const FixPos& pi, const Fi
     return o;
 }
 
-template <bool kRdf>
+template <bool kRdf, bool kKelbg>
 __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
 {
     __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
@@ -287,14 +352,14 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
         const int hi = sj ? j1 : min(j1, a.ne_j);
         const float qq = (float)(a.q[si] * a.q[sj]);
         const float eqq = (float)(a.eps * a.q[si] * a.q[sj]);
-        const bool ei = si != sj;
+        const bool reg = si != sj || (si == 0 && sj == 0 && (a.flags & TCP_REG_EE));
         const int ht = si + sj;
         for (int jt = lo; jt < hi; jt += kMixTile) {
             const int je = min(hi, jt + kMixTile);
             float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
 #pragma unroll 4
             for (int j = jt; j < je; ++j) {
-                const MixPair o = mix_pair(k, pi, a.uj[j], qq, eqq, ei, live);
+                const MixPair o = mix_pair<kKelbg>(k, pi, a.uj[j], qq, eqq, reg, live);
                 fx = fmaf(o.df, o.dx, fx);
                 fy = fmaf(o.df, o.dy, fy);
                 fz = fmaf(o.df, o.dz, fz);
@@ -361,7 +426,7 @@ __device__ inline float rot_f(float v)
 // one species and only live particles (all but the tiles at the electron/ion
 // boundary and at the end), so species, charges and liveness are
 // wave-uniform and the flags do not travel with j.
-template <bool kRdf, bool kDiag, bool kUni>
+template <bool kRdf, bool kKelbg, bool kDiag, bool kUni>
 __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* hist,
                                const FixPos& pi, int si, bool ilive, int sj_uni, FixPos pj,
                                float& fxo, float& fyo, float& fzo, float& uo,
@@ -373,25 +438,26 @@ __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* h
     const float idx0 = si ? k.inv_dx[1] : k.inv_dx[0];
     const float idx1 = si ? k.inv_dx[2] : k.inv_dx[1];
     const float qj_uni = (float)a.q[sj_uni];
-    const bool ei_uni = si != sj_uni;
+    const bool regee = (a.flags & TCP_REG_EE) != 0;
+    const bool reg_uni = si != sj_uni || (regee && si == 0 && sj_uni == 0);
     float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
     rx = ry = rz = 0.f;
     for (int s = 0; s < kTile; ++s) {
         int sj;
         float qj;
-        bool live, ei;
+        bool live, reg;
         if (kUni) {
             sj = sj_uni;
             qj = qj_uni;
             live = true;
-            ei = ei_uni;
+            reg = reg_uni;
         } else {
             sj = (int)(pj.pad0 & 1u);
             qj = pj.pad1;  // the charge of j rides along with its coordinates
             live = ilive & ((pj.pad0 & 2u) != 0u);
-            ei = si != sj;
+            reg = si != sj || (regee && si == 0 && sj == 0);
         }
-        const MixPair o = mix_pair(k, pi, pj, qi * qj, eqi * qj, ei, live);
+        const MixPair o = mix_pair<kKelbg>(k, pi, pj, qi * qj, eqi * qj, reg, live);
         const float gx = o.df * o.dx, gy = o.df * o.dy, gz = o.df * o.dz;
         fx += gx;
         fy += gy;
@@ -431,7 +497,7 @@ __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* h
     uo = u;
 }
 
-template <bool kRdf>
+template <bool kRdf, bool kKelbg>
 __global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArgs a,
                                                                          const int2* blkmap)
 {
@@ -477,14 +543,14 @@ __global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArg
                              J * kTile + kTile - 1 < n;
             if (dblk && tj == ti) {
                 if (uni)
-                    n3_tile<kRdf, true, true>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                    n3_tile<kRdf, kKelbg, true, true>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
                 else
-                    n3_tile<kRdf, true, false>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                    n3_tile<kRdf, kKelbg, true, false>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
             } else {
                 if (uni)
-                    n3_tile<kRdf, false, true>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                    n3_tile<kRdf, kKelbg, false, true>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
                 else
-                    n3_tile<kRdf, false, false>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                    n3_tile<kRdf, kKelbg, false, false>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
                 const int q = tj * kTile + lane;
                 atomicAdd(&racc[0][q], (double)rx);
                 atomicAdd(&racc[1][q], (double)ry);
@@ -544,21 +610,28 @@ int n3_block_rows(int n)
 
 void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, hipStream_t s)
 {
-    if (a.rdf_weight)
-        hipLaunchKernelGGL(pair_mixed_n3_kernel<true>, dim3(nblk), dim3(kN3Waves * kTile), 0, s, a,
-                           blkmap);
-    else
-        hipLaunchKernelGGL(pair_mixed_n3_kernel<false>, dim3(nblk), dim3(kN3Waves * kTile), 0, s, a,
-                           blkmap);
+    const dim3 g(nblk), b(kN3Waves * kTile);
+    const bool kel = (a.flags & TCP_KELBG) != 0;
+    if (a.rdf_weight) {
+        if (kel) hipLaunchKernelGGL((pair_mixed_n3_kernel<true, true>), g, b, 0, s, a, blkmap);
+        else hipLaunchKernelGGL((pair_mixed_n3_kernel<true, false>), g, b, 0, s, a, blkmap);
+    } else {
+        if (kel) hipLaunchKernelGGL((pair_mixed_n3_kernel<false, true>), g, b, 0, s, a, blkmap);
+        else hipLaunchKernelGGL((pair_mixed_n3_kernel<false, false>), g, b, 0, s, a, blkmap);
+    }
 }
 
 void launch_pair_mixed(const PairArgs& a, int nchunk, hipStream_t s)
 {
-    dim3 grid((a.n_i + kPairBlock - 1) / kPairBlock, nchunk);
-    if (a.rdf_weight)
-        hipLaunchKernelGGL(pair_mixed_kernel<true>, grid, dim3(kPairBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL(pair_mixed_kernel<false>, grid, dim3(kPairBlock), 0, s, a);
+    const dim3 grid((a.n_i + kPairBlock - 1) / kPairBlock, nchunk), b(kPairBlock);
+    const bool kel = (a.flags & TCP_KELBG) != 0;
+    if (a.rdf_weight) {
+        if (kel) hipLaunchKernelGGL((pair_mixed_kernel<true, true>), grid, b, 0, s, a);
+        else hipLaunchKernelGGL((pair_mixed_kernel<true, false>), grid, b, 0, s, a);
+    } else {
+        if (kel) hipLaunchKernelGGL((pair_mixed_kernel<false, true>), grid, b, 0, s, a);
+        else hipLaunchKernelGGL((pair_mixed_kernel<false, false>), grid, b, 0, s, a);
+    }
 }
 
 }  // namespace tcp

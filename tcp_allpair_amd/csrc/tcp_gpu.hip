@@ -204,6 +204,7 @@ PairArgs pair_args(tcp_ctx* c)
         a.inv_dx[t] = 1.0 / dx[t];
     }
     a.rdf = c->rdf;
+    a.flags = c->p.flags;
     return a;
 }
 
@@ -449,6 +450,8 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
     *out = nullptr;
     if (p->ne < 0 || p->ni < 0 || p->ne + p->ni < 2)
         return fail(nullptr, TCP_ERR_ARG, "need at least two particles");
+    if (p->flags & ~(TCP_KELBG | TCP_REG_EE))
+        return fail(nullptr, TCP_ERR_ARG, "unknown regularisation flags");
     if (precision != TCP_FP64 && precision != TCP_MIXED)
         return fail(nullptr, TCP_ERR_ARG, "precision must be TCP_FP64 or TCP_MIXED");
     for (int d = 0; d < 3; ++d)

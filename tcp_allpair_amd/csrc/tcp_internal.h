@@ -47,6 +47,7 @@ struct PairArgs {
     double inv_dx[3];          // RDF bin widths ee, ei, ii (reciprocal, mixed path)
     double dx[3];              // RDF bin widths (FP64 path divides as the reference)
     unsigned rdf_weight;       // 0: no RDF, 1: ordered pairs, 2: unordered pairs
+    int flags;                 // TCP_KELBG | TCP_REG_EE
     unsigned long long* rdf;   // [3][kHistStride] device counts
     double* pf;                // [nchunk][3][n_i] partial forces (eps included)
     double* pu;                // [nchunk][n_i] partial energies sum_j q q (erfc - y)/r

@@ -37,7 +37,7 @@ SUBROUTINE ensure_ctx(NS, param, single)
    p%kmax = 0
    p%recip = TCP_RECIP_SPME
    IF (.NOT. single) p%recip = TCP_RECIP_NONE
-   p%pad_ = 0
+   p%flags = 0                       ! the reference's model: Deutsch e-i only
    p%box = param%box
    p%q = param%q
    p%xm = param%xm

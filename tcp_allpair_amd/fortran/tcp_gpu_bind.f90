@@ -13,7 +13,7 @@ INTEGER, PARAMETER :: TCP_NBIN = 501
 TYPE, BIND(C) :: tcp_params_t
    INTEGER(C_INT32_T) :: ne, ni
    INTEGER(C_INT32_T) :: K(3)
-   INTEGER(C_INT32_T) :: order, kmax, recip, pad_
+   INTEGER(C_INT32_T) :: order, kmax, recip, flags
    REAL(C_DOUBLE) :: box(3), q(2), xm(2)
    REAL(C_DOUBLE) :: rc, rs, alpha, eps
    REAL(C_DOUBLE) :: dx_ee, dx_ei, dx_ii, dv_e, dv_i

@@ -27,6 +27,9 @@ RECIP_NONE, RECIP_SPME, RECIP_EWALD = 0, 1, 2
 # Arithmetic of the pair kernel
 FP64, MIXED = 0, 1
 
+# Regularisation options (no reference counterpart; 0 = Deutsch e-i only)
+KELBG, REG_EE = 1, 2
+
 
 @dataclasses.dataclass
 class TcpParams:
@@ -49,6 +52,7 @@ class TcpParams:
     recip: int = RECIP_SPME
     tau: float = 1.0                      # param%tau (Berendsen)
     fric: float = 0.0                     # param%alpha (Langevin friction)
+    flags: int = 0                        # KELBG | REG_EE (0: the reference's model)
 
     @property
     def n(self) -> int:
@@ -76,7 +80,7 @@ class CParams(ctypes.Structure):
     _fields_ = [
         ("ne", ctypes.c_int), ("ni", ctypes.c_int),
         ("K", ctypes.c_int * 3), ("order", ctypes.c_int), ("kmax", ctypes.c_int),
-        ("recip", ctypes.c_int), ("pad_", ctypes.c_int),
+        ("recip", ctypes.c_int), ("flags", ctypes.c_int),
         ("box", ctypes.c_double * 3), ("q", ctypes.c_double * 2), ("xm", ctypes.c_double * 2),
         ("rc", ctypes.c_double), ("rs", ctypes.c_double), ("alpha", ctypes.c_double),
         ("eps", ctypes.c_double),
@@ -89,7 +93,7 @@ class CParams(ctypes.Structure):
         c = cls()
         c.ne, c.ni = int(p.ne), int(p.ni)
         c.K[:] = [int(k) for k in p.K]
-        c.order, c.kmax, c.recip = int(p.order), int(p.kmax), int(p.recip)
+        c.order, c.kmax, c.recip, c.flags = int(p.order), int(p.kmax), int(p.recip), int(p.flags)
         c.box[:] = [float(b) for b in p.box]
         c.q[:] = [float(v) for v in p.q]
         c.xm[:] = [float(v) for v in p.xm]

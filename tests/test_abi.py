@@ -74,6 +74,7 @@ def test_cutoff_error_before_device_use():
     (dict(recip=2, kmax=0), -1),
     (dict(ne=1, ni=0), -1),
     (dict(rs=0.0), -1),
+    (dict(flags=4), -1),             # unknown regularisation flag
 ])
 def test_parameter_errors(change, code):
     p, _, _ = make_plasma(10, 10)

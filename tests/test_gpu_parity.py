@@ -297,3 +297,41 @@ def test_fullsize_mixed_step_properties():
     assert np.all(np.isfinite(t12)) and np.all(np.isfinite(f))
     assert np.abs(f.sum(0)).max() <= 1e-10 * np.abs(f).sum()
     assert np.all(np.abs(s1[0]) <= 0.5 * p.box[0])
+
+
+# ------------------------------------------------ regularisation options ---
+from tcp_allpair_amd.params import KELBG, REG_EE  # noqa: E402
+
+
+@pytest.mark.parametrize("flags", [KELBG, REG_EE, KELBG | REG_EE])
+def test_regularised_forms_vs_oracle(flags):
+    """Kelbg form and e-e regularisation (no reference counterpart) against the oracle,
+    FP64 at 1e-10 / 1e-12 and the mixed Newton kernel at 1e-6 RMS."""
+    say("case regularised", flags)
+    p, x, _ = make_plasma(1500, 1300, seed=61, recip=RECIP_NONE)
+    p = p.replace(flags=flags, rs=0.7)
+    fo, uo, rdfo = cpu.force_allpair(p, x)
+    f64, u64, rdf64 = direct(p, x, FP64)
+    assert rel_err(f64, fo).max() <= 1e-10
+    assert abs(u64 - uo) <= 1e-12 * abs(uo)
+    np.testing.assert_array_equal(rdf64, rdfo)
+    fmx, umx, _ = direct(p, x, MIXED)
+    assert rms_rel(fmx, f64) <= 1e-6
+    assert abs(umx - u64) <= 1e-6 * abs(u64)
+
+
+def test_regularised_ext_mixed_vs_fp64():
+    say("case regularised ext")
+    p, x, _ = make_plasma(700, 600, seed=62, recip=RECIP_NONE)
+    p = p.replace(flags=KELBG | REG_EE, rs=0.6)
+    own, ext = x[:650], x[650:]
+    po = p.replace(ne=650, ni=0)
+    res = []
+    for prec in (FP64, MIXED):
+        with Context(po, prec) as c:
+            c.set_state(x=own, f=np.zeros_like(own))
+            gemit, ur = c.force_direct_ext(ext, 50, 600)
+            res.append((c.get_state()[2], gemit, ur))
+    assert rms_rel(res[1][0], res[0][0]) <= 1e-6
+    assert rms_rel(res[1][1], res[0][1]) <= 1e-6
+    assert abs(res[1][2] - res[0][2]) <= 1e-6 * abs(res[0][2])

@@ -163,3 +163,53 @@ def test_ewald_self_closed_form():
                                rel=1e-15)
     with pytest.raises(RuntimeError):
         cpu.ewald_self(p.replace(rc=0.51 * p.box[0]))
+
+
+# ------------------------------------------------ regularisation options ---
+from tcp_allpair_amd.params import KELBG, REG_EE  # noqa: E402
+
+
+@pytest.mark.parametrize("flags", [KELBG, REG_EE, KELBG | REG_EE])
+def test_regularised_force_is_minus_energy_gradient(flags):
+    p, x = small_system(seed=19, ne=14, ni=12)
+    q = p.replace(flags=flags, rs=0.9)
+    fun = lambda y: cpu.force_direct_self(q, y)[1] * q.eps
+    f = cpu.force_direct_self(q, x)[0]
+    for i, k in [(0, 0), (3, 1), (15, 2), (22, 0)]:
+        g = fd_grad(fun, x, i, k, 1e-6)
+        assert f[i, k] == pytest.approx(-g, rel=1e-6, abs=1e-6 * np.abs(f).max())
+    fa, ua, _ = cpu.force_allpair(q, x)
+    fs, us, _ = cpu.force_direct_self(q, x)
+    np.testing.assert_allclose(fa, fs, rtol=0, atol=1e-11 * np.abs(fs).max())
+    assert ua == pytest.approx(us, rel=1e-13)
+
+
+def pair_energy(flags, r, alpha, lam=0.5, species=(0, 1)):
+    """Energy (no eps) of one pair at separation r along x in a large box."""
+    L = 200.0
+    ne = 2 if species == (0, 0) else 1
+    p, _, _ = make_plasma(ne, 2 - ne)
+    p = p.replace(box=(L, L, L), rc=L / 2, rs=lam, alpha=alpha, flags=flags)
+    x = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+    return cpu.force_direct_self(p, x)[1]
+
+
+def test_kelbg_closed_forms():
+    lam, alpha = 0.5, 1e-7
+    for r in (0.05, 0.3, 1.0, 2.5):
+        s = r / lam
+        kelbg = (1 - math.exp(-s * s) + math.sqrt(math.pi) * s * math.erfc(s)) / r
+        # q1 q2 = -1 for an electron-ion pair; alpha -> 0 leaves the bare pair potential
+        assert pair_energy(KELBG, r, alpha) == pytest.approx(-kelbg, rel=1e-6)
+        deutsch = (1 - math.exp(-s)) / r
+        assert pair_energy(0, r, alpha) == pytest.approx(-deutsch, rel=1e-6)
+    # r -> 0: the Kelbg pair energy is finite, q1 q2 (sqrt(pi)/lambda - 2 alpha/sqrt(pi))
+    alpha = 0.7
+    u0 = pair_energy(KELBG, 1e-7, alpha)
+    assert u0 == pytest.approx(-(math.sqrt(math.pi) / lam - 2 * alpha / math.sqrt(math.pi)), rel=1e-5)
+    # e-e pairs are bare unless REG_EE
+    r = 0.4
+    assert pair_energy(0, r, alpha, species=(0, 0)) == pytest.approx(math.erfc(alpha * r) / r, rel=1e-12)
+    s = r / lam
+    assert pair_energy(KELBG | REG_EE, r, alpha, species=(0, 0)) == pytest.approx(
+        (math.erfc(alpha * r) - math.exp(-s * s) + math.sqrt(math.pi) * s * math.erfc(s)) / r, rel=1e-12)
