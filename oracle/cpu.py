@@ -166,3 +166,18 @@ def step(p: TcpParams, thermo, dt, x, v, f, rdf=None, vdf=None, fast=False, nthr
                            ctypes.c_double(p.fric), _p(x), _p(v), _p(f), _p(rdf), _p(vdf), _p(out),
                            int(bool(fast)), int(nthreads)), "step")
     return dict(Ur=out[0], Uo=out[1], Um=out[2], Epot=out[3], mv2el=out[4], mv2ion=out[5])
+
+
+def adjoint(p: TcpParams, x, q=None, nthreads=0):
+    """(U, dU/dx, dU/dq, dU/dlambda, dF/dlambda); q: per-particle charges or None."""
+    x = _c(x)
+    n = p.n
+    qa = None if q is None else _c(q)
+    U, dl = ctypes.c_double(), ctypes.c_double()
+    dudx = np.zeros((n, 3))
+    dudq = np.zeros(n)
+    dfdl = np.zeros((n, 3))
+    _check(lib().tcpo_adjoint(ctypes.byref(CParams.from_params(p)), _p(x), _p(qa), ctypes.byref(U),
+                              _p(dudx), _p(dudq), ctypes.byref(dl), _p(dfdl), int(nthreads)),
+           "adjoint")
+    return U.value, dudx, dudq, dl.value, dfdl

@@ -411,7 +411,17 @@ static void dft3d(double complex* a, const int* K, int sign)
 /* Force_SPME4 (force.f90:652-929) + SPME_SUM (:932-969), order p:
  *   Q = sum q w w w;  Qinv = IDFT_norm(Q);  Qinv *= PSI * B;  Qfin = DFT(Qinv);
  *   Epme = K^3 sum Re(Q Qfin);  F = -eps K^3 sum dQdx Re(Qfin)/(V pi);  Um = eps Epme/(2 pi V). */
+static int spme_core(const tcpo_params* p, double alpha, const double* x, const double* qp,
+                     double* frec, double* pot, double* Um);
+
 int tcpo_spme(const tcpo_params* p, double alpha, const double* x, double* frec, double* Um)
+{
+    return spme_core(p, alpha, x, NULL, frec, NULL, Um);
+}
+
+/* qp: per-particle charges (NULL: the species charges); pot[m] = dUm/dq_m */
+static int spme_core(const tcpo_params* p, double alpha, const double* x, const double* qp,
+                     double* frec, double* pot, double* Um)
 {
     const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2], P = p->order, n = p->ne + p->ni;
     if (P < 2 || P > 16 || (P & 1) || K1 < P || K2 < P || K3 < P) return -1;
@@ -432,7 +442,7 @@ int tcpo_spme(const tcpo_params* p, double alpha, const double* x, double* frec,
     int base[3];
     /* spread, force.f90:696-820 */
     for (int m = 0; m < n; ++m) {
-        double q1 = p->q[m >= p->ne];
+        double q1 = qp ? qp[m] : p->q[m >= p->ne];
         for (int d = 0; d < 3; ++d) {
             double u = (double)p->K[d] * (x[3 * (size_t)m + d] + p->box[d] * 0.5) / p->box[d];
             tcpo_bspline(P, u, w[d], dw[d]);
@@ -467,7 +477,7 @@ int tcpo_spme(const tcpo_params* p, double alpha, const double* x, double* frec,
     double q1g = p->eps * (double)G;
     /* gather, force.f90:868-919 */
     for (int m = 0; m < n; ++m) {
-        double q1 = p->q[m >= p->ne];
+        double q1 = qp ? qp[m] : p->q[m >= p->ne];
         double s[3];
         for (int d = 0; d < 3; ++d) {
             double u = (double)p->K[d] * (x[3 * (size_t)m + d] + p->box[d] * 0.5) / p->box[d];
@@ -475,7 +485,7 @@ int tcpo_spme(const tcpo_params* p, double alpha, const double* x, double* frec,
             base[d] = (int)floor(u) - P / 2 + 1;
             s[d] = (double)p->K[d] / p->box[d];
         }
-        double fx = 0.0, fy = 0.0, fz = 0.0;
+        double fx = 0.0, fy = 0.0, fz = 0.0, ph = 0.0;
         for (int i = 0; i < P; ++i) {
             int ix = ((base[0] + i) % K1 + K1) % K1;
             for (int j = 0; j < P; ++j) {
@@ -483,6 +493,7 @@ int tcpo_spme(const tcpo_params* p, double alpha, const double* x, double* frec,
                 for (int k = 0; k < P; ++k) {
                     int iz = ((base[2] + k) % K3 + K3) % K3;
                     double c = creal(A[((size_t)iz * K2 + iy) * K1 + ix]);
+                    ph += w[0][i] * w[1][j] * w[2][k] * c;
                     fx -= q1g * (q1 * dw[0][i] * w[1][j] * w[2][k] * s[0]) * c / V / PI;
                     fy -= q1g * (q1 * w[0][i] * dw[1][j] * w[2][k] * s[1]) * c / V / PI;
                     fz -= q1g * (q1 * w[0][i] * w[1][j] * dw[2][k] * s[2]) * c / V / PI;
@@ -492,6 +503,7 @@ int tcpo_spme(const tcpo_params* p, double alpha, const double* x, double* frec,
         frec[3 * (size_t)m + 0] = fx;
         frec[3 * (size_t)m + 1] = fy;
         frec[3 * (size_t)m + 2] = fz;
+        if (pot) pot[m] = q1g * ph / V / PI;  /* dUm/dq_m = eps K^3 sum W Qfin/(pi V) */
     }
     *Um = p->eps * Epme / V / PI / 2.0; /* force.f90:921 */
     free(Q); free(psi); free(A); free(b1);
@@ -500,7 +512,17 @@ int tcpo_spme(const tcpo_params* p, double alpha, const double* x, double* frec,
 
 /* Direct reciprocal Ewald sum: E = 1/(2 pi V) sum_{m != 0} exp(-pi^2 m^2/alpha^2)/m^2 |S(m)|^2,
  * S(m) = sum_j q_j exp(2 pi i m.r_j), F_j = (2 eps q_j/V) sum_m G(m) m Im(conj(S) e^{2 pi i m.r_j}). */
+static int kspace_core(const tcpo_params* p, double alpha, const double* x, const double* qp,
+                       double* frec, double* pot, double* Um);
+
 int tcpo_ewald_kspace(const tcpo_params* p, double alpha, const double* x, double* frec, double* Um)
+{
+    return kspace_core(p, alpha, x, NULL, frec, NULL, Um);
+}
+
+/* pot[j] = dUm/dq_j = eps/(pi V) sum_m G Re(conj(S) e^{2 pi i m.r_j}) */
+static int kspace_core(const tcpo_params* p, double alpha, const double* x, const double* qp,
+                       double* frec, double* pot, double* Um)
 {
     const int n = p->ne + p->ni, km = p->kmax, nk = 2 * km + 1;
     if (km < 1) return -1;
@@ -514,7 +536,8 @@ int tcpo_ewald_kspace(const tcpo_params* p, double alpha, const double* x, doubl
                 double th = 2.0 * PI * (double)m * x[3 * (size_t)j + d] / p->box[d];
                 ph[((size_t)j * 3 + d) * nk + (m + km)] = cos(th) + I * sin(th);
             }
-    double* fs = (double*)calloc((size_t)n * 3, sizeof(double));
+    double* fs = (double*)calloc((size_t)n * 4, sizeof(double));
+    double* ps = fs + 3 * (size_t)n;
     double e = 0.0, ec = 0.0;
     for (int mx = -km; mx <= km; ++mx)
         for (int my = -km; my <= km; ++my)
@@ -526,23 +549,131 @@ int tcpo_ewald_kspace(const tcpo_params* p, double alpha, const double* x, doubl
                 double complex S = 0.0;
                 for (int j = 0; j < n; ++j) {
                     const double complex* pj = ph + (size_t)j * 3 * nk;
-                    S += p->q[j >= p->ne] * pj[mx + km] * pj[nk + my + km] * pj[2 * nk + mz + km];
+                    S += (qp ? qp[j] : p->q[j >= p->ne]) * pj[mx + km] * pj[nk + my + km] * pj[2 * nk + mz + km];
                 }
                 nsum(&e, &ec, G * (creal(S) * creal(S) + cimag(S) * cimag(S)));
                 for (int j = 0; j < n; ++j) {
                     const double complex* pj = ph + (size_t)j * 3 * nk;
                     double complex ej = pj[mx + km] * pj[nk + my + km] * pj[2 * nk + mz + km];
-                    double s = cimag(conj(S) * ej) * G * p->q[j >= p->ne];
+                    double s = cimag(conj(S) * ej) * G * (qp ? qp[j] : p->q[j >= p->ne]);
                     fs[3 * (size_t)j + 0] += s * kx;
                     fs[3 * (size_t)j + 1] += s * ky;
                     fs[3 * (size_t)j + 2] += s * kz;
+                    ps[j] += creal(conj(S) * ej) * G;
                 }
             }
     for (size_t t = 0; t < (size_t)n * 3; ++t) frec[t] = 2.0 * p->eps / V * fs[t];
+    if (pot)
+        for (int j = 0; j < n; ++j) pot[j] = p->eps / (PI * V) * ps[j];
     *Um = p->eps * (e + ec) / (2.0 * PI * V);
     free(fs);
     free(ph);
     return 0;
+}
+
+/* ------------------------------------------------------------- adjoint --- */
+
+/* U = eps (Ur + Uo) + Um is homogeneous of degree 2 in the charges, so
+ * dU/dq_i = eps (phi_i - 2 alpha q_i/sqrt(pi)) + dUm/dq_i with
+ * phi_i = sum_j q_j (erfc(alpha r) + R)/r.  lambda enters only R:
+ *   Deutsch R = -e^{-s}:            d(R/r)/dlambda = -e^{-s}/lambda^2,
+ *   Kelbg   R = -e^{-s^2} + sqrt(pi) s erfc(s): d(R/r)/dlambda = -sqrt(pi) erfc(s)/lambda^2,
+ * and dF_i/dlambda = -eps sum_j q_i q_j h'(r) r_ij/r with h = d(R/r)/dlambda:
+ *   h' = e^{-s}/lambda^3 (Deutsch), 2 e^{-s^2}/lambda^3 (Kelbg). */
+int tcpo_adjoint(const tcpo_params* p, const double* x, const double* qp, double* U,
+                 double* dUdx, double* dUdq, double* dUdl, double* dFdl, int nthreads)
+{
+    const int ne = p->ne, n = p->ne + p->ni;
+    double alpha, rc2, Uo;
+    int rc = tcpo_ewald_self(p, &alpha, &rc2, &Uo);
+    if (rc) return rc;
+    double q2sum = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double qi = qp ? qp[i] : p->q[i >= ne];
+        q2sum += qi * qi;
+    }
+    Uo = -q2sum * alpha / SQRTPI;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    double* ui = (double*)calloc((size_t)n * 2, sizeof(double));
+    double* li = ui + n;
+    if (!ui) return -2;
+    const double lam = p->rs;
+    const int kelbg = (p->flags & TCPO_KELBG) != 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+    for (int i = 0; i < n; ++i) {
+        const int si = i >= ne;
+        const double qi = qp ? qp[i] : p->q[si];
+        double phs = 0.0, phc = 0.0, us = 0.0, uc = 0.0, ls = 0.0, lc = 0.0;
+        double fs[3] = {0, 0, 0}, fc[3] = {0, 0, 0}, gs[3] = {0, 0, 0}, xr[3];
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            const double r2 = min_image(p->box, x + 3 * (size_t)i, x + 3 * (size_t)j, xr);
+            if (!(r2 < rc2)) continue;
+            const int sj = j >= ne;
+            const double qj = qp ? qp[j] : p->q[sj];
+            const int reg = si != sj || (si == 0 && sj == 0 && (p->flags & TCPO_REG_EE));
+            const double r = sqrt(r2);
+            const double ec = erfc(alpha * r);
+            const double g = 2.0 * alpha * r * exp(-(alpha * alpha * r2)) / SQRTPI;
+            double R = 0.0, num = g + ec, h = 0.0, hp = 0.0;
+            if (reg) {
+                const double s = r / lam;
+                if (kelbg) {
+                    const double y2 = exp(-(s * s));
+                    R = -y2 + SQRTPI * s * erfc(s);
+                    num -= y2;
+                    h = -SQRTPI * erfc(s) / (lam * lam);
+                    hp = 2.0 * y2 / (lam * lam * lam);
+                } else {
+                    const double y = exp(-s);
+                    R = -y;
+                    num -= y * (1.0 + s);
+                    h = -y / (lam * lam);
+                    hp = y / (lam * lam * lam);
+                }
+            }
+            const double phi = (ec + R) / r;
+            nsum(&phs, &phc, qj * phi);
+            nsum(&us, &uc, qi * qj * phi);
+            nsum(&ls, &lc, qi * qj * h);
+            const double df = p->eps * qi * qj * num / (r2 * r);
+            for (int k = 0; k < 3; ++k) {
+                nsum(&fs[k], &fc[k], df * xr[k]);
+                gs[k] -= p->eps * qi * qj * hp * xr[k] / r;
+            }
+        }
+        dUdq[i] = p->eps * ((phs + phc) - 2.0 * alpha * qi / SQRTPI);
+        for (int k = 0; k < 3; ++k) {
+            dUdx[3 * (size_t)i + k] = -(fs[k] + fc[k]);
+            if (dFdl) dFdl[3 * (size_t)i + k] = gs[k];
+        }
+        ui[i] = us + uc;
+        li[i] = ls + lc;
+    }
+    double s = 0.0, c = 0.0, sl = 0.0, cl = 0.0;
+    for (int i = 0; i < n; ++i) {
+        nsum(&s, &c, ui[i]);
+        nsum(&sl, &cl, li[i]);
+    }
+    double Um = 0.0;
+    double* frec = (double*)calloc((size_t)n * 4, sizeof(double));
+    double* pot = frec + 3 * (size_t)n;
+    if (p->recip == TCPO_RECIP_SPME) rc = spme_core(p, alpha, x, qp, frec, pot, &Um);
+    else if (p->recip == TCPO_RECIP_EWALD) rc = kspace_core(p, alpha, x, qp, frec, pot, &Um);
+    if (!rc)
+        for (int i = 0; i < n; ++i) {
+            dUdq[i] += pot[i];
+            for (int k = 0; k < 3; ++k) dUdx[3 * (size_t)i + k] -= frec[3 * (size_t)i + k];
+        }
+    *U = p->eps * (0.5 * (s + c) + Uo) + Um;
+    *dUdl = p->eps * 0.5 * (sl + cl);
+    free(frec);
+    free(ui);
+    return rc;
 }
 
 /* ------------------------------------------------------------ integrate --- */

@@ -76,6 +76,12 @@ int tcpo_bspline_modulus(int order, int K, double* bsq);
 /* Direct Ewald reciprocal sum over |m_d| <= kmax, m != 0: frec (overwrite), Um. */
 int tcpo_ewald_kspace(const tcpo_params* p, double alpha, const double* x, double* frec, double* Um);
 
+/* Energy and its derivatives (no reference counterpart): U = eps (Ur + Uo) + Um,
+ * dUdx = -F, dUdq[i] = dU/dq_i, *dUdl = dU/dlambda, dFdl = dF/dlambda (real-space
+ * regularisation only; may be NULL).  qp: per-particle charges (NULL: species). */
+int tcpo_adjoint(const tcpo_params* p, const double* x, const double* qp, double* U,
+                 double* dUdx, double* dUdq, double* dUdl, double* dFdl, int nthreads);
+
 /* REMOVE_RIGIDMTN (force.f90:974-1007). */
 int tcpo_remove_rigidmtn(const tcpo_params* p, double* f);
 /* VVerletNotemp1 / VVerletStoch1 (vverlet.f90:132-171 / 9-59). fric = param%alpha. */
