@@ -129,6 +129,24 @@ int tcp_get_histograms(tcp_ctx* ctx, double* rdf, double* vdf, int reset);
 int tcp_md_steps(tcp_ctx* ctx, int nsteps, int thermo, double dt, double Te, double Ti,
                  double tau, double fric, uint64_t seed, double* trace);
 
+/* Sets lambda (param%rs) of the regularised pairs.  No counterpart in the
+ * reference, which reads param%rs once from tcp.prm (main.f90:260). */
+int tcp_set_lambda(tcp_ctx* ctx, double lambda);
+
+/* Total potential energy and its derivatives at the current positions, for
+ * fitting charges and lambda to reference forces or energies.  No counterpart
+ * in the reference.  Per-particle charges q[N] (NULL: the species charges
+ * param%q) replace the species charges in every term, including the self
+ * energy -alpha/sqrt(pi) sum q_i^2 and the reciprocal part; always FP64.
+ *   *U      = eps (Ur + Uo) + Um, sys%Epot (main.f90:138) + sys%Um
+ *   dUdx[3N] = -F (real-space + reciprocal forces)
+ *   dUdq[N]  = dU/dq_i
+ *   *dUdl    = dU/dlambda
+ *   dFdl[3N] = dF/dlambda (may be NULL); lambda enters only the regularised pairs.
+ * Host arrays; the context's forces f are left unchanged. */
+int tcp_energy_adjoint(tcp_ctx* ctx, const double* q, double* U, double* dUdx, double* dUdq,
+                       double* dUdl, double* dFdl);
+
 /* Device time in milliseconds of the pair kernel's last launch and of the
  * last tcp_md_steps call (HIP events on the context's stream). */
 int tcp_last_timings(tcp_ctx* ctx, double* pair_ms, double* steps_ms);

@@ -128,6 +128,100 @@ __global__ __launch_bounds__(kPairBlock) void pair_f64_kernel(PairArgs a)
     }
 }
 
+// --------------------------------------------------------------- ADJOINT ---
+//
+// Energy derivatives for fitting charges and lambda: the FP64 ordered-pair
+// shape above with per-particle charges.  With R the regularisation and
+// h = d(R/r)/dlambda:
+//   Deutsch R = -e^{-s}:                          h = -e^{-s}/lambda^2,         dh/dr = e^{-s}/lambda^3
+//   Kelbg   R = -e^{-s^2} + sqrt(pi) s erfc(s):   h = -sqrt(pi) erfc(s)/lambda^2, dh/dr = 2 e^{-s^2}/lambda^3
+// (s = r/lambda), and dF_i/dlambda = -eps sum_j q_i q_j (dh/dr) r_ij / r.
+
+__global__ __launch_bounds__(kPairBlock) void pair_adjoint_kernel(AdjArgs a)
+{
+    const int i = blockIdx.x * kPairBlock + threadIdx.x;
+    const bool live = i < a.n;
+    const int ii = live ? i : a.n - 1;
+    const double xi = a.p.x[ii], yi = a.p.y[ii], zi = a.p.z[ii];
+    const double qi = a.qp[ii];
+    const int si = ii >= a.ne;
+    const int j0 = blockIdx.y * a.jchunk;
+    const int j1 = min(a.n, j0 + a.jchunk);
+    const double alpha = a.alpha, a2 = alpha * alpha, two_a_sqrtpi = 2.0 * alpha / kSqrtPi;
+    const double lam = a.rs, inv_l = 1.0 / lam, inv_l2 = inv_l * inv_l, inv_l3 = inv_l2 * inv_l;
+    const bool kelbg = (a.flags & TCP_KELBG) != 0;
+    double acc[kAdjQ];
+#pragma unroll
+    for (int k = 0; k < kAdjQ; ++k) acc[k] = 0.0;
+    for (int sj = 0; sj < 2; ++sj) {
+        const int lo = sj ? max(j0, a.ne) : j0;
+        const int hi = sj ? j1 : min(j1, a.ne);
+        const bool reg = si != sj || (si == 0 && sj == 0 && (a.flags & TCP_REG_EE));
+        for (int j = lo; j < hi; ++j) {
+            double xr, yr, zr, r2;
+            {
+#pragma clang fp contract(off)
+                const double dx = xi - a.p.x[j];
+                const double dy = yi - a.p.y[j];
+                const double dz = zi - a.p.z[j];
+                xr = dx - a.box[0] * rint(dx * a.inv_box[0]);
+                yr = dy - a.box[1] * rint(dy * a.inv_box[1]);
+                zr = dz - a.box[2] * rint(dz * a.inv_box[2]);
+                r2 = xr * xr + yr * yr + zr * zr;
+            }
+            if (!(r2 < a.rc2 && r2 > 0.0)) continue;
+            const double qj = a.qp[j];
+            const double r = sqrt(r2), inv_r = 1.0 / r;
+            const double ec = erfc(alpha * r);
+            double num = two_a_sqrtpi * r * exp(-(a2 * r2)) + ec, en = ec, h = 0.0, hp = 0.0;
+            if (reg) {
+                const double s = r * inv_l;
+                if (kelbg) {
+                    const double y2 = exp(-(s * s)), es = erfc(s);
+                    num -= y2;
+                    en += kSqrtPi * s * es - y2;
+                    h = -kSqrtPi * es * inv_l2;
+                    hp = 2.0 * y2 * inv_l3;
+                } else {
+                    const double y = exp(-s);
+                    num -= y * (1.0 + s);
+                    en -= y;
+                    h = -y * inv_l2;
+                    hp = y * inv_l3;
+                }
+            }
+            const double qq = qi * qj;
+            const double df = a.eps * qq * num * inv_r * inv_r * inv_r;
+            const double dg = -a.eps * qq * hp * inv_r;
+            acc[0] += df * xr;
+            acc[1] += df * yr;
+            acc[2] += df * zr;
+            acc[3] += qq * en * inv_r;
+            acc[4] += qj * en * inv_r;
+            acc[5] += qq * h;
+            acc[6] += dg * xr;
+            acc[7] += dg * yr;
+            acc[8] += dg * zr;
+        }
+    }
+    if (live) {
+        double* out = a.part + (size_t)blockIdx.y * kAdjQ * a.n + i;
+#pragma unroll
+        for (int k = 0; k < kAdjQ; ++k) out[(size_t)k * a.n] = acc[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void adjoint_reduce_kernel(const double* part, int nchunk, int n,
+                                                             double* out)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t m = (size_t)kAdjQ * n;
+    if (t >= m) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * m + t];
+    out[t] = s;
+}
+
 // ----------------------------------------------------------------- MIXED ---
 //
 // Separations come from FixPos: the wrapped 32-bit difference of the fixed-point
@@ -600,6 +694,19 @@ void launch_pair_f64(const PairArgs& a, int nchunk, hipStream_t s)
         hipLaunchKernelGGL(pair_f64_kernel<true>, grid, dim3(kPairBlock), 0, s, a);
     else
         hipLaunchKernelGGL(pair_f64_kernel<false>, grid, dim3(kPairBlock), 0, s, a);
+}
+
+void launch_pair_adjoint(const AdjArgs& a, int nchunk, hipStream_t s)
+{
+    dim3 grid((a.n + kPairBlock - 1) / kPairBlock, nchunk);
+    hipLaunchKernelGGL(pair_adjoint_kernel, grid, dim3(kPairBlock), 0, s, a);
+}
+
+void launch_adjoint_reduce(const double* part, int nchunk, int n, double* out, hipStream_t s)
+{
+    const size_t m = (size_t)kAdjQ * n;
+    hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
+                       part, nchunk, n, out);
 }
 
 int n3_block_rows(int n)

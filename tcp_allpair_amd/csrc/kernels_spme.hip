@@ -56,6 +56,11 @@ __device__ inline void bspline(double f, double* w, double* dw)
     for (int j = 0; j < P; ++j) w[P - 1 - j] = W[j];
 }
 
+__device__ inline double charge(const RecipArgs& a, int m)
+{
+    return a.qp ? a.qp[m] : a.q[m >= a.ne];
+}
+
 __device__ inline int wrap(int i, int K)
 {
     i %= K;
@@ -67,7 +72,7 @@ __global__ __launch_bounds__(256) void spread_kernel(RecipArgs a)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= a.n) return;
-    const double q = a.q[m >= a.ne];
+    const double q = charge(a, m);
     const double pos[3] = {a.x[m], a.y[m], a.z[m]};
     double w[3][P], dw[3][P];
     int base[3];
@@ -100,7 +105,7 @@ __global__ __launch_bounds__(256) void gather_kernel(RecipArgs a)
     if (t >= a.n) return;
     // in brick order neighbouring threads read neighbouring grid rows
     const int m = a.nbrick[0] ? a.sorted[t] : t;
-    const double q = a.q[m >= a.ne];
+    const double q = charge(a, m);
     const double pos[3] = {a.x[m], a.y[m], a.z[m]};
     double w[3][P], dw[3][P];
     int base[3];
@@ -112,7 +117,7 @@ __global__ __launch_bounds__(256) void gather_kernel(RecipArgs a)
         base[d] = (int)fl - P / 2 + 1;
     }
     const int K1 = a.K[0], K2 = a.K[1], K3 = a.K[2];
-    double sx = 0.0, sy = 0.0, sz = 0.0;
+    double sx = 0.0, sy = 0.0, sz = 0.0, sp = 0.0;
     for (int k = 0; k < P; ++k) {
         const int iz = wrap(base[2] + k, K3);
         for (int j = 0; j < P; ++j) {
@@ -128,10 +133,12 @@ __global__ __launch_bounds__(256) void gather_kernel(RecipArgs a)
             sx += cx * w[1][j] * w[2][k];
             sy += c0 * dw[1][j] * w[2][k];
             sz += c0 * w[1][j] * dw[2][k];
+            sp += c0 * w[1][j] * w[2][k];
         }
     }
     const double V = a.box[0] * a.box[1] * a.box[2];
     const double pre = -a.eps * q / (kPi * V);
+    if (a.pot) a.pot[m] = a.eps / (kPi * V) * sp;  // dUm/dq_m
     a.frx[m] = pre * sx * ((double)K1 / a.box[0]);
     a.fry[m] = pre * sy * ((double)K2 / a.box[1]);
     a.frz[m] = pre * sz * ((double)K3 / a.box[2]);
@@ -223,7 +230,7 @@ __global__ __launch_bounds__(256) void spread_brick_kernel(RecipArgs a)
         __syncthreads();
         if ((int)threadIdx.x < nq) {
             const int m = a.sorted[q0 + threadIdx.x];
-            const double q = a.q[m >= a.ne];
+            const double q = charge(a, m);
             const double pos[3] = {a.x[m], a.y[m], a.z[m]};
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -468,7 +475,7 @@ __global__ __launch_bounds__(256) void ewald_sk_kernel(RecipArgs a, int kmax, do
         const double th = 2.0 * kPi * (kx * a.x[j] + ky * a.y[j] + kz * a.z[j]);
         double sn, cs;
         sincos(th, &sn, &cs);
-        const double q = a.q[j >= a.ne];
+        const double q = charge(a, j);
         sr += q * cs;
         si += q * sn;
     }
@@ -499,7 +506,7 @@ __global__ __launch_bounds__(256) void ewald_force_kernel(RecipArgs a, int kmax,
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int nk = 2 * kmax + 1, nm = nk * nk * nk;
     if (j < a.n) {
-        double fx = 0.0, fy = 0.0, fz = 0.0;
+        double fx = 0.0, fy = 0.0, fz = 0.0, pt = 0.0;
         const double xj = a.x[j], yj = a.y[j], zj = a.z[j];
         for (int mi = 0; mi < nm; ++mi) {
             const double G = sk[3 * mi + 2];
@@ -513,9 +520,11 @@ __global__ __launch_bounds__(256) void ewald_force_kernel(RecipArgs a, int kmax,
             fx += G * im * kx;
             fy += G * im * ky;
             fz += G * im * kz;
+            pt += G * (sk[3 * mi] * cs + sk[3 * mi + 1] * sn);  // Re(conj(S) e)
         }
         const double V = a.box[0] * a.box[1] * a.box[2];
-        const double pre = 2.0 * a.eps * a.q[j >= a.ne] / V;
+        if (a.pot) a.pot[j] = a.eps / (kPi * V) * pt;  // dUm/dq_j
+        const double pre = 2.0 * a.eps * charge(a, j) / V;
         a.frx[j] = pre * fx;
         a.fry[j] = pre * fy;
         a.frz[j] = pre * fz;

@@ -77,6 +77,8 @@ struct tcp_ctx {
     FixPos* gupos = nullptr;
     double *gpf = nullptr, *gpu = nullptr;
     size_t gpf_cap = 0;
+    // energy derivatives (allocated by the first tcp_energy_adjoint call)
+    double *adj_part = nullptr, *adj_out = nullptr, *adj_q = nullptr, *adj_pot = nullptr;
     // timing
     hipEvent_t ev_pair[2][kEvPool];
     hipEvent_t ev_all[2];
@@ -248,10 +250,13 @@ int direct_all(tcp_ctx* c, int rdf_weight, const double* frx, const double* fry,
     return 0;
 }
 
-// Reciprocal forces into frx/fry/frz and Um into sc[SC_UM].
-int recip(tcp_ctx* c)
+// Reciprocal forces into frx/fry/frz and Um into sc[SC_UM]; with qp the
+// per-particle charges replace the species charges, pot receives dUm/dq.
+int recip(tcp_ctx* c, const double* qp = nullptr, double* pot = nullptr)
 {
     RecipArgs a = recip_args(c);
+    a.qp = qp;
+    a.pot = pot;
     const double V = c->p.box[0] * c->p.box[1] * c->p.box[2];
     if (c->p.recip == TCP_RECIP_SPME) {
         launch_spme_spread(a, c->stream);
@@ -806,6 +811,95 @@ int tcp_md_steps(tcp_ctx* c, int nsteps, int thermo, double dt, double Te, doubl
     if (trace) {
         HIPCK(c, hipMemcpy(trace, c->trace, 6 * (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost));
     }
+    return 0;
+}
+
+int tcp_set_lambda(tcp_ctx* c, double lambda)
+{
+    if (!c) return TCP_ERR_ARG;
+    if (!(lambda > 0) || !std::isfinite(lambda)) return fail(c, TCP_ERR_ARG, "lambda must be positive");
+    c->p.rs = lambda;
+    return 0;
+}
+
+int tcp_energy_adjoint(tcp_ctx* c, const double* q, double* U, double* dUdx, double* dUdq,
+                       double* dUdl, double* dFdl)
+{
+    if (!c) return TCP_ERR_ARG;
+    if (!U || !dUdx || !dUdq || !dUdl) return fail(c, TCP_ERR_ARG, "null output");
+    const int n = c->N;
+    if (!c->adj_part) {
+        TRY(dalloc(c, &c->adj_part, (size_t)kAdjQ * c->nchunk * n));
+        TRY(dalloc(c, &c->adj_out, (size_t)kAdjQ * n));
+        TRY(dalloc(c, &c->adj_q, (size_t)n));
+        TRY(dalloc(c, &c->adj_pot, (size_t)n));
+    }
+    std::vector<double> qh(n);
+    double q2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+        qh[i] = q ? q[i] : c->p.q[i >= c->Ne];
+        if (!std::isfinite(qh[i])) return fail(c, TCP_ERR_ARG, "charges must be finite");
+        q2 += qh[i] * qh[i];
+    }
+    HIPCK(c, hipMemcpyAsync(c->adj_q, qh.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    AdjArgs a{};
+    a.p = PosSoA{c->x, c->y, c->z};
+    a.qp = c->adj_q;
+    a.n = n;
+    a.ne = c->Ne;
+    a.jchunk = c->jchunk;
+    a.flags = c->p.flags;
+    for (int d = 0; d < 3; ++d) {
+        a.box[d] = c->p.box[d];
+        a.inv_box[d] = 1.0 / c->p.box[d];
+    }
+    a.rc2 = c->rc2;
+    a.alpha = c->alpha;
+    a.rs = c->p.rs;
+    a.eps = c->p.eps;
+    a.part = c->adj_part;
+    launch_pair_adjoint(a, c->nchunk, c->stream);
+    LAUNCHCK(c);
+    launch_adjoint_reduce(c->adj_part, c->nchunk, n, c->adj_out, c->stream);
+    LAUNCHCK(c);
+    const bool rec = c->p.recip != TCP_RECIP_NONE;
+    if (rec) TRY(recip(c, c->adj_q, c->adj_pot));
+    std::vector<double> out((size_t)kAdjQ * n), fr(rec ? 4 * (size_t)n : 0);
+    double um = 0.0;
+    HIPCK(c, hipMemcpyAsync(out.data(), c->adj_out, out.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    if (rec) {
+        const double* src[4] = {c->frx, c->fry, c->frz, c->adj_pot};
+        for (int k = 0; k < 4; ++k)
+            HIPCK(c, hipMemcpyAsync(fr.data() + (size_t)k * n, src[k], n * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipMemcpyAsync(&um, c->sc + SC_UM, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    TRY(sync(c));
+    // compensated sums of the per-particle energies in particle order
+    auto nsum = [](double& s, double& e, double v) {
+        const double t = s + v;
+        e += std::fabs(s) >= std::fabs(v) ? (s - t) + v : (v - t) + s;
+        s = t;
+    };
+    double us = 0.0, ue = 0.0, ls = 0.0, le = 0.0;
+    const double self = 2.0 * c->alpha / kSqrtPi;
+    for (int i = 0; i < n; ++i) {
+        nsum(us, ue, out[3 * (size_t)n + i]);
+        nsum(ls, le, out[5 * (size_t)n + i]);
+        double dq = c->p.eps * (out[4 * (size_t)n + i] - self * qh[i]);
+        if (rec) dq += fr[3 * (size_t)n + i];
+        dUdq[i] = dq;
+        for (int k = 0; k < 3; ++k) {
+            double f = out[(size_t)k * n + i];
+            if (rec) f += fr[(size_t)k * n + i];
+            dUdx[3 * (size_t)i + k] = -f;
+            if (dFdl) dFdl[3 * (size_t)i + k] = out[(size_t)(6 + k) * n + i];
+        }
+    }
+    const double Uo = -q2 * c->alpha / kSqrtPi;
+    *U = c->p.eps * (0.5 * (us + ue) + Uo) + um;
+    *dUdl = c->p.eps * 0.5 * (ls + le);
     return 0;
 }
 

@@ -71,6 +71,8 @@ struct RecipArgs {
     int order;
     double box[3];
     double q[2];
+    const double* qp;          // per-particle charges (null: the species charges q)
+    double* pot;               // [n] dUm/dq_m when not null (gather and k-sum kernels)
     double eps;
     double* Q;                 // real charge grid K1*K2*K3 (x fastest)
     double2* A;                // complex work grid
@@ -104,6 +106,23 @@ int fft_epart_blocks(const int* K);
 // Direct Ewald k-space sum (small systems, reference-independent extension)
 void launch_ewald_kspace(const RecipArgs& a, int kmax, double alpha, double* sk_work,
                          hipStream_t s);  // energy sum into epart[0]
+
+// Energy derivatives (FP64, ordered pairs, per-particle charges): per chunk of j
+// and particle i, part[(chunk * kAdjQ + k) * n + i] holds k = 0..2 the force
+// (eps included), 3 sum_j q_i q_j (erfc + R)/r, 4 phi_i = sum_j q_j (erfc + R)/r,
+// 5 sum_j q_i q_j d(R/r)/dlambda, 6..8 dF_i/dlambda (eps included).
+constexpr int kAdjQ = 9;
+struct AdjArgs {
+    PosSoA p;
+    const double* qp;
+    int n, ne, jchunk, flags;
+    double box[3], inv_box[3];
+    double rc2, alpha, rs, eps;
+    double* part;              // [nchunk][kAdjQ][n]
+};
+void launch_pair_adjoint(const AdjArgs& a, int nchunk, hipStream_t s);
+// out[k * n + i] = sum over chunks of part, chunk order
+void launch_adjoint_reduce(const double* part, int nchunk, int n, double* out, hipStream_t s);
 
 // Integrator / reductions
 struct MdArgs {

@@ -37,6 +37,8 @@ SIGNATURES = {
     "tcp_md_steps": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _dp],
     "tcp_last_timings": [_vp, _dp, _dp],
+    "tcp_set_lambda": [_vp, ctypes.c_double],
+    "tcp_energy_adjoint": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
 }
 
 _LIB = None
@@ -185,6 +187,27 @@ class Context:
                                        int(self.seed if seed is None else seed), _ptr(tr)),
                  "tcp_md_steps")
         return tr
+
+    # -- extensions with no reference counterpart --------------------------------
+    def set_lambda(self, lam):
+        self._ck(self.lib.tcp_set_lambda(self.h, float(lam)), "tcp_set_lambda")
+        self.p = self.p.replace(rs=float(lam))
+
+    def energy_adjoint(self, q=None, with_dfdl=True):
+        """(U, dU/dx, dU/dq, dU/dlambda, dF/dlambda or None) at the current positions."""
+        n = self.p.n
+        qa = None
+        if q is not None:
+            qa = np.ascontiguousarray(q, dtype=np.float64)
+            if qa.shape != (n,):
+                raise ValueError(f"expected {n} charges, got shape {qa.shape}")
+        U, dl = ctypes.c_double(), ctypes.c_double()
+        dudx = np.zeros((n, 3))
+        dudq = np.zeros(n)
+        dfdl = np.zeros((n, 3)) if with_dfdl else None
+        self._ck(self.lib.tcp_energy_adjoint(self.h, _ptr(qa), ctypes.byref(U), _ptr(dudx), _ptr(dudq),
+                                             ctypes.byref(dl), _ptr(dfdl)), "tcp_energy_adjoint")
+        return U.value, dudx, dudq, dl.value, dfdl
 
     def timings(self):
         a, b = ctypes.c_double(), ctypes.c_double()
