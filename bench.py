@@ -19,7 +19,10 @@ unordered pair: 62.5 per pair plus 6 for the Newton's-third-law reaction) over
 its average launch time from HIP events on its stream, against the
 157.3 TFLOP/s FP32 vector peak.  cpu_baseline: the reference's own
 Force_direct_ext (compiled from source, oracle/_ref) timed on the host cores on
-a bounded row slice of the same workload.
+a bounded row slice of the same workload.  batch: the 64-replica batching
+workload of tcp_allpair_amd/batchbench.py (one persistent pair launch and a
+captured step for all replicas) against the same replicas stepped one
+context after another.
 """
 from __future__ import annotations
 
@@ -119,6 +122,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fp64-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=10,
+                    help="steps of the 64-replica batching workload (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -149,6 +154,11 @@ def main():
         for line in open(PROFILE_TRAFFIC):
             if line.startswith("hbm_bytes_per_launch:"):
                 traffic = float(line.split(":")[1])
+    batch = None
+    if args.batch_steps > 0:
+        print("[bench] batching workload (64 replicas)", file=sys.stderr, flush=True)
+        from tcp_allpair_amd import batchbench
+        batch = batchbench.run(args.batch_steps, 2)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (reference on host cores)", file=sys.stderr, flush=True)
@@ -185,6 +195,7 @@ def main():
                      "traffic": traffic, "kernel": "pair_mixed_n3_kernel<true>",
                      "kernel_ms": mx["pair_ms"], "flop_per_launch": flops},
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(out))
 

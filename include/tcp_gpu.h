@@ -151,6 +151,36 @@ int tcp_energy_adjoint(tcp_ctx* ctx, const double* q, double* U, double* dUdx, d
  * last tcp_md_steps call (HIP events on the context's stream). */
 int tcp_last_timings(tcp_ctx* ctx, double* pair_ms, double* steps_ms);
 
+/* ---------------------------------------------------------------- batches ---
+ * Many independent plasmas of different sizes stepped together, for ensembles
+ * and parameter scans.  No counterpart in the reference, which runs one system
+ * per job.  Each replica behaves like its own context (same state layout, same
+ * histograms); one call steps all of them with one persistent pair launch per
+ * step for all replicas and the step captured in a hipGraph.  Replicas must
+ * share recip (TCP_RECIP_NONE or TCP_RECIP_EWALD), kmax and flags.  In FP64
+ * every replica reproduces tcp_md_steps on its own context bit for bit. */
+typedef struct tcp_batch tcp_batch;
+
+int tcp_batch_create(tcp_batch** out, int nrep, const tcp_params* p /* [nrep] */, int precision);
+int tcp_batch_destroy(tcp_batch* b);
+const char* tcp_batch_last_error(const tcp_batch* b);
+/* tcp_set_state / tcp_get_state / tcp_get_histograms of replica rep */
+int tcp_batch_set_state(tcp_batch* b, int rep, const double* x, const double* v, const double* f);
+int tcp_batch_get_state(tcp_batch* b, int rep, double* x, double* v, double* f);
+int tcp_batch_get_histograms(tcp_batch* b, int rep, double* rdf, double* vdf, int reset);
+/* tcp_md_steps for every replica; Te[nrep], Ti[nrep] (may be NULL for
+ * TCP_NOTEMP); trace (may be NULL) receives [nsteps][nrep][6] doubles. */
+int tcp_batch_md_steps(tcp_batch* b, int nsteps, int thermo, double dt, const double* Te,
+                       const double* Ti, double tau, double fric, uint64_t seed, double* trace);
+/* Block shape of the persistent pair kernel (TCP_MIXED: 0..3 = 16x16 tiles by
+ * 8 waves, 8x8 by 8, 8x8 by 4, 4x4 by 4; TCP_FP64: 0 only). */
+int tcp_batch_set_shape(tcp_batch* b, int shape);
+/* Times `reps` pair launches per shape on the current positions, keeps the
+ * fastest; *best = its index, ms[4] (may be NULL) = time per launch and shape. */
+int tcp_batch_autotune(tcp_batch* b, int reps, int* best, double* ms);
+/* pair_ms: best autotuned pair launch; steps_ms: last tcp_batch_md_steps call. */
+int tcp_batch_last_timings(tcp_batch* b, double* pair_ms, double* steps_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,7 +27,7 @@ FLAGS = [
     "-Wall", "-Wno-unused-function",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
-SOURCES = ["kernels_pair.hip", "kernels_spme.hip", "kernels_md.hip", "tcp_gpu.hip"]
+SOURCES = ["kernels_pair.hip", "kernels_spme.hip", "kernels_md.hip", "tcp_gpu.hip", "batch.hip"]
 PROBE = os.path.join(HERE, "copy_probe")   # HBM-counter calibration program
 
 

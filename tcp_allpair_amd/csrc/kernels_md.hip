@@ -16,7 +16,7 @@ namespace {
 
 // Block sum of W values per thread into part[blockIdx.x * W + c], fixed order.
 template <int W>
-__device__ inline void block_sum_store(double (&v)[W], double* part)
+__device__ inline void block_sum_store(double (&v)[W], double* part, int blk)
 {
     __shared__ double red[W][kRedBlock];
 #pragma unroll
@@ -31,7 +31,7 @@ __device__ inline void block_sum_store(double (&v)[W], double* part)
     }
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int c = 0; c < W; ++c) part[blockIdx.x * W + c] = red[c][0];
+        for (int c = 0; c < W; ++c) part[blk * W + c] = red[c][0];
     }
 }
 
@@ -74,10 +74,10 @@ __device__ inline FixPos make_fixpos(double x, double y, double z, const double*
     return f;
 }
 
-__global__ __launch_bounds__(kRedBlock) void vv1_kernel(MdArgs m, int thermo, double dt, double fric)
+__device__ inline void vv1_body(const MdArgs& m, int blk, int thermo, double dt, double fric)
 {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * kRedBlock + threadIdx.x;
     if (i >= m.n) return;
     const double xm = m.xm[i >= m.ne];
     double* xs[3] = {m.x, m.y, m.z};
@@ -99,6 +99,11 @@ __global__ __launch_bounds__(kRedBlock) void vv1_kernel(MdArgs m, int thermo, do
     if (m.upos) m.upos[i] = make_fixpos(xn[0], xn[1], xn[2], m.box);
 }
 
+__global__ __launch_bounds__(kRedBlock) void vv1_kernel(MdArgs m, int thermo, double dt, double fric)
+{
+    vv1_body(m, blockIdx.x, thermo, dt, fric);
+}
+
 __global__ __launch_bounds__(kRedBlock) void upos_kernel(MdArgs m)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,13 +113,11 @@ __global__ __launch_bounds__(kRedBlock) void upos_kernel(MdArgs m)
 
 // f = sum over chunks of the pair partials (+ reciprocal forces), block
 // partials of sum f (3) and of sum u (1).
-__global__ __launch_bounds__(kRedBlock) void combine_kernel(MdArgs m, const double* pf,
-                                                            const double* pu, int nchunk,
-                                                            const double* frx, const double* fry,
-                                                            const double* frz, double* part,
-                                                            int accumulate)
+__device__ inline void combine_body(const MdArgs& m, int blk, const double* pf, const double* pu,
+                                    int nchunk, const double* frx, const double* fry,
+                                    const double* frz, double* part, int accumulate)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * kRedBlock + threadIdx.x;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (i < m.n) {
         const size_t n = m.n;
@@ -141,12 +144,20 @@ __global__ __launch_bounds__(kRedBlock) void combine_kernel(MdArgs m, const doub
         m.fz[i] = f3[2];
         v[0] = f3[0]; v[1] = f3[1]; v[2] = f3[2]; v[3] = u;
     }
-    block_sum_store<4>(v, part);
+    block_sum_store<4>(v, part, blk);
+}
+
+__global__ __launch_bounds__(kRedBlock) void combine_kernel(MdArgs m, const double* pf,
+                                                            const double* pu, int nchunk,
+                                                            const double* frx, const double* fry,
+                                                            const double* frz, double* part,
+                                                            int accumulate)
+{
+    combine_body(m, blockIdx.x, pf, pu, nchunk, frx, fry, frz, part, accumulate);
 }
 
 // sums[c] = sum over blocks of part[b * width + c], in block order (one block).
-__global__ __launch_bounds__(kRedBlock) void reduce_parts_kernel(const double* part, int nblk,
-                                                                 int width, double* sums)
+__device__ inline void reduce_parts_body(const double* part, int nblk, int width, double* sums)
 {
     __shared__ double red[kRedBlock];
     for (int c = 0; c < width; ++c) {
@@ -161,6 +172,12 @@ __global__ __launch_bounds__(kRedBlock) void reduce_parts_kernel(const double* p
         if (threadIdx.x == 0) sums[c] = red[0];
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(kRedBlock) void reduce_parts_kernel(const double* part, int nblk,
+                                                                 int width, double* sums)
+{
+    reduce_parts_body(part, nblk, width, sums);
 }
 
 // f += fr (SPME_SUM alone)
@@ -180,7 +197,7 @@ __global__ __launch_bounds__(kRedBlock) void force_sum_kernel(MdArgs m, double* 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
     if (i < m.n) { v[0] = m.fx[i]; v[1] = m.fy[i]; v[2] = m.fz[i]; }
-    block_sum_store<3>(v, part);
+    block_sum_store<3>(v, part, blockIdx.x);
 }
 
 // REMOVE_RIGIDMTN alone: f -= fsum / N (force.f90:996-1004)
@@ -216,10 +233,9 @@ __device__ inline double gauss(uint64_t seed, uint64_t counter, uint64_t idx)
 // Second half kick (vverlet.f90:190-219, 245-253, 319-329, 85-98), with the
 // mean-force removal fused in front when fsum != null.  part[blk*2 + s]:
 // BEREND/ISOKIN: sum m v_k^2 before rescale; NOTEMP/STOCH: sum m v^2 (final).
-__global__ __launch_bounds__(kRedBlock) void vv2_kick_kernel(MdArgs m, int thermo, double dt,
-                                                             double Te, double Ti, double fric,
-                                                             uint64_t seed, uint64_t counter,
-                                                             const double* fsum, double* part)
+__device__ inline void vv2_kick_body(const MdArgs& m, int blk, int thermo, double dt, double Te,
+                                     double Ti, double fric, uint64_t seed, uint64_t counter,
+                                     const double* fsum, double* part)
 {
     __shared__ unsigned hist[2 * kHistStride];
     const bool final_now = thermo == TCP_NOTEMP || thermo == TCP_STOCH;
@@ -227,7 +243,7 @@ __global__ __launch_bounds__(kRedBlock) void vv2_kick_kernel(MdArgs m, int therm
         vhist_clear(hist);
         __syncthreads();
     }
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * kRedBlock + threadIdx.x;
     double acc[2] = {0.0, 0.0};
     if (i < m.n) {
 #pragma clang fp contract(off)
@@ -266,16 +282,24 @@ __global__ __launch_bounds__(kRedBlock) void vv2_kick_kernel(MdArgs m, int therm
             acc[s] = a;
         }
     }
-    block_sum_store<2>(acc, part);
+    block_sum_store<2>(acc, part, blk);
     if (final_now) {
         __syncthreads();
         vhist_flush(hist, m.vdf);
     }
 }
 
+__global__ __launch_bounds__(kRedBlock) void vv2_kick_kernel(MdArgs m, int thermo, double dt,
+                                                             double Te, double Ti, double fric,
+                                                             uint64_t seed, uint64_t counter,
+                                                             const double* fsum, double* part)
+{
+    vv2_kick_body(m, blockIdx.x, thermo, dt, Te, Ti, fric, seed, counter, fsum, part);
+}
+
 // lambda per species from the pre-rescale kinetic sums (vverlet.f90:256-258, 330-332).
-__global__ void thermo_lambda_kernel(const double* mv2, int thermo, int ne, int ni, double Te,
-                                     double Ti, double dt, double tau, double* lam)
+__device__ inline void thermo_lambda_body(const double* mv2, int thermo, int ne, int ni, double Te,
+                                          double Ti, double dt, double tau, double* lam)
 {
 #pragma clang fp contract(off)
     if (threadIdx.x != 0) return;
@@ -287,13 +311,18 @@ __global__ void thermo_lambda_kernel(const double* mv2, int thermo, int ne, int 
     }
 }
 
-__global__ __launch_bounds__(kRedBlock) void vv2_scale_kernel(MdArgs m, const double* lam,
-                                                              double* part)
+__global__ void thermo_lambda_kernel(const double* mv2, int thermo, int ne, int ni, double Te,
+                                     double Ti, double dt, double tau, double* lam)
+{
+    thermo_lambda_body(mv2, thermo, ne, ni, Te, Ti, dt, tau, lam);
+}
+
+__device__ inline void vv2_scale_body(const MdArgs& m, int blk, const double* lam, double* part)
 {
     __shared__ unsigned hist[2 * kHistStride];
     vhist_clear(hist);
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * kRedBlock + threadIdx.x;
     double acc[2] = {0.0, 0.0};
     if (i < m.n) {
 #pragma clang fp contract(off)
@@ -305,14 +334,20 @@ __global__ __launch_bounds__(kRedBlock) void vv2_scale_kernel(MdArgs m, const do
         acc[s] = mv2;
         vdf_count(hist, s, mv2, m.dv[s]);
     }
-    block_sum_store<2>(acc, part);
+    block_sum_store<2>(acc, part, blk);
     __syncthreads();
     vhist_flush(hist, m.vdf);
 }
 
+__global__ __launch_bounds__(kRedBlock) void vv2_scale_kernel(MdArgs m, const double* lam,
+                                                              double* part)
+{
+    vv2_scale_body(m, blockIdx.x, lam, part);
+}
+
 // One row of the per-step trace: Ur, Uo, Um, Epot = (Ur + Uo) eps, mv2el, mv2ion.
-__global__ void trace_kernel(double* row, const double* usum, double ur_scale, double Uo,
-                             const double* um, const double* mv2, double eps)
+__device__ inline void trace_body(double* row, const double* usum, double ur_scale, double Uo,
+                                  const double* um, const double* mv2, double eps)
 {
 #pragma clang fp contract(off)
     if (threadIdx.x != 0) return;
@@ -325,9 +360,14 @@ __global__ void trace_kernel(double* row, const double* usum, double ur_scale, d
     row[5] = mv2[1];
 }
 
+__global__ void trace_kernel(double* row, const double* usum, double ur_scale, double Uo,
+                             const double* um, const double* mv2, double eps)
+{
+    trace_body(row, usum, ur_scale, Uo, um, mv2, eps);
+}
+
 // Um = eps/(2 pi V) * sum of the energy partials, one block, fixed order.
-__global__ __launch_bounds__(kRedBlock) void um_kernel(const double* epart, int n, double scale,
-                                                       double* um)
+__device__ inline void um_body(const double* epart, int n, double scale, double* um)
 {
     __shared__ double red[kRedBlock];
     double s = 0.0;
@@ -339,6 +379,99 @@ __global__ __launch_bounds__(kRedBlock) void um_kernel(const double* epart, int 
         __syncthreads();
     }
     if (threadIdx.x == 0) um[0] = red[0] * scale;
+}
+
+__global__ __launch_bounds__(kRedBlock) void um_kernel(const double* epart, int n, double scale,
+                                                       double* um)
+{
+    um_body(epart, n, scale, um);
+}
+
+// ------------------------------------------------------------- batches ---
+//
+// Many independent replicas per launch: blockIdx.y selects the replica, whose
+// arguments live in a device table, and blockIdx.x is the block index the
+// single-replica kernel would have had (blocks past the replica's size exit
+// at once).  The bodies are the ones above, so every replica computes exactly
+// what a launch for it alone computes.
+
+__device__ inline bool batch_block_live(const BatchRep& r)
+{
+    return (int)blockIdx.x < md_blocks_dev(r.m.n);
+}
+
+__global__ __launch_bounds__(kRedBlock) void vv1_batch_kernel(const BatchRep* reps, int thermo,
+                                                              double dt, double fric)
+{
+    const BatchRep& r = reps[blockIdx.y];
+    if (!batch_block_live(r)) return;
+    vv1_body(r.m, blockIdx.x, thermo, dt, fric);
+}
+
+__global__ __launch_bounds__(kRedBlock) void combine_batch_kernel(const BatchRep* reps)
+{
+    const BatchRep& r = reps[blockIdx.y];
+    if (!batch_block_live(r)) return;
+    const bool rec = r.recip != TCP_RECIP_NONE;
+    combine_body(r.m, blockIdx.x, r.pa.pf, r.pa.pu, r.slots, rec ? r.ra.frx : nullptr,
+                 rec ? r.ra.fry : nullptr, rec ? r.ra.frz : nullptr, r.part, 0);
+}
+
+__global__ __launch_bounds__(kRedBlock) void reduce_parts_batch_kernel(const BatchRep* reps,
+                                                                       int width, int slot)
+{
+    const BatchRep& r = reps[blockIdx.x];
+    reduce_parts_body(r.part, md_blocks_dev(r.m.n), width, r.sc + slot);
+}
+
+__global__ __launch_bounds__(kRedBlock) void vv2_kick_batch_kernel(const BatchRep* reps, int thermo,
+                                                                   double dt, double fric,
+                                                                   uint64_t seed,
+                                                                   const uint64_t* counter)
+{
+    const BatchRep& r = reps[blockIdx.y];
+    if (!batch_block_live(r)) return;
+    vv2_kick_body(r.m, blockIdx.x, thermo, dt, r.Te, r.Ti, fric, seed, counter[0],
+                  r.sc + SC_FSUM, r.part);
+}
+
+__global__ void thermo_lambda_batch_kernel(const BatchRep* reps, int thermo, double dt, double tau)
+{
+    const BatchRep& r = reps[blockIdx.x];
+    thermo_lambda_body(r.sc + SC_MV2, thermo, r.m.ne, r.m.n - r.m.ne, r.Te, r.Ti, dt, tau,
+                       r.sc + SC_LAM);
+}
+
+__global__ __launch_bounds__(kRedBlock) void vv2_scale_batch_kernel(const BatchRep* reps)
+{
+    const BatchRep& r = reps[blockIdx.y];
+    if (!batch_block_live(r)) return;
+    vv2_scale_body(r.m, blockIdx.x, r.sc + SC_LAM, r.part);
+}
+
+__global__ void trace_batch_kernel(const BatchRep* reps, int nrep, double* trace,
+                                   const uint64_t* counter)
+{
+    const BatchRep& r = reps[blockIdx.x];
+    double* row = trace + 6 * ((size_t)counter[1] * nrep + blockIdx.x);
+    trace_body(row, r.sc + SC_USUM, r.ur_scale, r.Uo,
+               r.recip != TCP_RECIP_NONE ? r.sc + SC_UM : nullptr, r.sc + SC_MV2, r.eps);
+}
+
+__global__ __launch_bounds__(kRedBlock) void um_batch_kernel(const BatchRep* reps)
+{
+    const BatchRep& r = reps[blockIdx.x];
+    if (r.recip == TCP_RECIP_NONE) return;
+    um_body(r.ra.epart, 1, r.um_scale, r.sc + SC_UM);
+}
+
+// counter[0]: step counter of the Gaussian noise, counter[1]: trace row
+__global__ void step_advance_kernel(uint64_t* counter)
+{
+    if (threadIdx.x == 0) {
+        counter[0] += 1;
+        counter[1] += 1;
+    }
 }
 
 }  // namespace
@@ -414,6 +547,56 @@ void launch_trace(double* row, const double* usum, double ur_scale, double Uo, c
 void launch_um(const double* epart, int n, double scale, double* um, hipStream_t s)
 {
     hipLaunchKernelGGL(um_kernel, dim3(1), dim3(kRedBlock), 0, s, epart, n, scale, um);
+}
+
+void launch_vv1_batch(const BatchRep* reps, int nrep, int maxblk, int thermo, double dt,
+                      double fric, hipStream_t s)
+{
+    hipLaunchKernelGGL(vv1_batch_kernel, dim3(maxblk, nrep), dim3(kRedBlock), 0, s, reps, thermo,
+                       dt, fric);
+}
+
+void launch_combine_batch(const BatchRep* reps, int nrep, int maxblk, hipStream_t s)
+{
+    hipLaunchKernelGGL(combine_batch_kernel, dim3(maxblk, nrep), dim3(kRedBlock), 0, s, reps);
+}
+
+void launch_reduce_parts_batch(const BatchRep* reps, int nrep, int width, int slot, hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_parts_batch_kernel, dim3(nrep), dim3(kRedBlock), 0, s, reps, width,
+                       slot);
+}
+
+void launch_vv2_batch(const BatchRep* reps, int nrep, int maxblk, int thermo, double dt, double tau,
+                      double fric, uint64_t seed, const uint64_t* counter, hipStream_t s)
+{
+    hipLaunchKernelGGL(vv2_kick_batch_kernel, dim3(maxblk, nrep), dim3(kRedBlock), 0, s, reps,
+                       thermo, dt, fric, seed, counter);
+    if (thermo == TCP_BEREND || thermo == TCP_ISOKIN) {
+        hipLaunchKernelGGL(reduce_parts_batch_kernel, dim3(nrep), dim3(kRedBlock), 0, s, reps, 2,
+                           (int)SC_MV2);
+        hipLaunchKernelGGL(thermo_lambda_batch_kernel, dim3(nrep), dim3(64), 0, s, reps, thermo, dt,
+                           tau);
+        hipLaunchKernelGGL(vv2_scale_batch_kernel, dim3(maxblk, nrep), dim3(kRedBlock), 0, s, reps);
+    }
+    hipLaunchKernelGGL(reduce_parts_batch_kernel, dim3(nrep), dim3(kRedBlock), 0, s, reps, 2,
+                       (int)SC_MV2);
+}
+
+void launch_um_batch(const BatchRep* reps, int nrep, hipStream_t s)
+{
+    hipLaunchKernelGGL(um_batch_kernel, dim3(nrep), dim3(kRedBlock), 0, s, reps);
+}
+
+void launch_trace_batch(const BatchRep* reps, int nrep, double* trace, const uint64_t* counter,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL(trace_batch_kernel, dim3(nrep), dim3(64), 0, s, reps, nrep, trace, counter);
+}
+
+void launch_step_advance(uint64_t* counter, hipStream_t s)
+{
+    hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, s, counter);
 }
 
 }  // namespace tcp

@@ -36,20 +36,20 @@ __device__ inline void hist_flush(const unsigned* h, unsigned long long* g, unsi
 
 // ------------------------------------------------------------------ FP64 ---
 
+// One block: i block bx against j chunk by.  hist: the block's LDS histogram.
 template <bool kRdf>
-__global__ __launch_bounds__(kPairBlock) void pair_f64_kernel(PairArgs a)
+__device__ inline void f64_block(const PairArgs& a, int bx, int by, unsigned* hist)
 {
-    __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
     if (kRdf) {
         hist_clear(hist);
         __syncthreads();
     }
-    const int i = blockIdx.x * kPairBlock + threadIdx.x;
+    const int i = bx * kPairBlock + threadIdx.x;
     const bool live = i < a.n_i;
     const int ii = live ? i : a.n_i - 1;
     const double xi = a.pi.x[ii], yi = a.pi.y[ii], zi = a.pi.z[ii];
     const int si = ii >= a.ne_i;
-    const int j0 = blockIdx.y * a.jchunk;
+    const int j0 = by * a.jchunk;
     const int j1 = min(a.n_j, j0 + a.jchunk);
     const double Lx = a.box[0], Ly = a.box[1], Lz = a.box[2];
     const double iLx = a.inv_box[0], iLy = a.inv_box[1], iLz = a.inv_box[2];
@@ -116,16 +116,23 @@ __global__ __launch_bounds__(kPairBlock) void pair_f64_kernel(PairArgs a)
     }
     if (live) {
         const size_t n = a.n_i;
-        double* pf = a.pf + (size_t)blockIdx.y * 3 * n;
+        double* pf = a.pf + (size_t)by * 3 * n;
         pf[i] = fx;
         pf[n + i] = fy;
         pf[2 * n + i] = fz;
-        a.pu[(size_t)blockIdx.y * n + i] = u;
+        a.pu[(size_t)by * n + i] = u;
     }
     if (kRdf) {
         __syncthreads();
         hist_flush(hist, a.rdf, a.rdf_weight);
     }
+}
+
+template <bool kRdf>
+__global__ __launch_bounds__(kPairBlock) void pair_f64_kernel(PairArgs a)
+{
+    __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
+    f64_block<kRdf>(a, blockIdx.x, blockIdx.y, hist);
 }
 
 // --------------------------------------------------------------- ADJOINT ---
@@ -591,16 +598,13 @@ __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* h
     uo = u;
 }
 
-template <bool kRdf, bool kKelbg>
-__global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArgs a,
-                                                                         const int2* blkmap)
+// One block pair (row br, column bc) of kBT x kBT tile pairs by kW wavefronts.
+// jpos, racc, hist: the block's LDS (kBT * kTile particles).
+template <bool kRdf, bool kKelbg, int kBT, int kW>
+__device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
+                                double (*racc)[kBT * kTile], unsigned* hist)
 {
-    constexpr int kBP = kBlkTiles * kTile;  // particles per block side
-    __shared__ FixPos jpos[kBP];
-    __shared__ double racc[4][kBP];
-    __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
-    const int2 rc = blkmap[blockIdx.x];
-    const int br = rc.x, bc = rc.y;
+    constexpr int kBP = kBT * kTile;  // particles per block side
     const bool dblk = br == bc;
     const int n = a.n_i, ne = a.ne_i;
     for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
@@ -616,9 +620,9 @@ __global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArg
     const MixConst k = mix_const(a);
     const int wave = threadIdx.x / kTile, lane = threadIdx.x % kTile;
     const int ntiles = (n + kTile - 1) / kTile;
-    for (int h = 0; h < kBlkTiles / kN3Waves; ++h) {
-        const int ti = wave + h * kN3Waves;
-        const int I = br * kBlkTiles + ti;
+    for (int h = 0; h < kBT / kW; ++h) {
+        const int ti = wave + h * kW;
+        const int I = br * kBT + ti;
         if (I >= ntiles) continue;
         const int ip = I * kTile + lane;
         const bool ilive = ip < n;
@@ -627,8 +631,8 @@ __global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArg
         // the i tile holds one species and only live particles
         const bool iuni = (I * kTile >= ne || I * kTile + kTile - 1 < ne) && I * kTile + kTile - 1 < n;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0, U = 0.0;
-        for (int tj = dblk ? ti : 0; tj < kBlkTiles; ++tj) {
-            const int J = bc * kBlkTiles + tj;
+        for (int tj = dblk ? ti : 0; tj < kBT; ++tj) {
+            const int J = bc * kBT + tj;
             if (J >= ntiles) break;
             float fx, fy, fz, u, rx, ry, rz;
             const FixPos pj = jpos[tj * kTile + lane];
@@ -685,6 +689,79 @@ __global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArg
     }
 }
 
+template <bool kRdf, bool kKelbg>
+__global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArgs a,
+                                                                         const int2* blkmap)
+{
+    __shared__ FixPos jpos[kBlkTiles * kTile];
+    __shared__ double racc[4][kBlkTiles * kTile];
+    __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
+    const int2 rc = blkmap[blockIdx.x];
+    n3_block<kRdf, kKelbg, kBlkTiles, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist);
+}
+
+// ------------------------------------------------ persistent (batches) ---
+//
+// A fixed grid of resident blocks takes work items (replica, block pair) from
+// a device queue in list order (the host sorts the list by decreasing cost),
+// so replicas of very different sizes share one launch.  The item body is the
+// single-context kernel's, so a replica's partial slots hold exactly what its
+// own launch would write.  The last block to finish resets the queue, which
+// keeps the launch free of memset nodes when it is captured in a graph.
+
+__device__ inline int queue_next(int* queue, int* slot)
+{
+    __syncthreads();  // every thread has read the previous item, LDS is free again
+    if (threadIdx.x == 0) *slot = atomicAdd(queue, 1);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*slot);
+}
+
+__device__ inline void queue_done(int* queue)
+{
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(queue + 1, 1) == (int)gridDim.x - 1) {
+            atomicExch(queue, 0);
+            atomicExch(queue + 1, 0);
+        }
+    }
+}
+
+template <bool kRdf, bool kKelbg, int kBT, int kW>
+__global__ __launch_bounds__(kW * kTile) void pair_mixed_n3_batch_kernel(const BatchRep* reps,
+                                                                         const PairItem* items,
+                                                                         int nitems, int* queue)
+{
+    __shared__ FixPos jpos[kBT * kTile];
+    __shared__ double racc[4][kBT * kTile];
+    __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
+    __shared__ int slot;
+    for (;;) {
+        const int it = queue_next(queue, &slot);
+        if (it >= nitems) break;
+        const PairItem w = items[it];
+        n3_block<kRdf, kKelbg, kBT, kW>(reps[w.rep].pa, w.a, w.b, jpos, racc, hist);
+    }
+    queue_done(queue);
+}
+
+template <bool kRdf>
+__global__ __launch_bounds__(kPairBlock) void pair_f64_batch_kernel(const BatchRep* reps,
+                                                                    const PairItem* items,
+                                                                    int nitems, int* queue)
+{
+    __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
+    __shared__ int slot;
+    for (;;) {
+        const int it = queue_next(queue, &slot);
+        if (it >= nitems) break;
+        const PairItem w = items[it];
+        f64_block<kRdf>(reps[w.rep].pa, w.a, w.b, hist);
+    }
+    queue_done(queue);
+}
+
 }  // namespace
 
 void launch_pair_f64(const PairArgs& a, int nchunk, hipStream_t s)
@@ -739,6 +816,76 @@ void launch_pair_mixed(const PairArgs& a, int nchunk, hipStream_t s)
         if (kel) hipLaunchKernelGGL((pair_mixed_kernel<false, true>), grid, b, 0, s, a);
         else hipLaunchKernelGGL((pair_mixed_kernel<false, false>), grid, b, 0, s, a);
     }
+}
+
+N3Shape n3_shape(int shape)
+{
+    static const N3Shape shapes[kNumN3Shapes] = {{kBlkTiles, kN3Waves}, {8, 8}, {8, 4}, {4, 4}};
+    return shapes[shape < 0 || shape >= kNumN3Shapes ? 0 : shape];
+}
+
+int n3_block_rows_shape(int n, int shape)
+{
+    const int ntiles = (n + kTile - 1) / kTile;
+    const int bt = n3_shape(shape).tiles;
+    return (ntiles + bt - 1) / bt;
+}
+
+namespace {
+
+template <bool kRdf, bool kKelbg>
+const void* n3_batch_fn(int shape)
+{
+    switch (shape) {
+        case 1: return (const void*)pair_mixed_n3_batch_kernel<kRdf, kKelbg, 8, 8>;
+        case 2: return (const void*)pair_mixed_n3_batch_kernel<kRdf, kKelbg, 8, 4>;
+        case 3: return (const void*)pair_mixed_n3_batch_kernel<kRdf, kKelbg, 4, 4>;
+        default: return (const void*)pair_mixed_n3_batch_kernel<kRdf, kKelbg, kBlkTiles, kN3Waves>;
+    }
+}
+
+const void* n3_batch_kernel(int shape, bool rdf, bool kelbg)
+{
+    if (rdf) return kelbg ? n3_batch_fn<true, true>(shape) : n3_batch_fn<true, false>(shape);
+    return kelbg ? n3_batch_fn<false, true>(shape) : n3_batch_fn<false, false>(shape);
+}
+
+}  // namespace
+
+void launch_pair_mixed_n3_batch(const BatchRep* reps, const PairItem* items, int nitems,
+                                int* queue, int shape, bool rdf, bool kelbg, int grid,
+                                hipStream_t s)
+{
+    const dim3 g(grid), b(n3_shape(shape).waves * kTile);
+    void* args[] = {(void*)&reps, (void*)&items, (void*)&nitems, (void*)&queue};
+    (void)hipLaunchKernel(n3_batch_kernel(shape, rdf, kelbg), g, b, args, 0, s);
+}
+
+void launch_pair_f64_batch(const BatchRep* reps, const PairItem* items, int nitems, int* queue,
+                           bool rdf, int grid, hipStream_t s)
+{
+    if (rdf)
+        hipLaunchKernelGGL(pair_f64_batch_kernel<true>, dim3(grid), dim3(kPairBlock), 0, s, reps,
+                           items, nitems, queue);
+    else
+        hipLaunchKernelGGL(pair_f64_batch_kernel<false>, dim3(grid), dim3(kPairBlock), 0, s, reps,
+                           items, nitems, queue);
+}
+
+int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg)
+{
+    int nb = 0;
+    if (mixed) {
+        const void* f = n3_batch_kernel(shape, rdf, kelbg);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, n3_shape(shape).waves * kTile, 0) !=
+            hipSuccess)
+            nb = 1;
+    } else {
+        const void* f = rdf ? (const void*)pair_f64_batch_kernel<true>
+                            : (const void*)pair_f64_batch_kernel<false>;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kPairBlock, 0) != hipSuccess) nb = 1;
+    }
+    return nb < 1 ? 1 : nb;
 }
 
 }  // namespace tcp

@@ -460,88 +460,211 @@ __global__ void fill_kernel(double* p, size_t n, double v)
 }
 
 // ------------------------------------------------------ direct k-space sum ---
+//
+// Only the half space of m is summed (linear index of the (2 kmax + 1)^3 cube
+// above its centre, i.e. mz > 0, or mz = 0 and my > 0, or mz = my = 0 and
+// mx > 0): S(-m) = conj S(m), so both the energy and the forces are twice the
+// half-space sums.  Phases e^{2 pi i m x/L} come from one sincos per particle
+// and axis and complex-multiplication recurrences.
+//
+// S(m): blocks of `chunk` particles write per-chunk partials of every half-space
+// m (phase tables of the chunk in LDS, one thread per m), then one thread per
+// m sums the chunks in order, so S is independent of scheduling.
 
-// S(m) for every m in the (2 kmax + 1)^3 cube: one block per m, a block-wide sum.
-__global__ __launch_bounds__(256) void ewald_sk_kernel(RecipArgs a, int kmax, double alpha,
-                                                       double* sk)
+__device__ inline double2 cexpi(double th)
 {
-    __shared__ double rs[256], is[256];
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    return make_double2(cs, sn);
+}
+
+__device__ inline void decode_m(int mi, int kmax, int* mx, int* my, int* mz)
+{
     const int nk = 2 * kmax + 1;
-    const int mi = blockIdx.x;
-    const int mx = mi % nk - kmax, my = (mi / nk) % nk - kmax, mz = mi / (nk * nk) - kmax;
-    const double kx = mx / a.box[0], ky = my / a.box[1], kz = mz / a.box[2];
-    double sr = 0.0, si = 0.0;
-    for (int j = threadIdx.x; j < a.n; j += blockDim.x) {
-        const double th = 2.0 * kPi * (kx * a.x[j] + ky * a.y[j] + kz * a.z[j]);
-        double sn, cs;
-        sincos(th, &sn, &cs);
-        const double q = charge(a, j);
-        sr += q * cs;
-        si += q * sn;
-    }
-    rs[threadIdx.x] = sr;
-    is[threadIdx.x] = si;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            rs[threadIdx.x] += rs[threadIdx.x + w];
-            is[threadIdx.x] += is[threadIdx.x + w];
+    *mx = mi % nk - kmax;
+    *my = (mi / nk) % nk - kmax;
+    *mz = mi / (nk * nk) - kmax;
+}
+
+// ph: LDS table [3][kmax + 1][chunk] of q^(d == 0) e^{i m theta_d}, m = 0..kmax
+__device__ inline void ewald_sk_part_body(const RecipArgs& a, int kmax, int chunk, int c,
+                                          double2* ph)
+{
+    const int K1 = kmax + 1;
+    for (int t = threadIdx.x; t < 3 * chunk; t += blockDim.x) {
+        const int d = t / chunk, p = t % chunk, j = c * chunk + p;
+        const double* xd = d == 0 ? a.x : (d == 1 ? a.y : a.z);
+        double2 e = make_double2(j < a.n ? (d == 0 ? charge(a, j) : 1.0) : 0.0, 0.0);
+        const double2 u = cexpi(2.0 * kPi * (j < a.n ? xd[j] : 0.0) / a.box[d]);
+        for (int m = 0; m < K1; ++m) {
+            ph[(d * K1 + m) * chunk + p] = e;
+            e = cmul(e, u);
         }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const double m2 = kx * kx + ky * ky + kz * kz;
-        const double G = (mx == 0 && my == 0 && mz == 0)
-                             ? 0.0 : exp(-kPi * kPi * m2 / (alpha * alpha)) / m2;
-        sk[3 * mi + 0] = rs[0];
-        sk[3 * mi + 1] = is[0];
-        sk[3 * mi + 2] = G;
+    __syncthreads();
+    const int nk = 2 * kmax + 1, nh = (nk * nk * nk - 1) / 2;
+    for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+        int mx, my, mz;
+        decode_m(nh + 1 + h, kmax, &mx, &my, &mz);
+        const double sx = mx < 0 ? -1.0 : 1.0, sy = my < 0 ? -1.0 : 1.0;
+        const double2* X = ph + (0 * K1 + abs(mx)) * chunk;
+        const double2* Y = ph + (1 * K1 + abs(my)) * chunk;
+        const double2* Z = ph + (2 * K1 + mz) * chunk;
+        double sr = 0.0, si = 0.0;
+        for (int p = 0; p < chunk; ++p) {
+            const double2 x = make_double2(X[p].x, sx * X[p].y);
+            const double2 y = make_double2(Y[p].x, sy * Y[p].y);
+            const double2 e = cmul(cmul(x, y), Z[p]);
+            sr += e.x;
+            si += e.y;
+        }
+        a.skpart[(size_t)c * nh + h] = make_double2(sr, si);
     }
 }
 
-// Forces from S(m): F_j = (2 eps q_j / V) sum_m G m Im(conj(S) e^{2 pi i m.r_j});
-// energy partials G |S|^2 written by particle-thread 0 of block 0 loop.
-__global__ __launch_bounds__(256) void ewald_force_kernel(RecipArgs a, int kmax, const double* sk)
+// S(m) and G(m) = exp(-pi^2 m^2/alpha^2)/m^2 of every half-space m into sk[3 mi ..]
+__device__ inline void ewald_sk_reduce_body(const RecipArgs& a, int kmax, double alpha, int nchunk,
+                                            double* sk, int h)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nk = 2 * kmax + 1, nm = nk * nk * nk;
+    const int nk = 2 * kmax + 1, nh = (nk * nk * nk - 1) / 2;
+    if (h >= nh) return;
+    double sr = 0.0, si = 0.0;
+    for (int c = 0; c < nchunk; ++c) {
+        const double2 v = a.skpart[(size_t)c * nh + h];
+        sr += v.x;
+        si += v.y;
+    }
+    const int mi = nh + 1 + h;
+    int mx, my, mz;
+    decode_m(mi, kmax, &mx, &my, &mz);
+    const double kx = mx / a.box[0], ky = my / a.box[1], kz = mz / a.box[2];
+    const double m2 = kx * kx + ky * ky + kz * kz;
+    sk[3 * mi + 0] = sr;
+    sk[3 * mi + 1] = si;
+    sk[3 * mi + 2] = exp(-kPi * kPi * m2 / (alpha * alpha)) / m2;
+}
+
+// Forces F_j = (2 eps q_j / V) sum_m G m/L Im(conj(S) e^{2 pi i m.r_j}) (all m),
+// potential dUm/dq_j = eps/(pi V) sum_m G Re(conj(S) e^{2 pi i m.r_j}), and in
+// block 0 the energy sum_m G |S|^2 into epart[0]; all as twice the half space.
+// Every lane reads the same m at the same time, so the half-space table is
+// staged in LDS when it fits (kStaged) and read from memory otherwise.
+constexpr int kSkLds = 2048;  // half-space entries staged in LDS (kmax <= 7)
+
+template <bool kStaged>
+__device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const double* skg, int blk)
+{
+    const int j = blk * 256 + threadIdx.x;
+    const int nk = 2 * kmax + 1, nh = (nk * nk * nk - 1) / 2;
+    __shared__ double sks[kStaged ? 3 * kSkLds : 1];
+    if (kStaged) {
+        // entry mi of the half space at sks[3 (mi - nh - 1) ..]
+        for (int t = threadIdx.x; t < 3 * nh; t += blockDim.x) sks[t] = skg[3 * (nh + 1) + t];
+        __syncthreads();
+    }
+    // half-space entry h = mi - nh - 1
+    const double* sk = kStaged ? sks : skg + 3 * (nh + 1);
     if (j < a.n) {
+        const double2 ux = cexpi(2.0 * kPi * a.x[j] / a.box[0]);
+        const double2 uy = cexpi(2.0 * kPi * a.y[j] / a.box[1]);
+        const double2 uz = cexpi(2.0 * kPi * a.z[j] / a.box[2]);
+        // e^{-i kmax theta} for x and y
+        double2 xk = make_double2(1.0, 0.0), yk = make_double2(1.0, 0.0);
+        for (int m = 0; m < kmax; ++m) {
+            xk = cmul(xk, make_double2(ux.x, -ux.y));
+            yk = cmul(yk, make_double2(uy.x, -uy.y));
+        }
         double fx = 0.0, fy = 0.0, fz = 0.0, pt = 0.0;
-        const double xj = a.x[j], yj = a.y[j], zj = a.z[j];
-        for (int mi = 0; mi < nm; ++mi) {
-            const double G = sk[3 * mi + 2];
-            if (G == 0.0) continue;
-            const int mx = mi % nk - kmax, my = (mi / nk) % nk - kmax, mz = mi / (nk * nk) - kmax;
-            const double kx = mx / a.box[0], ky = my / a.box[1], kz = mz / a.box[2];
-            double sn, cs;
-            sincos(2.0 * kPi * (kx * xj + ky * yj + kz * zj), &sn, &cs);
-            // Im(conj(S) e) = Sr sin - Si cos
-            const double im = sk[3 * mi] * sn - sk[3 * mi + 1] * cs;
-            fx += G * im * kx;
-            fy += G * im * ky;
-            fz += G * im * kz;
-            pt += G * (sk[3 * mi] * cs + sk[3 * mi + 1] * sn);  // Re(conj(S) e)
+        double2 ez = make_double2(1.0, 0.0);
+        for (int mz = 0; mz <= kmax; ++mz) {
+            const int my0 = mz == 0 ? 0 : -kmax;
+            double2 ey = mz == 0 ? make_double2(1.0, 0.0) : yk;
+            for (int my = my0; my <= kmax; ++my) {
+                const double2 eyz = cmul(ey, ez);
+                const int mx0 = (mz == 0 && my == 0) ? 1 : -kmax;
+                double2 e = cmul(eyz, mx0 == 1 ? ux : xk);
+                int h = ((mz + kmax) * nk + (my + kmax)) * nk + (mx0 + kmax) - (nh + 1);
+                double gx = 0.0, gi = 0.0;
+                for (int mx = mx0; mx <= kmax; ++mx, ++h) {
+                    const double Sr = sk[3 * h], Si = sk[3 * h + 1], G = sk[3 * h + 2];
+                    const double im = G * (Sr * e.y - Si * e.x);  // G Im(conj(S) e)
+                    gx += im * (double)mx;
+                    gi += im;
+                    pt += G * (Sr * e.x + Si * e.y);               // G Re(conj(S) e)
+                    e = cmul(e, ux);
+                }
+                fx += gx;
+                fy += gi * (double)my;
+                fz += gi * (double)mz;
+                ey = cmul(ey, uy);
+            }
+            ez = cmul(ez, uz);
         }
         const double V = a.box[0] * a.box[1] * a.box[2];
-        if (a.pot) a.pot[j] = a.eps / (kPi * V) * pt;  // dUm/dq_j
-        const double pre = 2.0 * a.eps * charge(a, j) / V;
-        a.frx[j] = pre * fx;
-        a.fry[j] = pre * fy;
-        a.frz[j] = pre * fz;
+        if (a.pot) a.pot[j] = 2.0 * a.eps / (kPi * V) * pt;
+        const double pre = 4.0 * a.eps * charge(a, j) / V;
+        a.frx[j] = pre * fx / a.box[0];
+        a.fry[j] = pre * fy / a.box[1];
+        a.frz[j] = pre * fz / a.box[2];
     }
-    if (blockIdx.x == 0) {
+    if (blk == 0) {
         __shared__ double red[256];
         double e = 0.0;
-        for (int mi = threadIdx.x; mi < nm; mi += blockDim.x)
-            e += sk[3 * mi + 2] * (sk[3 * mi] * sk[3 * mi] + sk[3 * mi + 1] * sk[3 * mi + 1]);
+        for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+            const int mi = nh + 1 + h;
+            e += skg[3 * mi + 2] * (skg[3 * mi] * skg[3 * mi] + skg[3 * mi + 1] * skg[3 * mi + 1]);
+        }
         red[threadIdx.x] = e;
         __syncthreads();
         for (int w = 128; w > 0; w >>= 1) {
             if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0) a.epart[0] = red[0];
+        if (threadIdx.x == 0) a.epart[0] = 2.0 * red[0];
     }
+}
+
+__global__ __launch_bounds__(256) void ewald_sk_part_kernel(RecipArgs a, int kmax, int chunk)
+{
+    extern __shared__ double2 ph_dyn[];
+    ewald_sk_part_body(a, kmax, chunk, blockIdx.x, ph_dyn);
+}
+
+__global__ __launch_bounds__(256) void ewald_sk_reduce_kernel(RecipArgs a, int kmax, double alpha,
+                                                              int nchunk, double* sk)
+{
+    ewald_sk_reduce_body(a, kmax, alpha, nchunk, sk, blockIdx.x * 256 + threadIdx.x);
+}
+
+template <bool kStaged>
+__global__ __launch_bounds__(256) void ewald_force_kernel(RecipArgs a, int kmax, const double* sk)
+{
+    ewald_force_body<kStaged>(a, kmax, sk, blockIdx.x);
+}
+
+// batched: blockIdx.y selects the replica (see kernels_md.hip)
+__global__ __launch_bounds__(256) void ewald_sk_part_batch_kernel(const BatchRep* reps, int chunk)
+{
+    extern __shared__ double2 ph_dyn[];
+    const BatchRep& r = reps[blockIdx.y];
+    if (r.recip != TCP_RECIP_EWALD || (int)blockIdx.x >= (r.ra.n + chunk - 1) / chunk) return;
+    ewald_sk_part_body(r.ra, r.kmax, chunk, blockIdx.x, ph_dyn);
+}
+
+__global__ __launch_bounds__(256) void ewald_sk_reduce_batch_kernel(const BatchRep* reps, int chunk)
+{
+    const BatchRep& r = reps[blockIdx.y];
+    if (r.recip != TCP_RECIP_EWALD) return;
+    ewald_sk_reduce_body(r.ra, r.kmax, r.alpha, (r.ra.n + chunk - 1) / chunk, r.sk,
+                         blockIdx.x * 256 + threadIdx.x);
+}
+
+template <bool kStaged>
+__global__ __launch_bounds__(256) void ewald_force_batch_kernel(const BatchRep* reps)
+{
+    const BatchRep& r = reps[blockIdx.y];
+    if (r.recip != TCP_RECIP_EWALD || (int)blockIdx.x >= (r.ra.n + 255) / 256) return;
+    ewald_force_body<kStaged>(r.ra, r.kmax, r.sk, blockIdx.x);
 }
 
 }  // namespace
@@ -637,12 +760,50 @@ void launch_fft3d_inverse(const RecipArgs& a, hipStream_t s)
     run_pass<kInComplex | kOutReal>(f0, nlines(a.K, 0), s);
 }
 
-void launch_ewald_kspace(const RecipArgs& a, int kmax, double alpha, double* sk, hipStream_t s)
+int ewald_chunk(int kmax)
+{
+    // LDS phase table 3 (kmax + 1) chunk double2 within 64 KiB
+    int c = 128;
+    while (c > 16 && (size_t)48 * (kmax + 1) * c > 65536) c /= 2;
+    return c;
+}
+
+int ewald_half(int kmax)
 {
     const int nk = 2 * kmax + 1;
-    hipLaunchKernelGGL(ewald_sk_kernel, dim3(nk * nk * nk), dim3(256), 0, s, a, kmax, alpha, sk);
-    hipLaunchKernelGGL(ewald_force_kernel, dim3((a.n + 255) / 256), dim3(256), 0, s, a, kmax,
-                       (const double*)sk);
+    return (nk * nk * nk - 1) / 2;
+}
+
+void launch_ewald_kspace(const RecipArgs& a, int kmax, double alpha, double* sk, hipStream_t s)
+{
+    const int chunk = ewald_chunk(kmax), nchunk = (a.n + chunk - 1) / chunk;
+    const size_t lds = (size_t)48 * (kmax + 1) * chunk;
+    hipLaunchKernelGGL(ewald_sk_part_kernel, dim3(nchunk), dim3(256), lds, s, a, kmax, chunk);
+    hipLaunchKernelGGL(ewald_sk_reduce_kernel, dim3((ewald_half(kmax) + 255) / 256), dim3(256), 0,
+                       s, a, kmax, alpha, nchunk, sk);
+    if (ewald_half(kmax) <= kSkLds)
+        hipLaunchKernelGGL(ewald_force_kernel<true>, dim3((a.n + 255) / 256), dim3(256), 0, s, a,
+                           kmax, (const double*)sk);
+    else
+        hipLaunchKernelGGL(ewald_force_kernel<false>, dim3((a.n + 255) / 256), dim3(256), 0, s, a,
+                           kmax, (const double*)sk);
+}
+
+void launch_ewald_batch(const BatchRep* reps, int nrep, int kmax, int maxn, hipStream_t s)
+{
+    const int chunk = ewald_chunk(kmax);
+    const size_t lds = (size_t)48 * (kmax + 1) * chunk;
+    hipLaunchKernelGGL(ewald_sk_part_batch_kernel, dim3((maxn + chunk - 1) / chunk, nrep), dim3(256),
+                       lds, s, reps, chunk);
+    hipLaunchKernelGGL(ewald_sk_reduce_batch_kernel, dim3((ewald_half(kmax) + 255) / 256, nrep),
+                       dim3(256), 0, s, reps, chunk);
+    if (ewald_half(kmax) <= kSkLds)
+        hipLaunchKernelGGL(ewald_force_batch_kernel<true>, dim3((maxn + 255) / 256, nrep), dim3(256),
+                           0, s, reps);
+    else
+        hipLaunchKernelGGL(ewald_force_batch_kernel<false>, dim3((maxn + 255) / 256, nrep), dim3(256),
+                           0, s, reps);
+    launch_um_batch(reps, nrep, s);
 }
 
 }  // namespace tcp

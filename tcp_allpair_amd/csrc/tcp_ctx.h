@@ -1,0 +1,81 @@
+// The context object behind the C ABI (tcp_gpu.hip) and the argument
+// builders the batch code (batch.hip) shares with it.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "tcp_internal.h"
+
+namespace tcp {
+constexpr int kEvPool = 64;  // timed pair launches kept per call
+}
+
+struct tcp_ctx {
+    tcp_params p{};
+    int precision = TCP_FP64;
+    int N = 0, Ne = 0;
+    int device = 0;
+    double alpha = 0, rc2 = 0, Uo = 0;
+    hipStream_t stream = nullptr;
+    // state
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    tcp::FixPos* upos = nullptr;
+    // pair partials
+    int nchunk = 1, jchunk = 1;
+    int n3_rows = 0, n3_blocks = 0;  // Newton's-third-law kernel: block-row slots and blocks
+    int2* blkmap = nullptr;
+    size_t pf_cap = 0;  // doubles
+    double *pf = nullptr, *pu = nullptr;
+    // reciprocal part
+    double *Q = nullptr, *C = nullptr, *gb = nullptr, *epart = nullptr, *sk = nullptr;
+    double2* skpart = nullptr;
+    double2* A = nullptr;
+    double2* tw[3] = {nullptr, nullptr, nullptr};
+    double *frx = nullptr, *fry = nullptr, *frz = nullptr;
+    int nepart = 0;
+    int nbrick[3] = {0, 0, 0};
+    int *brick_of = nullptr, *brick_count = nullptr, *brick_start = nullptr;
+    int *brick_cursor = nullptr, *sorted = nullptr;
+    double* tiles = nullptr;
+    // reductions and histograms
+    double* part = nullptr;
+    double* sc = nullptr;
+    unsigned long long* rdf = nullptr;
+    unsigned long long* vdf = nullptr;
+    double* trace = nullptr;
+    int trace_cap = 0;
+    // external block (Force_direct_ext)
+    int g_cap = 0;
+    double *gx = nullptr, *gy = nullptr, *gz = nullptr;
+    double *gfx = nullptr, *gfy = nullptr, *gfz = nullptr;
+    tcp::FixPos* gupos = nullptr;
+    double *gpf = nullptr, *gpu = nullptr;
+    size_t gpf_cap = 0;
+    // energy derivatives (allocated by the first tcp_energy_adjoint call)
+    double *adj_part = nullptr, *adj_out = nullptr, *adj_q = nullptr, *adj_pot = nullptr;
+    // timing
+    hipEvent_t ev_pair[2][tcp::kEvPool];
+    hipEvent_t ev_all[2];
+    bool events = false;
+    double pair_ms = 0, steps_ms = 0;
+    double ur_scale = 0.5;
+    uint64_t counter = 0;
+    std::vector<void*> allocs;
+    std::string err;
+};
+
+
+namespace tcp {
+
+MdArgs md_args(tcp_ctx* c);
+RecipArgs recip_args(tcp_ctx* c);
+PairArgs pair_args(tcp_ctx* c);
+// j chunks of an ordered-pair launch: >= 2048 blocks, >= 128 j per chunk, <= 64 chunks
+void choose_chunks(int n_i, int n_j, int* nchunk, int* jchunk);
+// Parameter checks of tcp_create (no device use): 0 or a TCP_ERR_* code and *msg.
+int validate_params(const tcp_params* p, int precision, std::string* msg);
+
+}  // namespace tcp

@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "tcp_internal.h"
+#include "tcp_ctx.h"
 
 using namespace tcp;
 
@@ -19,76 +19,10 @@ namespace {
 
 constexpr double kPi = 3.1415926535897931;      // data.f90:23
 constexpr double kSqrtPi = 1.7724538509055159;  // data.f90:24
-constexpr int kEvPool = 64;                     // timed pair launches kept per call
-
-// device scalar slots
-enum {
-    SC_FSUM = 0,   // 3: sum of forces
-    SC_USUM = 3,   // sum_i u_i
-    SC_UM = 4,     // Um
-    SC_MV2 = 5,    // 2: mv2el, mv2ion
-    SC_LAM = 7,    // 2: thermostat lambdas
-    SC_N = 16
-};
 
 thread_local std::string g_create_error;
 
 }  // namespace
-
-struct tcp_ctx {
-    tcp_params p{};
-    int precision = TCP_FP64;
-    int N = 0, Ne = 0;
-    int device = 0;
-    double alpha = 0, rc2 = 0, Uo = 0;
-    hipStream_t stream = nullptr;
-    // state
-    double *x = nullptr, *y = nullptr, *z = nullptr;
-    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
-    double *fx = nullptr, *fy = nullptr, *fz = nullptr;
-    FixPos* upos = nullptr;
-    // pair partials
-    int nchunk = 1, jchunk = 1;
-    int n3_rows = 0, n3_blocks = 0;  // Newton's-third-law kernel: block-row slots and blocks
-    int2* blkmap = nullptr;
-    size_t pf_cap = 0;  // doubles
-    double *pf = nullptr, *pu = nullptr;
-    // reciprocal part
-    double *Q = nullptr, *C = nullptr, *gb = nullptr, *epart = nullptr, *sk = nullptr;
-    double2* A = nullptr;
-    double2* tw[3] = {nullptr, nullptr, nullptr};
-    double *frx = nullptr, *fry = nullptr, *frz = nullptr;
-    int nepart = 0;
-    int nbrick[3] = {0, 0, 0};
-    int *brick_of = nullptr, *brick_count = nullptr, *brick_start = nullptr;
-    int *brick_cursor = nullptr, *sorted = nullptr;
-    double* tiles = nullptr;
-    // reductions and histograms
-    double* part = nullptr;
-    double* sc = nullptr;
-    unsigned long long* rdf = nullptr;
-    unsigned long long* vdf = nullptr;
-    double* trace = nullptr;
-    int trace_cap = 0;
-    // external block (Force_direct_ext)
-    int g_cap = 0;
-    double *gx = nullptr, *gy = nullptr, *gz = nullptr;
-    double *gfx = nullptr, *gfy = nullptr, *gfz = nullptr;
-    FixPos* gupos = nullptr;
-    double *gpf = nullptr, *gpu = nullptr;
-    size_t gpf_cap = 0;
-    // energy derivatives (allocated by the first tcp_energy_adjoint call)
-    double *adj_part = nullptr, *adj_out = nullptr, *adj_q = nullptr, *adj_pot = nullptr;
-    // timing
-    hipEvent_t ev_pair[2][kEvPool];
-    hipEvent_t ev_all[2];
-    bool events = false;
-    double pair_ms = 0, steps_ms = 0;
-    double ur_scale = 0.5;
-    uint64_t counter = 0;
-    std::vector<void*> allocs;
-    std::string err;
-};
 
 namespace {
 
@@ -125,6 +59,10 @@ int dalloc(tcp_ctx* c, T** p, size_t count)
     } while (0)
 
 bool pow2(int k) { return k >= 2 && (k & (k - 1)) == 0; }
+
+}  // namespace
+
+namespace tcp {
 
 MdArgs md_args(tcp_ctx* c)
 {
@@ -163,6 +101,7 @@ RecipArgs recip_args(tcp_ctx* c)
     a.epart = c->epart;
     a.nepart = c->nepart;
     a.frx = c->frx; a.fry = c->fry; a.frz = c->frz;
+    a.skpart = c->skpart;
     for (int d = 0; d < 3; ++d) a.nbrick[d] = c->nbrick[d];
     a.brick_of = c->brick_of;
     a.brick_count = c->brick_count;
@@ -209,6 +148,10 @@ PairArgs pair_args(tcp_ctx* c)
     a.flags = c->p.flags;
     return a;
 }
+
+}  // namespace tcp
+
+namespace {
 
 void launch_pair(tcp_ctx* c, const PairArgs& a, int nchunk)
 {
@@ -365,6 +308,8 @@ int setup_recip(tcp_ctx* c)
     if (c->p.recip == TCP_RECIP_EWALD) {
         const int nk = 2 * c->p.kmax + 1;
         TRY(dalloc(c, &c->sk, (size_t)3 * nk * nk * nk));
+        const int nchunk = (c->N + ewald_chunk(c->p.kmax) - 1) / ewald_chunk(c->p.kmax);
+        TRY(dalloc(c, &c->skpart, (size_t)nchunk * ewald_half(c->p.kmax)));
         TRY(dalloc(c, &c->epart, (size_t)1));
         c->nepart = 1;
         return 0;
@@ -442,6 +387,43 @@ int setup_recip(tcp_ctx* c)
 
 }  // namespace
 
+namespace tcp {
+
+int validate_params(const tcp_params* p, int precision, std::string* msg)
+{
+    auto vfail = [](std::string* m, int code, const char* text) {
+        *m = text;
+        return code;
+    };
+    if (p->ne < 0 || p->ni < 0 || p->ne + p->ni < 2)
+        return vfail(msg, TCP_ERR_ARG, "need at least two particles");
+    if (p->flags & ~(TCP_KELBG | TCP_REG_EE))
+        return vfail(msg, TCP_ERR_ARG, "unknown regularisation flags");
+    if (precision != TCP_FP64 && precision != TCP_MIXED)
+        return vfail(msg, TCP_ERR_ARG, "precision must be TCP_FP64 or TCP_MIXED");
+    for (int d = 0; d < 3; ++d)
+        if (!(p->box[d] > 0)) return vfail(msg, TCP_ERR_ARG, "box lengths must be positive");
+    if (!(p->rc > 0) || !(p->rs > 0) || !(p->dx_ee > 0) || !(p->dx_ei > 0) || !(p->dx_ii > 0))
+        return vfail(msg, TCP_ERR_ARG, "rc, rs and the RDF bin widths must be positive");
+    const double rc_cut = std::fmin(std::fmin(p->box[0], p->box[1]), p->box[2]) / 2.0;
+    if (p->rc > rc_cut) return vfail(msg, TCP_ERR_CUTOFF, " *** error at cut-off radius *** ");
+    if (p->recip == TCP_RECIP_SPME) {
+        for (int d = 0; d < 3; ++d)
+            if (!pow2(p->K[d]) || p->K[d] > 1024 || p->K[d] < p->order)
+                return vfail(msg, TCP_ERR_UNSUPPORTED,
+                            "SPME grid sizes must be powers of two in [order, 1024]");
+        if (p->order < 4 || p->order > 12 || (p->order & 1))
+            return vfail(msg, TCP_ERR_UNSUPPORTED, "B-spline order must be 4, 6, 8, 10 or 12");
+    } else if (p->recip == TCP_RECIP_EWALD) {
+        if (p->kmax < 1 || p->kmax > 40) return vfail(msg, TCP_ERR_ARG, "kmax must be in [1, 40]");
+    } else if (p->recip != TCP_RECIP_NONE) {
+        return vfail(msg, TCP_ERR_ARG, "unknown reciprocal method");
+    }
+    return 0;
+}
+
+}  // namespace tcp
+
 extern "C" {
 
 const char* tcp_last_error(const tcp_ctx* ctx)
@@ -453,29 +435,10 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
 {
     if (!out || !p) return fail(nullptr, TCP_ERR_ARG, "null argument");
     *out = nullptr;
-    if (p->ne < 0 || p->ni < 0 || p->ne + p->ni < 2)
-        return fail(nullptr, TCP_ERR_ARG, "need at least two particles");
-    if (p->flags & ~(TCP_KELBG | TCP_REG_EE))
-        return fail(nullptr, TCP_ERR_ARG, "unknown regularisation flags");
-    if (precision != TCP_FP64 && precision != TCP_MIXED)
-        return fail(nullptr, TCP_ERR_ARG, "precision must be TCP_FP64 or TCP_MIXED");
-    for (int d = 0; d < 3; ++d)
-        if (!(p->box[d] > 0)) return fail(nullptr, TCP_ERR_ARG, "box lengths must be positive");
-    if (!(p->rc > 0) || !(p->rs > 0) || !(p->dx_ee > 0) || !(p->dx_ei > 0) || !(p->dx_ii > 0))
-        return fail(nullptr, TCP_ERR_ARG, "rc, rs and the RDF bin widths must be positive");
-    const double rc_cut = std::fmin(std::fmin(p->box[0], p->box[1]), p->box[2]) / 2.0;
-    if (p->rc > rc_cut) return fail(nullptr, TCP_ERR_CUTOFF, " *** error at cut-off radius *** ");
-    if (p->recip == TCP_RECIP_SPME) {
-        for (int d = 0; d < 3; ++d)
-            if (!pow2(p->K[d]) || p->K[d] > 1024 || p->K[d] < p->order)
-                return fail(nullptr, TCP_ERR_UNSUPPORTED,
-                            "SPME grid sizes must be powers of two in [order, 1024]");
-        if (p->order < 4 || p->order > 12 || (p->order & 1))
-            return fail(nullptr, TCP_ERR_UNSUPPORTED, "B-spline order must be 4, 6, 8, 10 or 12");
-    } else if (p->recip == TCP_RECIP_EWALD) {
-        if (p->kmax < 1 || p->kmax > 40) return fail(nullptr, TCP_ERR_ARG, "kmax must be in [1, 40]");
-    } else if (p->recip != TCP_RECIP_NONE) {
-        return fail(nullptr, TCP_ERR_ARG, "unknown reciprocal method");
+    {
+        std::string msg;
+        const int rc = validate_params(p, precision, &msg);
+        if (rc) return fail(nullptr, rc, msg);
     }
     tcp_ctx* c = new tcp_ctx();
     c->p = *p;
@@ -546,15 +509,15 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
 int tcp_destroy(tcp_ctx* c)
 {
     if (!c) return 0;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (void* a : c->allocs) hipFree(a);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* a : c->allocs) (void)hipFree(a);
     if (c->events) {
         for (int e = 0; e < kEvPool; ++e)
-            for (int k = 0; k < 2; ++k) hipEventDestroy(c->ev_pair[k][e]);
-        hipEventDestroy(c->ev_all[0]);
-        hipEventDestroy(c->ev_all[1]);
+            for (int k = 0; k < 2; ++k) (void)hipEventDestroy(c->ev_pair[k][e]);
+        (void)hipEventDestroy(c->ev_all[0]);
+        (void)hipEventDestroy(c->ev_all[1]);
     }
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
 }

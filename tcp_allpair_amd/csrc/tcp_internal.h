@@ -84,6 +84,7 @@ struct RecipArgs {
     double* frx;               // reciprocal forces (SoA)
     double* fry;
     double* frz;
+    double2* skpart;           // direct k-sum: per-chunk structure-factor partials
     // brick-binned spreading (used when every K_d is a multiple of kBrick)
     int nbrick[3];             // bricks per axis (0: unbinned atomic spreading)
     int* brick_of;             // [n] brick of each particle's first grid point
@@ -103,9 +104,13 @@ void launch_spme_gather(const RecipArgs& a, hipStream_t s);
 void launch_fill(double* p, size_t n, double v, hipStream_t s);
 int fft_epart_blocks(const int* K);
 
-// Direct Ewald k-space sum (small systems, reference-independent extension)
+// Direct Ewald k-space sum (small systems, reference-independent extension):
+// sk_work holds 3 (2 kmax + 1)^3 doubles, a.skpart ewald_half(kmax) x
+// ceil(n / ewald_chunk(kmax)) structure-factor partials; energy sum into epart[0].
+int ewald_chunk(int kmax);
+int ewald_half(int kmax);
 void launch_ewald_kspace(const RecipArgs& a, int kmax, double alpha, double* sk_work,
-                         hipStream_t s);  // energy sum into epart[0]
+                         hipStream_t s);
 
 // Energy derivatives (FP64, ordered pairs, per-particle charges): per chunk of j
 // and particle i, part[(chunk * kAdjQ + k) * n + i] holds k = 0..2 the force
@@ -138,6 +143,7 @@ struct MdArgs {
 };
 
 int md_blocks(int n);
+__host__ __device__ inline int md_blocks_dev(int n) { return (n + kRedBlock - 1) / kRedBlock; }
 void launch_vv1(const MdArgs& m, int thermo, double dt, double fric, hipStream_t s);
 void launch_make_upos(const MdArgs& m, hipStream_t s);
 // f (=|+=) sum over chunks of the pair partials (+ fr when frx != null);
@@ -160,5 +166,72 @@ void launch_vv2_scale(const MdArgs& m, const double* lam, double* part, hipStrea
 void launch_trace(double* row, const double* usum, double ur_scale, double Uo, const double* um,
                   const double* mv2, double eps, hipStream_t s);
 void launch_um(const double* epart, int n, double scale, double* um, hipStream_t s);
+
+// Device scalars of a context (double slots)
+enum {
+    SC_FSUM = 0,   // 3: sum of forces
+    SC_USUM = 3,   // sum_i u_i
+    SC_UM = 4,     // Um
+    SC_MV2 = 5,    // 2: mv2el, mv2ion
+    SC_LAM = 7,    // 2: thermostat lambdas
+    SC_N = 16
+};
+
+// ------------------------------------------------------------- batches ---
+// One replica of a batch of independent plasmas: the arguments every batched
+// kernel needs, kept in a device table indexed by replica.
+struct BatchRep {
+    MdArgs m;
+    PairArgs pa;               // pair kernel arguments; pf/pu hold `slots` partial slots
+    RecipArgs ra;              // direct k-space sum arguments (recip == TCP_RECIP_EWALD)
+    double* sk;                // k-sum work, 3 (2 kmax + 1)^3
+    double* part;              // block partials
+    double* sc;                // device scalars SC_*
+    int slots;                 // pair partial slots summed by combine
+    int recip;                 // TCP_RECIP_NONE or TCP_RECIP_EWALD
+    int kmax;
+    double alpha;
+    double Te, Ti;             // thermostat targets
+    double ur_scale, Uo, eps, um_scale;
+};
+
+// Work item of a persistent pair launch: replica, then (row, column) block pair
+// (Newton's-third-law kernel) or (i block, j chunk) (FP64 kernel).
+struct PairItem {
+    int rep, a, b, cost;
+};
+
+void launch_vv1_batch(const BatchRep* reps, int nrep, int maxblk, int thermo, double dt,
+                      double fric, hipStream_t s);
+void launch_combine_batch(const BatchRep* reps, int nrep, int maxblk, hipStream_t s);
+void launch_reduce_parts_batch(const BatchRep* reps, int nrep, int width, int slot, hipStream_t s);
+// second half kick (+ thermostat) and the kinetic sums into sc[SC_MV2]
+void launch_vv2_batch(const BatchRep* reps, int nrep, int maxblk, int thermo, double dt, double tau,
+                      double fric, uint64_t seed, const uint64_t* counter, hipStream_t s);
+void launch_um_batch(const BatchRep* reps, int nrep, hipStream_t s);
+void launch_trace_batch(const BatchRep* reps, int nrep, double* trace, const uint64_t* counter,
+                        hipStream_t s);
+void launch_step_advance(uint64_t* counter, hipStream_t s);
+// direct k-space sums of every replica (same kmax): S(m), forces, energy partial;
+// maxn: particles of the largest replica
+void launch_ewald_batch(const BatchRep* reps, int nrep, int kmax, int maxn, hipStream_t s);
+
+// Pair-kernel shapes of the persistent Newton's-third-law kernel: tiles per
+// block side and wavefronts per block.  Shape 0 is the single-context kernel's.
+constexpr int kNumN3Shapes = 4;
+struct N3Shape {
+    int tiles, waves;
+};
+N3Shape n3_shape(int shape);
+int n3_block_rows_shape(int n, int shape);
+// Persistent launches over a work list; queue[2] must be zero before the first
+// launch and is left zero by every launch.  grid: resident blocks to start.
+void launch_pair_mixed_n3_batch(const BatchRep* reps, const PairItem* items, int nitems,
+                                int* queue, int shape, bool rdf, bool kelbg, int grid,
+                                hipStream_t s);
+void launch_pair_f64_batch(const BatchRep* reps, const PairItem* items, int nitems, int* queue,
+                           bool rdf, int grid, hipStream_t s);
+// resident blocks per CU of the persistent kernels
+int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg);
 
 }  // namespace tcp

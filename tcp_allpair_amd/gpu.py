@@ -39,6 +39,17 @@ SIGNATURES = {
     "tcp_last_timings": [_vp, _dp, _dp],
     "tcp_set_lambda": [_vp, ctypes.c_double],
     "tcp_energy_adjoint": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
+    "tcp_batch_create": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(CParams), ctypes.c_int],
+    "tcp_batch_destroy": [_vp],
+    "tcp_batch_last_error": [_vp],
+    "tcp_batch_set_state": [_vp, ctypes.c_int, _dp, _dp, _dp],
+    "tcp_batch_get_state": [_vp, ctypes.c_int, _dp, _dp, _dp],
+    "tcp_batch_get_histograms": [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int],
+    "tcp_batch_md_steps": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp,
+                           ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _dp],
+    "tcp_batch_set_shape": [_vp, ctypes.c_int],
+    "tcp_batch_autotune": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp],
+    "tcp_batch_last_timings": [_vp, _dp, _dp],
 }
 
 _LIB = None
@@ -58,7 +69,7 @@ def load():
         for name, args in SIGNATURES.items():
             fn = getattr(lib, name)
             fn.argtypes = args
-            fn.restype = ctypes.c_char_p if name == "tcp_last_error" else ctypes.c_int
+            fn.restype = ctypes.c_char_p if name.endswith("_last_error") else ctypes.c_int
         _LIB = lib
     return _LIB
 
@@ -213,4 +224,98 @@ class Context:
         a, b = ctypes.c_double(), ctypes.c_double()
         self._ck(self.lib.tcp_last_timings(self.h, ctypes.byref(a), ctypes.byref(b)),
                  "tcp_last_timings")
+        return a.value, b.value
+
+
+class Batch:
+    """Independent replicas stepped together (tcp_batch_*): one persistent pair
+    launch per step for all replicas, the step captured in a hipGraph."""
+
+    NSHAPES = 4
+
+    def __init__(self, params, precision: int = MIXED):
+        self.lib = load()
+        self.params = list(params)
+        self.precision = precision
+        arr = (CParams * len(self.params))(*[CParams.from_params(p) for p in self.params])
+        h = _vp()
+        rc = self.lib.tcp_batch_create(ctypes.byref(h), len(self.params), arr, int(precision))
+        if rc != 0:
+            raise TcpError(f"tcp_batch_create: {self.lib.tcp_batch_last_error(None).decode()} ({rc})")
+        self.h = h
+        self.seed = 12345
+
+    @property
+    def nrep(self):
+        return len(self.params)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tcp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise TcpError(f"{what}: {self.lib.tcp_batch_last_error(self.h).decode()} ({rc})")
+
+    def set_state(self, rep, x=None, v=None, f=None):
+        n = self.params[rep].n if 0 <= rep < self.nrep else 0
+        arrs = [None if a is None else _aos(a, n) for a in (x, v, f)]
+        self._ck(self.lib.tcp_batch_set_state(self.h, int(rep), *(_ptr(a) for a in arrs)),
+                 "tcp_batch_set_state")
+
+    def get_state(self, rep):
+        n = self.params[rep].n if 0 <= rep < self.nrep else 1
+        x, v, f = (np.zeros((n, 3)) for _ in range(3))
+        self._ck(self.lib.tcp_batch_get_state(self.h, int(rep), _ptr(x), _ptr(v), _ptr(f)),
+                 "tcp_batch_get_state")
+        return x, v, f
+
+    def histograms(self, rep, reset=False):
+        rdf = np.zeros(3 * NBIN)
+        vdf = np.zeros(2 * NBIN)
+        self._ck(self.lib.tcp_batch_get_histograms(self.h, int(rep), _ptr(rdf), _ptr(vdf),
+                                                   int(bool(reset))), "tcp_batch_get_histograms")
+        return rdf.reshape(3, NBIN), vdf.reshape(2, NBIN)
+
+    def md_steps(self, nsteps, thermo, dt, Te=None, Ti=None, tau=1.0, fric=0.0, seed=None,
+                 trace=True):
+        te = None if Te is None else np.ascontiguousarray(Te, dtype=np.float64)
+        ti = None if Ti is None else np.ascontiguousarray(Ti, dtype=np.float64)
+        for a in (te, ti):
+            if a is not None and a.shape != (self.nrep,):
+                raise ValueError(f"expected {self.nrep} thermostat targets")
+        tr = np.zeros((nsteps, self.nrep, 6)) if trace else None
+        self._ck(self.lib.tcp_batch_md_steps(self.h, int(nsteps), int(thermo), float(dt), _ptr(te),
+                                             _ptr(ti), float(tau), float(fric),
+                                             int(self.seed if seed is None else seed), _ptr(tr)),
+                 "tcp_batch_md_steps")
+        return tr
+
+    def set_shape(self, shape):
+        self._ck(self.lib.tcp_batch_set_shape(self.h, int(shape)), "tcp_batch_set_shape")
+
+    def autotune(self, reps=3):
+        best = ctypes.c_int()
+        ms = np.full(self.NSHAPES, np.nan)
+        self._ck(self.lib.tcp_batch_autotune(self.h, int(reps), ctypes.byref(best), _ptr(ms)),
+                 "tcp_batch_autotune")
+        return best.value, ms
+
+    def timings(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._ck(self.lib.tcp_batch_last_timings(self.h, ctypes.byref(a), ctypes.byref(b)),
+                 "tcp_batch_last_timings")
         return a.value, b.value

@@ -1,6 +1,7 @@
 """Summaries of rocprofv3 CSV output for profiles/ (text, a few KB).
 
     python -m tcp_allpair_amd.profsum stats <kernel_stats.csv> <out.md> [note]
+    python -m tcp_allpair_amd.profsum batchstats <kernel_stats.csv> <out.md> [note]
     python -m tcp_allpair_amd.profsum pmc <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv> <out.md>
 
 `stats` tabulates rocprofv3 --stats per kernel.  `pmc` scales FETCH_SIZE and
@@ -17,7 +18,9 @@ import sys
 
 PAIR_KERNEL = "pair_mixed_n3_kernel<true>"
 STATS_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_stats -o bench "
-             "-- python3 bench.py --steps 5 --warmup 1 --fp64-steps 2 --no-cpu-baseline")
+             "-- python3 bench.py --steps 5 --warmup 1 --fp64-steps 2 --no-cpu-baseline --batch-steps 0")
+BATCH_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_batch -o batch "
+             "-- python3 -m tcp_allpair_amd.batchbench --steps 5 --warmup 1")
 PMC_CMDS = [
     "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o bench -- python3 bench.py "
     "--steps 2 --warmup 0 --fp64-steps 1 --no-cpu-baseline",
@@ -35,14 +38,23 @@ def short(name: str) -> str:
     return name.split("(")[0]
 
 
-def stats(src: str, out: str, note: str = "") -> None:
+def stats(src: str, out: str, note: str = "", kind: str = "step") -> None:
     rows = list(csv.DictReader(open(src)))
-    lines = ["# Kernel time summary: N = 131072 step, mixed and FP64 paths\n\n",
-             "Workload: 65536 e + 65536 i, L = 50.797 A, SPME 128^3 order 6, RDF/VDF on; bench.py with "
-             "1 warm-up and 5 timed mixed steps, 2 FP64 steps, and one initial force pass per precision.\n\n",
-             "Command (repository root on the GPU box, TMPDIR=/tmp):\n\n    " + STATS_CMD + "\n\n",
-             "| kernel | calls | average ms | total ms | % |\n|---|---|---|---|---|\n"]
-    for r in rows[:18]:
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    if kind == "batch":
+        head = ["# Kernel time summary: 64-replica batch, mixed precision, direct k-sum kmax 6\n\n",
+                "Workload: tcp_allpair_amd/batchbench.py (64 replicas, 526..16238 particles, 279616 in "
+                "all); the autotuner times each of the 4 block shapes, then 1 warm-up and 5 timed "
+                "batch steps; the same replicas then run one context after another (the "
+                "single-context kernels below).\n\n", BATCH_CMD]
+    else:
+        head = ["# Kernel time summary: N = 131072 step, mixed and FP64 paths\n\n",
+                "Workload: 65536 e + 65536 i, L = 50.797 A, SPME 128^3 order 6, RDF/VDF on; bench.py "
+                "with 1 warm-up and 5 timed mixed steps, 2 FP64 steps, and one initial force pass per "
+                "precision.\n\n", STATS_CMD]
+    lines = head[:2] + ["Command (repository root on the GPU box, TMPDIR=/tmp):\n\n    " + head[2] + "\n\n",
+                        "| kernel | calls | average ms | total ms | % |\n|---|---|---|---|---|\n"]
+    for r in rows[:22]:
         lines.append("| %s | %s | %.4f | %.2f | %.2f |\n" % (
             short(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e6,
             float(r["TotalDurationNs"]) / 1e6, float(r["Percentage"])))
@@ -82,7 +94,8 @@ def pmc(fetch: str, write: str, cfetch: str, cwrite: str, out: str) -> None:
 
 
 if __name__ == "__main__":
-    if sys.argv[1] == "stats":
-        stats(sys.argv[2], sys.argv[3], sys.argv[4] if len(sys.argv) > 4 else "")
+    if sys.argv[1] in ("stats", "batchstats"):
+        stats(sys.argv[2], sys.argv[3], sys.argv[4] if len(sys.argv) > 4 else "",
+              "batch" if sys.argv[1] == "batchstats" else "step")
     else:
         pmc(*sys.argv[2:7])

@@ -86,3 +86,26 @@ def test_bad_precision():
     p, _, _ = make_plasma(10, 10)
     rc, msg, _ = _create(p, precision=7)
     assert rc == -1
+
+
+def _batch_create(ps, precision=FP64, nrep=None):
+    lib = gpu.load()
+    h = ctypes.c_void_p()
+    arr = (CParams * len(ps))(*[CParams.from_params(p) for p in ps])
+    rc = lib.tcp_batch_create(ctypes.byref(h), len(ps) if nrep is None else nrep, arr, precision)
+    return rc, lib.tcp_batch_last_error(None).decode(), h
+
+
+def test_batch_argument_errors_before_device_use():
+    p, _, _ = make_plasma(10, 10, recip=2, kmax=3)
+    rc, msg, h = _batch_create([p], nrep=0)
+    assert rc == -1 and msg and not h.value
+    rc, msg, h = _batch_create([p, p.replace(kmax=4)])
+    assert rc == -1 and "share" in msg and not h.value
+    rc, msg, h = _batch_create([p, p.replace(flags=1)])
+    assert rc == -1 and "share" in msg and not h.value
+    rc, msg, h = _batch_create([p.replace(recip=1), p.replace(recip=1)])
+    assert rc == -4 and "batches" in msg and not h.value
+    # a replica's own parameter error is reported with its index
+    rc, msg, h = _batch_create([p, p.replace(rc=0.51 * p.box[0])])
+    assert rc == -2 and "replica 1" in msg and not h.value

@@ -1,0 +1,147 @@
+"""Batched replicas (tcp_batch_*): one persistent pair launch and a captured step for all.
+
+Tolerances:
+  FP64 batch vs each replica's own context:   bit for bit (state, trace, histograms)
+  MIXED batch, block shape 0, vs own context: bit for bit
+  MIXED batch, other shapes, vs FP64:         RMS relative force error <= 1e-6
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tcp_allpair_amd.gpu import Batch, Context, TcpError
+from tcp_allpair_amd.inputs import make_plasma
+from tcp_allpair_amd.params import (BEREND, EPS, FP64, ISOKIN, KELBG, MIXED, NOTEMP, RECIP_EWALD,
+                                    RECIP_NONE, REG_EE, STOCH, reduced_dt)
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(150, 150), (260, 250), (389, 388), (512, 512), (1000, 1000)]
+DENS = [0.5, 1.0, 1.5, 2.0, 1.0]
+TEMP = [0.5, 1.0, 2.0, 4.0, 1.0]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def replicas(recip=RECIP_EWALD, flags=0, kmax=4):
+    out = []
+    for r, ((ne, ni), d, t) in enumerate(zip(SIZES, DENS, TEMP)):
+        p, x, v = make_plasma(ne, ni, density=d, T=t * EPS, seed=100 + r, recip=recip, kmax=kmax)
+        out.append((p.replace(flags=flags), x, v))
+    return out
+
+
+def standalone(p, x, v, precision, nsteps, thermo, dt, **kw):
+    with Context(p, precision) as c:
+        c.set_state(x=x, v=v, f=np.zeros_like(x))
+        tr = c.md_steps(nsteps, thermo, dt, **kw)
+        st = c.get_state()
+        h = c.histograms()
+    return tr, st, h
+
+
+def run_batch(reps, precision, nsteps, thermo, dt, shape=None, **kw):
+    with Batch([p for p, _, _ in reps], precision) as b:
+        if shape is not None:
+            b.set_shape(shape)
+        for r, (p, x, v) in enumerate(reps):
+            b.set_state(r, x=x, v=v, f=np.zeros_like(x))
+        tr = b.md_steps(nsteps, thermo, dt, **kw)
+        st = [b.get_state(r) for r in range(len(reps))]
+        h = [b.histograms(r) for r in range(len(reps))]
+    return tr, st, h
+
+
+def assert_bitwise(reps, precision, nsteps, thermo, dt, shape=None, batch_kw=None, **kw):
+    btr, bst, bh = run_batch(reps, precision, nsteps, thermo, dt, shape, **(batch_kw or kw))
+    for r, (p, x, v) in enumerate(reps):
+        skw = dict(kw)
+        if batch_kw:
+            skw.update({k: batch_kw[k] for k in ("tau", "fric", "seed") if k in batch_kw})
+            skw.update(Te=batch_kw["Te"][r], Ti=batch_kw["Ti"][r])
+        tr, st, h = standalone(p, x, v, precision, nsteps, thermo, dt, **skw)
+        np.testing.assert_array_equal(btr[:, r, :], tr, err_msg=f"trace of replica {r}")
+        for a, b_, name in zip(st, bst[r], "xvf"):
+            np.testing.assert_array_equal(b_, a, err_msg=f"{name} of replica {r}")
+        np.testing.assert_array_equal(bh[r][0], h[0])
+        np.testing.assert_array_equal(bh[r][1], h[1])
+
+
+def test_fp64_batch_nve_ewald_bitwise():
+    say("case batch fp64 nve ewald")
+    assert_bitwise(replicas(), FP64, 6, NOTEMP, reduced_dt(0.005))
+
+
+def test_fp64_batch_thermostats_bitwise():
+    reps = replicas()
+    Te = np.array([t * EPS for t in TEMP]) * 1.1
+    Ti = Te * 0.9
+    for thermo in (BEREND, ISOKIN):
+        say("case batch fp64 thermostat", thermo)
+        assert_bitwise(reps, FP64, 4, thermo, reduced_dt(0.005), batch_kw=dict(Te=Te, Ti=Ti, tau=0.5))
+    say("case batch fp64 stoch")
+    assert_bitwise(reps, FP64, 4, STOCH, reduced_dt(0.005),
+                   batch_kw=dict(Te=Te, Ti=Ti, fric=0.3, seed=77))
+
+
+def test_fp64_batch_no_recip_regularised_bitwise():
+    say("case batch fp64 recip none kelbg")
+    assert_bitwise(replicas(RECIP_NONE, KELBG | REG_EE), FP64, 5, NOTEMP, reduced_dt(0.005))
+
+
+def test_mixed_batch_shape0_bitwise():
+    say("case batch mixed shape 0")
+    assert_bitwise(replicas(), MIXED, 5, NOTEMP, reduced_dt(0.005), shape=0)
+
+
+@pytest.mark.parametrize("shape", [1, 2, 3])
+def test_mixed_batch_shapes_vs_fp64(shape):
+    say("case batch mixed shape", shape)
+    reps = replicas(flags=KELBG)
+    _, bst, bh = run_batch(reps, MIXED, 1, NOTEMP, reduced_dt(0.005), shape=shape)
+    _, fst, fh = run_batch(reps, FP64, 1, NOTEMP, reduced_dt(0.005))
+    for r in range(len(reps)):
+        f, f64 = bst[r][2], fst[r][2]
+        assert np.sqrt(((f - f64) ** 2).sum() / (f64 ** 2).sum()) <= 1e-6
+        assert abs(bh[r][0].sum() - fh[r][0].sum()) <= 1e-4 * fh[r][0].sum()
+
+
+def test_autotune_and_graph_reuse():
+    say("case batch autotune")
+    reps = replicas()
+    dt = reduced_dt(0.005)
+    with Batch([p for p, _, _ in reps], MIXED) as b:
+        for r, (p, x, v) in enumerate(reps):
+            b.set_state(r, x=x, v=v, f=np.zeros_like(x))
+        best, ms = b.autotune(2)
+        say("autotune", best, ms)
+        assert 0 <= best < Batch.NSHAPES
+        assert np.all(np.isfinite(ms)) and np.all(ms > 0) and ms[best] == ms.min()
+        pair_ms, _ = b.timings()
+        assert pair_ms == ms[best]
+        # the autotuner's launches leave the RDF counts alone
+        assert all(not b.histograms(r)[0].any() for r in range(len(reps)))
+        b.set_shape(0)
+        t1 = b.md_steps(3, NOTEMP, dt)
+        t2 = b.md_steps(3, NOTEMP, dt)  # replays the captured step
+        s1 = [b.get_state(r) for r in range(len(reps))]
+    tr, st, _ = run_batch(reps, MIXED, 6, NOTEMP, dt, shape=0)
+    np.testing.assert_array_equal(np.concatenate([t1, t2]), tr)
+    for r in range(len(reps)):
+        np.testing.assert_array_equal(s1[r][0], st[r][0])
+
+
+def test_batch_errors():
+    say("case batch errors")
+    reps = replicas()
+    with Batch([p for p, _, _ in reps], FP64) as b:
+        with pytest.raises(TcpError):
+            b.get_state(len(reps))
+        with pytest.raises(TcpError):
+            b.md_steps(1, BEREND, 0.001)
+        with pytest.raises(TcpError):
+            b.set_shape(1)  # FP64 batches have one shape
