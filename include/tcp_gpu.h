@@ -117,6 +117,16 @@ int tcp_vverlet1(tcp_ctx* ctx, int thermo, double dt, double fric);
 int tcp_vverlet2(tcp_ctx* ctx, int thermo, double dt, double Te, double Ti, double tau,
                  double fric, uint64_t seed, double* mv2el, double* mv2ion);
 
+/* VVerletBerend2 / VVerletIsokin2 (vverlet.f90:227, 301) split at their
+ * MPI_ALLREDUCE over COMM1D (vverlet.f90:254, 278), for several direct ranks:
+ * tcp_vverlet2_kick does the half kick and returns the local pre-rescale sums
+ * sum m v^2 per species (vverlet.f90:246-253); the caller reduces them over
+ * the ranks and forms lambda, and tcp_vverlet2_scale multiplies the velocities
+ * by lambda per species, counts the VDF and returns the local post-rescale
+ * sums sys%mv2el, sys%mv2ion (vverlet.f90:258-269). */
+int tcp_vverlet2_kick(tcp_ctx* ctx, double dt, double* mv2el, double* mv2ion);
+int tcp_vverlet2_scale(tcp_ctx* ctx, double lam_e, double lam_i, double* mv2el, double* mv2ion);
+
 /* Adds the device RDF/VDF counts into rdf[3*501] and vdf[2*501] (either may be
  * NULL) and zeroes them on the device when reset != 0: the NS%RDF_* / NS%VDF_*
  * accumulation read by DUMP (result.f90:226-268). */

@@ -710,6 +710,39 @@ int tcp_vverlet2(tcp_ctx* c, int thermo, double dt, double Te, double Ti, double
     return 0;
 }
 
+int tcp_vverlet2_kick(tcp_ctx* c, double dt, double* mv2el, double* mv2ion)
+{
+    if (!c) return TCP_ERR_ARG;
+    // the BEREND/ISOKIN kick: half kick and the pre-rescale sums, no VDF
+    launch_vv2_kick(md_args(c), TCP_BEREND, dt, 0.0, 0.0, 0.0, 0, 0, nullptr, c->part, c->stream);
+    LAUNCHCK(c);
+    launch_reduce_parts(c->part, md_blocks(c->N), 2, c->sc + SC_MV2, c->stream);
+    LAUNCHCK(c);
+    double mv2[2];
+    TRY(get_sc(c, SC_MV2, 2, mv2));
+    if (mv2el) *mv2el = mv2[0];
+    if (mv2ion) *mv2ion = mv2[1];
+    return 0;
+}
+
+int tcp_vverlet2_scale(tcp_ctx* c, double lam_e, double lam_i, double* mv2el, double* mv2ion)
+{
+    if (!c) return TCP_ERR_ARG;
+    if (!std::isfinite(lam_e) || !std::isfinite(lam_i))
+        return fail(c, TCP_ERR_ARG, "thermostat factors must be finite");
+    const double lam[2] = {lam_e, lam_i};
+    HIPCK(c, hipMemcpyAsync(c->sc + SC_LAM, lam, sizeof(lam), hipMemcpyHostToDevice, c->stream));
+    launch_vv2_scale(md_args(c), c->sc + SC_LAM, c->part, c->stream);
+    LAUNCHCK(c);
+    launch_reduce_parts(c->part, md_blocks(c->N), 2, c->sc + SC_MV2, c->stream);
+    LAUNCHCK(c);
+    double mv2[2];
+    TRY(get_sc(c, SC_MV2, 2, mv2));
+    if (mv2el) *mv2el = mv2[0];
+    if (mv2ion) *mv2ion = mv2[1];
+    return 0;
+}
+
 int tcp_get_histograms(tcp_ctx* c, double* rdf, double* vdf, int reset)
 {
     if (!c) return TCP_ERR_ARG;

@@ -5,13 +5,16 @@
 !
 ! The host arrays qel/qion stay authoritative: each subroutine uploads what it
 ! reads and downloads what it writes, so Restart/DUMP/SAMPLING (result.f90) see
-! exactly the state the reference would hold.  On one GPU the program runs as
-! `mpirun -np 2`: rank 0 is the PME rank of data.f90:151 and does nothing here
-! (the reference's LOCAL gather/scatter are dropped on both ranks alike), rank 1
-! is the only direct rank and owns the GPU context with every particle.  With
-! several direct ranks the real-space ring (Force_direct_ext, main.f90:107-130)
-! runs on each rank's GPU; the reciprocal part then needs every position on one
-! context, which this shim does not gather, so it stops with a message.
+! exactly the state the reference would hold.  With one direct rank
+! (`mpirun -np 2`) rank 0, the PME rank of data.f90:151, does nothing (the
+! LOCAL gather/scatter are dropped on both ranks alike) and rank 1 owns the GPU
+! context with every particle, reciprocal part included.  With several direct
+! ranks (one PME group, at most 9 ranks) every rank keeps the reference's
+! communication: the direct ranks run the real-space ring (Force_direct_ext,
+! main.f90:107-130) on their own contexts, the PME rank receives every position
+! through the LOCAL gather (force.f90:679) and computes the reciprocal part on
+! its own context, and the kinetic sums of BEREND/ISOKIN are reduced over
+! COMM1D between the kick and the rescale (vverlet.f90:254, 278).
 !
 ! Environment: TCP_PRECISION=MIXED selects the mixed-precision pair kernel
 ! (FP64 otherwise).
@@ -21,22 +24,32 @@ USE DATASTR
 USE TCP_GPU_BIND
 IMPLICIT NONE
 INTEGER(C_INT64_T), SAVE :: stoch_seed = 20240607_C_INT64_T
+! rank of this direct rank in COMM1D: separates the ranks' noise streams
+INTEGER(C_INT64_T), SAVE :: noise_rank = 0_C_INT64_T
 CONTAINS
 
-SUBROUTINE ensure_ctx(NS, param, single)
+! direct ranks of the (single) PME group
+INTEGER FUNCTION direct_ranks(NS)
+   TYPE(NM) :: NS
+   direct_ranks = NS%NCPUS - NS%Npe
+END FUNCTION direct_ranks
+
+! The rank's context: ne electrons and ni ions, reciprocal part on it or not.
+SUBROUTINE ensure_ctx(NS, param, ne, ni, with_recip)
    TYPE(NM) :: NS
    TYPE(PM) :: param
-   LOGICAL, INTENT(IN) :: single
+   INTEGER, INTENT(IN) :: ne, ni
+   LOGICAL, INTENT(IN) :: with_recip
    TYPE(tcp_params_t) :: p
    INTEGER(C_INT) :: rc
    IF (C_ASSOCIATED(tcp_ctx_h)) RETURN
-   p%ne = NS%Nel
-   p%ni = NS%Nion
+   p%ne = ne
+   p%ni = ni
    p%K = param%K
    p%order = 4                       ! Force_SPME4
    p%kmax = 0
    p%recip = TCP_RECIP_SPME
-   IF (.NOT. single) p%recip = TCP_RECIP_NONE
+   IF (.NOT. with_recip) p%recip = TCP_RECIP_NONE
    p%flags = 0                       ! the reference's model: Deutsch e-i only
    p%box = param%box
    p%q = param%q
@@ -53,8 +66,8 @@ SUBROUTINE ensure_ctx(NS, param, single)
    rc = tcp_create(tcp_ctx_h, p, tcp_precision())
    IF (rc == TCP_ERR_CUTOFF) STOP " *** error at cut-off radius *** "
    CALL tcp_check(rc, 'tcp_create')
-   tcp_n = NS%Nel + NS%Nion
-   tcp_ne = NS%Nel
+   tcp_n = ne + ni
+   tcp_ne = ne
    ALLOCATE(hx(3, tcp_n), hv(3, tcp_n), hf(3, tcp_n))
 END SUBROUTINE ensure_ctx
 
@@ -132,12 +145,50 @@ SUBROUTINE vv2_common(NS, qel, qion, param, sys, dt, code)
    CALL put(NS, qel, qion, .FALSE., .TRUE., .TRUE.)
    stoch_seed = stoch_seed + 1
    CALL tcp_check(tcp_vverlet2(tcp_ctx_h, code, dt, param%Te, param%Ti, param%tau, param%alpha, &
-        stoch_seed, a, b), 'tcp_vverlet2')
+        stoch_seed + noise_rank*1099511627776_C_INT64_T, a, b), 'tcp_vverlet2')
    sys%mv2el = a
    sys%mv2ion = b
    CALL get(NS, qel, qion, .FALSE., .TRUE., code == TCP_STOCH)
    CALL add_vdf(NS)
 END SUBROUTINE vv2_common
+
+! VVerletBerend2 / VVerletIsokin2 with several direct ranks: kick, the
+! COMM1D sums of m v^2 per species, lambda as vverlet.f90:255-256, 279-280
+! (BEREND) or 329-330, 353-354 (ISOKIN), rescale.
+SUBROUTINE vv2_split(NS, qel, qion, param, sys, dt, COMM, berend)
+   INCLUDE 'mpif.h'
+   TYPE(NM) :: NS
+   TYPE(PT) :: qel(NS%Nel), qion(NS%Nion)
+   TYPE(PM) :: param
+   TYPE(ST) :: sys
+   TYPE(MP) :: COMM
+   REAL(KIND=DP) :: dt
+   LOGICAL, INTENT(IN) :: berend
+   REAL(C_DOUBLE) :: a, b
+   REAL(KIND=DP) :: mv2, T0, lam_e, lam_i
+   INTEGER :: IERR
+   CALL put(NS, qel, qion, .FALSE., .TRUE., .TRUE.)
+   CALL tcp_check(tcp_vverlet2_kick(tcp_ctx_h, dt, a, b), 'tcp_vverlet2_kick')
+   CALL MPI_ALLREDUCE(a, mv2, 1, MPI_REAL8, MPI_SUM, COMM%COMM1D, IERR)
+   T0 = mv2/DBLE(3*NS%NTel)
+   IF (berend) THEN
+      lam_e = DSQRT(1.D0+ dt*(param%Te/T0 - 1.D0)/param%tau)
+   ELSE
+      lam_e = DSQRT(param%Te/T0)
+   END IF
+   CALL MPI_ALLREDUCE(b, mv2, 1, MPI_REAL8, MPI_SUM, COMM%COMM1D, IERR)
+   T0 = mv2/DBLE(3*NS%NTion)
+   IF (berend) THEN
+      lam_i = DSQRT(1.D0+ dt*(param%Ti/T0 - 1.D0)/param%tau)
+   ELSE
+      lam_i = DSQRT(param%Ti/T0)
+   END IF
+   CALL tcp_check(tcp_vverlet2_scale(tcp_ctx_h, lam_e, lam_i, a, b), 'tcp_vverlet2_scale')
+   sys%mv2el = a
+   sys%mv2ion = b
+   CALL get(NS, qel, qion, .FALSE., .TRUE., .FALSE.)
+   CALL add_vdf(NS)
+END SUBROUTINE vv2_split
 
 END MODULE TCP_GPU_STATE
 
@@ -162,10 +213,16 @@ END IF
 param%a = 3.1D0/param%rc
 param%rc2 = param%rc**2
 PSI = 0.0D0
+IF (NS%Npe > 1) STOP " *** GPU shim: one PME group only (at most 9 MPI ranks) *** "
 IF (COMM%TAG) THEN
-   CALL ensure_ctx(NS, param, COMM%NUMPROC == 1)
+   ! Uo from the rank's own particles, as force.f90:45-46
+   CALL ensure_ctx(NS, param, NS%Nel, NS%Nion, direct_ranks(NS) == 1)
+   noise_rank = INT(COMM%MYID2, C_INT64_T)
    CALL tcp_check(tcp_ewald_self(tcp_ctx_h, a, rc2, uo, C_NULL_PTR), 'tcp_ewald_self')
    sys%Uo = uo
+ELSE IF (direct_ranks(NS) > 1) THEN
+   ! the PME rank's context holds the group's particles for the reciprocal part
+   CALL ensure_ctx(NS, param, NS%NTel, NS%NTion, .TRUE.)
 END IF
 END SUBROUTINE EWALD_SELF
 
@@ -233,18 +290,64 @@ DO i = 1, NS%Nion
 END DO
 END SUBROUTINE Force_direct_sum
 
-! Positions reach the GPU in SPME_SUM; the LOCAL gather to the PME rank is not needed.
+! One direct rank: positions reach the GPU in SPME_SUM, no LOCAL gather.
+! Several: the reference's LOCAL gather (force.f90:679-680), then the PME rank
+! computes the reciprocal forces of every gathered particle on its context and
+! leaves them in pme for SPME_SUM's scatter.  pme holds each direct rank's
+! electrons then its ions (force.f90:694-697); the context holds all electrons
+! first.
 SUBROUTINE Force_SPME4(NS, param, sys, COMM, PSI, gsend, pme)
 USE DATASTR
+USE TCP_GPU_STATE
 IMPLICIT NONE
+INCLUDE 'mpif.h'
 TYPE(NM):: NS
 TYPE(PM):: param
 TYPE(ST):: sys
 TYPE(MP):: COMM
 TYPE(GH):: gsend(NS%Ng), pme(NS%NLpt)
 REAL(KIND=DP) :: PSI(param%K(1), param%K(2), param%K(3))
-IF (COMM%TAG .AND. COMM%NUMPROC > 1) &
-     STOP " *** GPU shim: the reciprocal part needs a single direct rank (mpirun -np 2) *** "
+REAL(C_DOUBLE) :: um
+INTEGER :: l, x, m, ke, ki, IERR
+IF (direct_ranks(NS) == 1) RETURN
+CALL MPI_GATHERV(gsend, NS%CNT(COMM%LID+1), COMM%PTCL, pme, NS%CNT, &
+     NS%DPL, COMM%PTCL, 0, COMM%LOCAL, IERR)
+IF (COMM%TAG) RETURN
+m = 0
+ke = 0
+ki = NS%NTel
+DO l = 1, COMM%Ncolmn - 1
+   DO x = 1, NS%NGHB(l,1)
+      m = m + 1
+      ke = ke + 1
+      hx(:, ke) = pme(m)%xx
+   END DO
+   DO x = 1, NS%NGHB(l,2)
+      m = m + 1
+      ki = ki + 1
+      hx(:, ki) = pme(m)%xx
+   END DO
+END DO
+hf = 0.0D0
+CALL tcp_check(tcp_set_state(tcp_ctx_h, C_LOC(hx), C_NULL_PTR, C_LOC(hf)), 'tcp_set_state')
+CALL tcp_check(tcp_force_recip(tcp_ctx_h, um), 'tcp_force_recip')
+sys%Um = um
+CALL tcp_check(tcp_get_state(tcp_ctx_h, C_NULL_PTR, C_NULL_PTR, C_LOC(hf)), 'tcp_get_state')
+m = 0
+ke = 0
+ki = NS%NTel
+DO l = 1, COMM%Ncolmn - 1
+   DO x = 1, NS%NGHB(l,1)
+      m = m + 1
+      ke = ke + 1
+      pme(m)%xx = hf(:, ke)
+   END DO
+   DO x = 1, NS%NGHB(l,2)
+      m = m + 1
+      ki = ki + 1
+      pme(m)%xx = hf(:, ki)
+   END DO
+END DO
 END SUBROUTINE Force_SPME4
 
 SUBROUTINE SPME_SUM(NS, qel, qion, sys, COMM, pme, grec)
@@ -257,11 +360,32 @@ TYPE(ST):: sys
 TYPE(MP):: COMM
 TYPE(GH):: grec(NS%Ng), pme(NS%NLpt)
 REAL(C_DOUBLE) :: um
-IF (.NOT. COMM%TAG) RETURN
-CALL put(NS, qel, qion, .TRUE., .FALSE., .TRUE.)
-CALL tcp_check(tcp_force_recip(tcp_ctx_h, um), 'tcp_force_recip')
-sys%Um = um
-CALL get(NS, qel, qion, .FALSE., .FALSE., .TRUE.)
+INTEGER :: i, k, IERR
+INCLUDE 'mpif.h'
+IF (direct_ranks(NS) == 1) THEN
+   IF (.NOT. COMM%TAG) RETURN
+   CALL put(NS, qel, qion, .TRUE., .FALSE., .TRUE.)
+   CALL tcp_check(tcp_force_recip(tcp_ctx_h, um), 'tcp_force_recip')
+   sys%Um = um
+   CALL get(NS, qel, qion, .FALSE., .FALSE., .TRUE.)
+   RETURN
+END IF
+! several direct ranks: the reference's broadcast and scatter (force.f90:950-965)
+CALL MPI_BCAST(sys%Um, 1, MPI_REAL8, 0, COMM%LOCAL, IERR)
+CALL MPI_SCATTERV(pme, NS%CNT, NS%DPL, COMM%PTCL, grec, NS%CNT(COMM%LID+1), &
+     COMM%PTCL, 0, COMM%LOCAL, IERR)
+IF (COMM%TAG) THEN
+   DO i = 1, NS%Nel
+      DO k = 1, 3
+         qel(i)%ff(k) = qel(i)%ff(k) + grec(i)%xx(k)
+      END DO
+   END DO
+   DO i = 1, NS%Nion
+      DO k = 1, 3
+         qion(i)%ff(k) = qion(i)%ff(k) + grec(i+NS%Nel)%xx(k)
+      END DO
+   END DO
+END IF
 END SUBROUTINE SPME_SUM
 
 SUBROUTINE REMOVE_RIGIDMTN(NS, qel, qion, COMM)
@@ -376,8 +500,11 @@ TYPE(PM)::param
 TYPE(ST)::sys
 TYPE(MP)::COMM
 REAL(KIND=DP)::dt
-IF (COMM%NUMPROC > 1) STOP " *** GPU shim: BEREND needs a single direct rank *** "
-CALL vv2_common(NS, qel, qion, param, sys, dt, TCP_BEREND)
+IF (direct_ranks(NS) > 1) THEN
+   CALL vv2_split(NS, qel, qion, param, sys, dt, COMM, .TRUE.)
+ELSE
+   CALL vv2_common(NS, qel, qion, param, sys, dt, TCP_BEREND)
+END IF
 END SUBROUTINE VVerletBerend2
 
 SUBROUTINE VVerletIsokin2(NS, qel, qion, param, sys, dt, COMM)
@@ -390,8 +517,11 @@ TYPE(PM)::param
 TYPE(ST)::sys
 TYPE(MP)::COMM
 REAL(KIND=DP)::dt
-IF (COMM%NUMPROC > 1) STOP " *** GPU shim: ISOKIN needs a single direct rank *** "
-CALL vv2_common(NS, qel, qion, param, sys, dt, TCP_ISOKIN)
+IF (direct_ranks(NS) > 1) THEN
+   CALL vv2_split(NS, qel, qion, param, sys, dt, COMM, .FALSE.)
+ELSE
+   CALL vv2_common(NS, qel, qion, param, sys, dt, TCP_ISOKIN)
+END IF
 END SUBROUTINE VVerletIsokin2
 
 ! Unit-variance Gaussian times x (Box-Muller on RANDOM_NUMBER); kept for link

@@ -83,6 +83,19 @@ INTERFACE
       INTEGER(C_INT64_T), VALUE :: seed
       REAL(C_DOUBLE) :: mv2el, mv2ion
    END FUNCTION
+   INTEGER(C_INT) FUNCTION tcp_vverlet2_kick(ctx, dt, mv2el, mv2ion) BIND(C, NAME='tcp_vverlet2_kick')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), VALUE :: dt
+      REAL(C_DOUBLE) :: mv2el, mv2ion
+   END FUNCTION
+   INTEGER(C_INT) FUNCTION tcp_vverlet2_scale(ctx, lam_e, lam_i, mv2el, mv2ion) &
+        BIND(C, NAME='tcp_vverlet2_scale')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), VALUE :: lam_e, lam_i
+      REAL(C_DOUBLE) :: mv2el, mv2ion
+   END FUNCTION
    INTEGER(C_INT) FUNCTION tcp_get_histograms(ctx, rdf, vdf, reset) BIND(C, NAME='tcp_get_histograms')
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx, rdf, vdf

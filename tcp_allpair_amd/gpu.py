@@ -37,6 +37,8 @@ SIGNATURES = {
     "tcp_md_steps": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _dp],
     "tcp_last_timings": [_vp, _dp, _dp],
+    "tcp_vverlet2_kick": [_vp, ctypes.c_double, _dp, _dp],
+    "tcp_vverlet2_scale": [_vp, ctypes.c_double, ctypes.c_double, _dp, _dp],
     "tcp_set_lambda": [_vp, ctypes.c_double],
     "tcp_energy_adjoint": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
     "tcp_batch_create": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(CParams), ctypes.c_int],
@@ -177,6 +179,18 @@ class Context:
                                        float(p.fric if fric is None else fric),
                                        int(self.seed if seed is None else seed),
                                        ctypes.byref(a), ctypes.byref(b)), "tcp_vverlet2")
+        return a.value, b.value
+
+    def vverlet2_kick(self, dt):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._ck(self.lib.tcp_vverlet2_kick(self.h, float(dt), ctypes.byref(a), ctypes.byref(b)),
+                 "tcp_vverlet2_kick")
+        return a.value, b.value
+
+    def vverlet2_scale(self, lam_e, lam_i):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._ck(self.lib.tcp_vverlet2_scale(self.h, float(lam_e), float(lam_i), ctypes.byref(a),
+                                             ctypes.byref(b)), "tcp_vverlet2_scale")
         return a.value, b.value
 
     def histograms(self, reset=False):

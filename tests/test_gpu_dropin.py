@@ -1,47 +1,57 @@
 """The unchanged reference driver (main.f90, init.f90, data.f90, result.f90)
-linked with the GPU drop-in for force.f90/vverlet.f90, run as `mpirun -np 2`
-(rank 0 the idle PME rank, rank 1 the direct rank on the GPU), against the
-oracle's step on the same restart state.
+linked with the GPU drop-in for force.f90/vverlet.f90, against the oracle's
+step on the same restart state.
+
+`mpirun -np 2`: rank 0 is the idle PME rank, rank 1 the only direct rank with
+every particle on its context.  `-np 3` and `-np 4`: two and three direct
+ranks run the reference's real-space ring (the even case with its antipodal
+exchange), the PME rank computes the reciprocal part of every gathered
+particle on its own context, and BEREND/ISOKIN reduce their kinetic sums
+over the direct ranks.
 
 Tolerances: the reference's own writers print ES15.8 energies and ES14.6
 positions (result.f90:324, :131), so the files are compared at 2e-8 and 2e-6
-relative respectively.
+relative respectively.  The RDF is compared where the reference counts each
+pair once (one direct rank, or an odd number of them); with an even number
+the antipodal blocks are counted by both partners (main.f90:121-130).
 """
 import os
-import shutil
 import subprocess
-import sys
 
 import numpy as np
 import pytest
 
 from oracle import cpu
 from tcp_allpair_amd.inputs import make_plasma, read_restart_xyz, write_restart_xyz, write_tcp_prm
-from tcp_allpair_amd.params import NBIN, NOTEMP, PI, TPS, reduced_dt
+from tcp_allpair_amd.params import BEREND, ISOKIN, NBIN, NOTEMP, PI, TPS, reduced_dt
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tcp_allpair_amd", "fortran", "_build", "tcp_pbc_gpu")
 MPIRUN = "/opt/conda/bin/mpirun"
+THERMO_NAME = {NOTEMP: "NOTEMP", BEREND: "BEREND", ISOKIN: "ISOKIN"}
 
 
 @pytest.mark.skipif(not os.path.exists(BIN), reason="drop-in driver not built (needs the reference sources)")
-def test_dropin_driver_nve(tmp_path):
-    print("case dropin nve", flush=True)
+@pytest.mark.parametrize("nranks,thermo", [(2, NOTEMP), (3, NOTEMP), (4, NOTEMP), (4, BEREND),
+                                           (3, ISOKIN)])
+def test_dropin_driver(tmp_path, nranks, thermo):
+    print("case dropin", nranks, THERMO_NAME[thermo], flush=True)
     nsteps = 10
-    p, x, v = make_plasma(250, 250, seed=51, K=(32, 32, 32), order=4)
-    p = p.replace(tau=1.0, fric=0.0)
+    p, x, v = make_plasma(250, 251, seed=51, K=(32, 32, 32), order=4)
+    p = p.replace(tau=0.5, fric=0.0, Te=p.Te * 1.2, Ti=p.Ti * 0.8)
     f0, _, _ = cpu.force_allpair(p, x)
     fr, _ = cpu.spme(p, x)
     f0 = cpu.remove_rigidmtn(p, f0 + fr)
     dt = reduced_dt(0.005)
     dt_fs = dt * TPS
     write_tcp_prm(tmp_path / "tcp.prm", p, tmax_fs=nsteps * dt_fs, trest_fs=nsteps * dt_fs,
-                  tdump_fs=5 * dt_fs, tsamp_fs=dt_fs, dt_fs=dt_fs, thermo="NOTEMP", rest="ON")
+                  tdump_fs=5 * dt_fs, tsamp_fs=dt_fs, dt_fs=dt_fs, thermo=THERMO_NAME[thermo],
+                  rest="ON")
     write_restart_xyz(tmp_path / "restart.xyz", p, x, v, f0)
     env = dict(os.environ, TCP_PRECISION="FP64")
-    r = subprocess.run([MPIRUN, "-launcher", "fork", "-n", "2", BIN], cwd=tmp_path, env=env,
+    r = subprocess.run([MPIRUN, "-launcher", "fork", "-n", str(nranks), BIN], cwd=tmp_path, env=env,
                        capture_output=True, text=True, timeout=240)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
@@ -52,7 +62,7 @@ def test_dropin_driver_nve(tmp_path):
     rdf = np.zeros(3 * NBIN)
     vdf = np.zeros(2 * NBIN)
     for s in range(nsteps):
-        o = cpu.step(p, NOTEMP, dt_int, xr, vr, frr, rdf, vdf, fast=True)
+        o = cpu.step(p, thermo, dt_int, xr, vr, frr, rdf, vdf, fast=True)
         ep = o["Epot"] + o["Um"]
         rows.append([(s + 1) * dt_int * TPS, o["mv2el"] / (3 * p.ne), o["mv2ion"] / (3 * p.ni), ep,
                      0.5 * (o["mv2el"] + o["mv2ion"]) + ep])
@@ -65,9 +75,16 @@ def test_dropin_driver_nve(tmp_path):
     np.testing.assert_allclose(xs, xr, rtol=2e-6, atol=1e-6 * p.box[0])
     np.testing.assert_allclose(vs, vr, rtol=2e-6, atol=1e-6 * np.abs(vr).max())
     np.testing.assert_allclose(fs, frr, rtol=2e-6, atol=1e-6 * np.abs(frr).max())
+    # VDF001.dat: counts of the first 5 steps
+    ntau = 5.0
+    vcols = np.loadtxt(tmp_path / "VDF001.dat")
+    hv = vdf5.reshape(2, NBIN)
+    np.testing.assert_allclose(vcols[:, 1], hv[0, :NBIN - 1] / ntau, rtol=2e-6)
+    np.testing.assert_allclose(vcols[:, 3], hv[1, :NBIN - 1] / ntau, rtol=2e-6)
+    if (nranks - 1) % 2 == 0 and nranks > 2:
+        return
     # RDF001.dat: counts of the first 5 steps, normalised as result.f90:244-252
     V = p.volume
-    ntau = 5.0
     cols = np.loadtxt(tmp_path / "RDF001.dat")
     i = np.arange(1, NBIN)
     shell = (i ** 3 - (i - 1) ** 3).astype(float)
@@ -76,7 +93,3 @@ def test_dropin_driver_nve(tmp_path):
     h = rdf5.reshape(3, NBIN)
     for t in range(3):
         np.testing.assert_allclose(cols[:, 2 * t + 1], h[t, :NBIN - 1] / norms[t] / shell, rtol=2e-6, atol=1e-12)
-    vcols = np.loadtxt(tmp_path / "VDF001.dat")
-    hv = vdf5.reshape(2, NBIN)
-    np.testing.assert_allclose(vcols[:, 1], hv[0, :NBIN - 1] / ntau, rtol=2e-6)
-    np.testing.assert_allclose(vcols[:, 3], hv[1, :NBIN - 1] / ntau, rtol=2e-6)

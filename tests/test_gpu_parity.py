@@ -142,6 +142,34 @@ def test_vverlet2_vs_reference(name):
     np.testing.assert_array_equal(vdf, d["out_vdf"])
 
 
+@pytest.mark.parametrize("name", ["vv2_berend", "vv2_isokin"])
+def test_vverlet2_split_matches_fused(name):
+    """Kick, lambda on the host from the returned sums (as the shim forms it after its
+    COMM1D reduction), rescale: bit for bit the fused BEREND/ISOKIN update."""
+    import math
+    say("case split", name)
+    p, d = load(name)
+    thermo, dt = int(d["thermo"]), float(d["dt"])
+    with Context(p.replace(recip=RECIP_NONE), FP64) as c:
+        c.set_state(x=np.zeros((p.n, 3)), v=d["in_v"], f=d["in_f"])
+        ref = c.vverlet2(thermo, dt)
+        v_ref = c.get_state()[1]
+        _, vdf_ref = c.histograms(reset=True)
+        c.set_state(v=d["in_v"], f=d["in_f"])
+        a, b = c.vverlet2_kick(dt)
+        lam = []
+        for mv2, n, T in ((a, p.ne, p.Te), (b, p.ni, p.Ti)):
+            T0 = mv2 / float(3 * n)
+            lam.append(math.sqrt(1.0 + dt * (T / T0 - 1.0) / p.tau) if thermo == BEREND
+                       else math.sqrt(T / T0))
+        out = c.vverlet2_scale(*lam)
+        v = c.get_state()[1]
+        _, vdf = c.histograms()
+    assert out == ref
+    np.testing.assert_array_equal(v, v_ref)
+    np.testing.assert_array_equal(vdf, vdf_ref)
+
+
 def test_stoch_noise_statistics():
     """Langevin kick with zero force and v = 0: v = eta beta dt/2/m/(1 + fric dt/2/m)."""
     say("case stoch")
