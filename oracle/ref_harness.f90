@@ -28,6 +28,11 @@
 ! mode 7: nsteps of the main.f90:81-170 step without the reciprocal part:
 !         VVerlet*1, Force_direct_self, REMOVE_RIGIDMTN, VVerlet*2
 ! mode 8: timing of Force_direct_ext for nown own particles against all N
+! mode 9: the output writers of result.f90 on given data: DUMP (RDF###.dat,
+!         VDF###.dat), SAMPLING (a row of ener00.dat) and Restart (REST##.xyz,
+!         then the header of the next ener##.dat), written into the current
+!         directory.  Extra input after f: int32 Ndump, Nloop, Nrest, id(N);
+!         real64 tnow, mv2el, mv2ion, Epot, Um, RDF(501,3), VDF(501,2).
 PROGRAM REF_HARNESS
 USE DATASTR
 IMPLICIT NONE
@@ -36,10 +41,14 @@ TYPE(NM) :: NS
 TYPE(PM) :: param
 TYPE(ST) :: sys
 TYPE(MP) :: COMM
+TYPE(TM) :: ttime
 TYPE(PT), ALLOCATABLE :: qel(:), qion(:)
 TYPE(GH), ALLOCATABLE :: gsend(:), gemit(:), grec(:)
 REAL(KIND=DP), ALLOCATABLE :: PSI(:,:,:), xin(:,:), vin(:,:), fin(:,:)
-REAL(KIND=DP), ALLOCATABLE :: trace(:,:)
+REAL(KIND=DP), ALLOCATABLE :: trace(:,:), hrdf(:,:), hvdf(:,:)
+INTEGER, ALLOCATABLE :: ids(:)
+INTEGER :: w_ndump, w_nloop, w_nrest
+REAL(KIND=DP) :: w_tnow, w_mv2el, w_mv2ion, w_epot, w_um
 INTEGER :: mode, Ne, Ni, K(3), nsteps, thermo, nown_e, nown_i, N, i, ierr
 INTEGER :: istep, next_e, next_i, nrep
 REAL(KIND=DP) :: dt, Ur, Ur_self, t0, t1
@@ -59,6 +68,14 @@ ALLOCATE(xin(3,N), vin(3,N), fin(3,N))
 READ(21) xin
 READ(21) vin
 READ(21) fin
+IF (mode == 9) THEN
+   ALLOCATE(ids(N), hrdf(Ndist+1, 3), hvdf(Ndist+1, 2))
+   READ(21) w_ndump, w_nloop, w_nrest
+   READ(21) ids
+   READ(21) w_tnow, w_mv2el, w_mv2ion, w_epot, w_um
+   READ(21) hrdf
+   READ(21) hvdf
+END IF
 CLOSE(21)
 
 param%K = K
@@ -183,6 +200,36 @@ CASE (8)
    END DO
    t1 = MPI_Wtime()
    WRITE(22) DBLE(NS%Npt)*DBLE(N)*DBLE(nrep), t1 - t0, Ur
+CASE (9)
+   COMM%NUMPROC = 1
+   COMM%MYID2 = 0
+   ALLOCATE(NS%NOPT(1, 2))
+   NS%NOPT(1, 1) = Ne
+   NS%NOPT(1, 2) = Ni
+   DO i = 1, Ne
+      qel(i)%id = ids(i)
+   END DO
+   DO i = 1, Ni
+      qion(i)%id = ids(Ne + i)
+   END DO
+   ttime%Ndump = w_ndump
+   ttime%Nloop = w_nloop
+   ttime%Nrest = w_nrest
+   ttime%tnow = w_tnow
+   NS%RDF_ee = hrdf(:, 1)
+   NS%RDF_ei = hrdf(:, 2)
+   NS%RDF_ii = hrdf(:, 3)
+   NS%VDF_e = hvdf(:, 1)
+   NS%VDF_i = hvdf(:, 2)
+   sys%mv2el = w_mv2el
+   sys%mv2ion = w_mv2ion
+   sys%Epot = w_epot
+   sys%Um = w_um
+   OPEN(UNIT=55, FILE="ener00.dat")
+   CALL DUMP(NS, ttime, param, COMM)
+   CALL SAMPLING(NS, ttime, sys, COMM, param)
+   CALL Restart(NS, qel, qion, ttime, param, COMM)
+   CLOSE(55)
 CASE DEFAULT
    PRINT *, "unknown mode", mode
    STOP 2

@@ -80,3 +80,23 @@ def run(mode, p: TcpParams, x, v=None, f=None, dt=0.0, nsteps=0, thermo=0, nown=
     elif mode == MODE_TRAJ:
         out["trace"] = r.f8(4 * nsteps).reshape(nsteps, 4)
     return out
+
+
+def writers(cwd, p: TcpParams, x, v, f, ids, ndump, nloop, nrest, tnow, mv2el, mv2ion, epot, um,
+            rdf, vdf, timeout=120):
+    """Mode 9: the reference's own DUMP, SAMPLING and Restart (result.f90) write
+    their files into cwd for the given data."""
+    if not available():
+        raise RuntimeError("oracle/_ref/ref_harness is not built (make -C oracle ref)")
+    fin = os.path.join(cwd, "writers_in.bin")
+    _write_input(fin, 9, p, x, v, f)
+    with open(fin, "ab") as fh:
+        fh.write(np.array([ndump, nloop, nrest], dtype="<i4").tobytes())
+        fh.write(np.asarray(ids, dtype="<i4").tobytes())
+        fh.write(np.array([tnow, mv2el, mv2ion, epot, um], dtype="<f8").tobytes())
+        fh.write(np.ascontiguousarray(np.asarray(rdf, dtype="<f8").reshape(3, NBIN)).tobytes())
+        fh.write(np.ascontiguousarray(np.asarray(vdf, dtype="<f8").reshape(2, NBIN)).tobytes())
+    subprocess.run([HARNESS, fin, os.path.join(cwd, "writers_out.bin")], check=True,
+                   timeout=timeout, cwd=cwd, stdout=subprocess.DEVNULL)
+    os.remove(fin)
+    os.remove(os.path.join(cwd, "writers_out.bin"))

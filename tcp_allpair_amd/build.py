@@ -4,7 +4,9 @@
 
 Each .hip file under csrc/ is compiled to an object in _build/ and the objects
 are linked into tcp_allpair_amd/libtcp_gpu.so.  Packed FP32 instructions are
-disabled for all device code (-packed-fp32-ops).
+disabled for all device code (-packed-fp32-ops).  The host programs under
+host/ are built with g++: libtcp_io.so (readers and writers of the reference's
+text files) and tcp_run (the standalone driver over libtcp_gpu.so).
 """
 from __future__ import annotations
 
@@ -29,6 +31,11 @@ FLAGS = [
 ]
 SOURCES = ["kernels_pair.hip", "kernels_spme.hip", "kernels_md.hip", "tcp_gpu.hip", "batch.hip"]
 PROBE = os.path.join(HERE, "copy_probe")   # HBM-counter calibration program
+HOST = os.path.join(HERE, "host")
+IO_LIB = os.path.join(HERE, "libtcp_io.so")
+RUN = os.path.join(HERE, "tcp_run")
+GXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-I", os.path.join(ROOT, "include")]
 
 
 def _compile(src: str) -> str:
@@ -54,9 +61,19 @@ def build(verbose: bool = False) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for copy_probe.hip:\n{r.stderr}")
+    _run([GXX, *HOST_FLAGS, "-fPIC", "-shared", os.path.join(HOST, "tcp_io.cpp"), "-o", IO_LIB],
+         "libtcp_io.so")
+    _run([GXX, *HOST_FLAGS, os.path.join(HOST, "tcp_run.cpp"), os.path.join(HOST, "tcp_io.cpp"),
+          "-L", HERE, "-ltcp_gpu", "-Wl,-rpath,$ORIGIN", "-o", RUN], "tcp_run")
     if verbose:
-        print("built", LIB, "and", PROBE)
+        print("built", LIB, PROBE, IO_LIB, "and", RUN)
     return LIB
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build of {what} failed:\n{r.stderr}")
 
 
 if __name__ == "__main__":

@@ -153,7 +153,12 @@ __global__ __launch_bounds__(256) void gather_kernel(RecipArgs a)
 // its particles into an LDS tile of (kBrick + p - 1)^3 points (the brick plus
 // the halo its splines reach), writes the tile out, and a gather over the
 // tiles that cover each grid point (up to 3 per axis) forms Q in a fixed
-// order.
+// order.  Inside a brick the contributions are added as 64-bit fixed-point
+// integers scaled by 2^k with k chosen per brick so that (particles in the
+// brick) x max|q| stays below 2^62: integer addition is associative, so the
+// tile and hence Q do not depend on the order in which threads add, and the
+// resolution 2^-k (2^-56 for 32 unit charges) is finer than FP64 rounding of
+// the tile values.
 
 __device__ inline int brick_axis(double u, int P, int K)
 {
@@ -216,15 +221,28 @@ __global__ __launch_bounds__(256) void spread_brick_kernel(RecipArgs a)
 {
     constexpr int E = kBrick + P - 1;      // tile edge
     constexpr int kBatch = 64;             // particles staged per pass
-    __shared__ double tile[E * E * E];
+    __shared__ unsigned long long tile[E * E * E];
     __shared__ double ws[kBatch][3][P];
     __shared__ int org[kBatch][3];
+    __shared__ double qred[256];
     const int bid = blockIdx.x;
     const int bx = bid % a.nbrick[0], by = (bid / a.nbrick[0]) % a.nbrick[1],
               bz = bid / (a.nbrick[0] * a.nbrick[1]);
-    for (int t = threadIdx.x; t < E * E * E; t += blockDim.x) tile[t] = 0.0;
+    for (int t = threadIdx.x; t < E * E * E; t += blockDim.x) tile[t] = 0ull;
     const int p0 = a.brick_start[bid], p1 = a.brick_start[bid + 1];
     const int borg[3] = {bx * kBrick, by * kBrick, bz * kBrick};
+    // fixed-point scale: (p1 - p0) max|q| < 2^(62 - k)
+    double qm = 0.0;
+    for (int t = p0 + threadIdx.x; t < p1; t += blockDim.x) qm = fmax(qm, fabs(charge(a, a.sorted[t])));
+    qred[threadIdx.x] = qm;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) qred[threadIdx.x] = fmax(qred[threadIdx.x], qred[threadIdx.x + w]);
+        __syncthreads();
+    }
+    int e = 0;
+    frexp(qred[0] * (double)(p1 - p0), &e);
+    const int k = qred[0] > 0.0 ? min(1000, max(-1000, 62 - e)) : 0;
     for (int q0 = p0; q0 < p1; q0 += kBatch) {
         const int nq = min(kBatch, p1 - q0);
         __syncthreads();
@@ -247,16 +265,18 @@ __global__ __launch_bounds__(256) void spread_brick_kernel(RecipArgs a)
         __syncthreads();
         // one work item = one particle and one (j, k) row of its support
         for (int it = threadIdx.x; it < nq * P * P; it += blockDim.x) {
-            const int qq = it / (P * P), jk = it % (P * P), j = jk % P, k = jk / P;
-            const double wyz = ws[qq][1][j] * ws[qq][2][k];
-            double* row = tile + ((org[qq][2] + k) * E + org[qq][1] + j) * E + org[qq][0];
+            const int qq = it / (P * P), jk = it % (P * P), j = jk % P, kk = jk / P;
+            const double wyz = ws[qq][1][j] * ws[qq][2][kk];
+            unsigned long long* row = tile + ((org[qq][2] + kk) * E + org[qq][1] + j) * E + org[qq][0];
 #pragma unroll
-            for (int i = 0; i < P; ++i) atomicAdd(row + i, ws[qq][0][i] * wyz);
+            for (int i = 0; i < P; ++i)
+                atomicAdd(row + i, (unsigned long long)llrint(ldexp(ws[qq][0][i] * wyz, k)));
         }
     }
     __syncthreads();
     double* out = a.tiles + (size_t)bid * (E * E * E);
-    for (int t = threadIdx.x; t < E * E * E; t += blockDim.x) out[t] = tile[t];
+    for (int t = threadIdx.x; t < E * E * E; t += blockDim.x)
+        out[t] = ldexp((double)(long long)tile[t], -k);
 }
 
 // Q(g) = sum over the tiles covering g, in a fixed order (x fastest grid index)

@@ -23,7 +23,7 @@ import pytest
 
 from oracle import cpu
 from tcp_allpair_amd.inputs import make_plasma, read_restart_xyz, write_restart_xyz, write_tcp_prm
-from tcp_allpair_amd.params import BEREND, ISOKIN, NBIN, NOTEMP, PI, TPS, reduced_dt
+from tcp_allpair_amd.params import BEREND, ISOKIN, NBIN, NOTEMP, PI, STOCH, TPS, reduced_dt
 
 pytestmark = pytest.mark.gpu
 
@@ -93,3 +93,44 @@ def test_dropin_driver(tmp_path, nranks, thermo):
     h = rdf5.reshape(3, NBIN)
     for t in range(3):
         np.testing.assert_allclose(cols[:, 2 * t + 1], h[t, :NBIN - 1] / norms[t] / shell, rtol=2e-6, atol=1e-12)
+
+
+RUN = os.path.join(ROOT, "tcp_allpair_amd", "tcp_run")
+
+
+@pytest.mark.skipif(not os.path.exists(BIN), reason="drop-in driver not built (needs the reference sources)")
+@pytest.mark.parametrize("thermo", [NOTEMP, BEREND, STOCH])
+def test_standalone_driver_matches_dropin_bytes(tmp_path, thermo):
+    """tcp_run (C++ host I/O over the C ABI, steps in tcp_md_steps) and the unchanged
+    Fortran driver over the drop-in write the same files byte for byte in FP64."""
+    name = {NOTEMP: "NOTEMP", BEREND: "BEREND", STOCH: "STOCH"}[thermo]
+    print("case standalone vs dropin", name, flush=True)
+    nsteps = 12
+    p, x, v = make_plasma(300, 301, seed=77, K=(32, 32, 32), order=4)
+    p = p.replace(tau=0.5, fric=0.2, Te=p.Te * 1.1, Ti=p.Ti * 0.9)
+    f0, _, _ = cpu.force_allpair(p, x)
+    fr, _ = cpu.spme(p, x)
+    f0 = cpu.remove_rigidmtn(p, f0 + fr)
+    dt_fs = reduced_dt(0.005) * TPS
+    dirs = {}
+    for who in ("dropin", "standalone"):
+        d = tmp_path / who
+        d.mkdir()
+        write_tcp_prm(d / "tcp.prm", p, tmax_fs=nsteps * dt_fs, trest_fs=6 * dt_fs,
+                      tdump_fs=4 * dt_fs, tsamp_fs=2 * dt_fs, dt_fs=dt_fs, thermo=name, rest="ON")
+        write_restart_xyz(d / "restart.xyz", p, x, v, f0)
+        dirs[who] = d
+    env = dict(os.environ, TCP_PRECISION="FP64")
+    r = subprocess.run([MPIRUN, "-launcher", "fork", "-n", "2", BIN], cwd=dirs["dropin"], env=env,
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = subprocess.run([RUN], cwd=dirs["standalone"], env=env, capture_output=True, text=True,
+                       timeout=240)
+    print(r.stdout[-500:], r.stderr[-500:], flush=True)
+    assert r.returncode == 0
+    outs = sorted(n for n in os.listdir(dirs["dropin"]) if n not in ("tcp.prm", "restart.xyz"))
+    assert outs == sorted(n for n in os.listdir(dirs["standalone"]) if n not in ("tcp.prm", "restart.xyz"))
+    assert {"ener00.dat", "ener01.dat", "ener02.dat", "REST01.xyz", "REST02.xyz", "RDF001.dat",
+            "RDF003.dat", "VDF002.dat"} <= set(outs)
+    for n in outs:
+        assert (dirs["dropin"] / n).read_bytes() == (dirs["standalone"] / n).read_bytes(), n

@@ -274,12 +274,16 @@ def test_md_steps_with_spme_vs_oracle():
     assert rel_err(fg, fo).max() <= 1e-8
 
 
-def test_fp64_deterministic():
-    say("case deterministic")
-    p, x, v = make_plasma(3000, 3000, seed=5, K=(32, 32, 32))
+@pytest.mark.parametrize("recip,order", [(RECIP_NONE, 4), (RECIP_SPME, 4), (RECIP_SPME, 8),
+                                         (RECIP_EWALD, 4)])
+def test_fp64_deterministic(recip, order):
+    """Two runs of the same FP64 steps are identical bit for bit, the reciprocal part
+    included (fixed-point brick spreading, chunked structure factors)."""
+    say("case deterministic", recip, order)
+    p, x, v = make_plasma(3000, 3000, seed=5, K=(32, 32, 32), order=order, recip=recip, kmax=5)
     outs = []
     for _ in range(2):
-        with Context(p.replace(recip=RECIP_NONE), FP64) as c:
+        with Context(p, FP64) as c:
             c.set_state(x=x, v=v, f=np.zeros_like(x))
             tr = c.md_steps(3, NOTEMP, reduced_dt(0.005))
             outs.append((tr, c.get_state()[2]))
