@@ -388,16 +388,25 @@ struct MixPair {
     bool in;
 };
 
-template <bool kKelbg>
-__device__ inline MixPair mix_pair(const MixConst& k, const FixPos& pi, const FixPos& pj,
-                                   float qq, float eqq, bool reg, bool live)
+// The charge-free part of one pair: separation, r, 1/r and the numerator and
+// energy factors num, en with df = eps qi qj num/r^3 and du = qi qj en/r.
+struct MixRaw {
+    float dx, dy, dz, r, rinv, num, en;
+    bool in;
+};
+
+// kCheck: the pair may be a particle with itself (r = 0) or involve a padding
+// lane (live false); without it only the cut-off masks the pair.
+template <bool kKelbg, bool kCheck>
+__device__ inline MixRaw mix_raw(const MixConst& k, const FixPos& pi, const FixPos& pj, bool reg,
+                                 bool live)
 {
-    MixPair o;
+    MixRaw o;
     o.dx = fmaf((float)(int)(pi.ux - pj.ux), k.sx, pi.rx - pj.rx);
     o.dy = fmaf((float)(int)(pi.uy - pj.uy), k.sy, pi.ry - pj.ry);
     o.dz = fmaf((float)(int)(pi.uz - pj.uz), k.sz, pi.rz - pj.rz);
     const float r2 = fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz));
-    o.in = (r2 < k.rc2) & (r2 > 0.f) & live;
+    o.in = kCheck ? (r2 < k.rc2) & (r2 > 0.f) & live : r2 < k.rc2;
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
     const float e2 = __builtin_amdgcn_exp2f(k.na2l * r2);
@@ -424,10 +433,27 @@ __device__ inline MixPair mix_pair(const MixConst& k, const FixPos& pi, const Fi
                 close_ei(r, sr, 2.0f * px, y, ec, k.two_a_sqrtpi * r * e2, &num, &en);
         }
     }
-    const float rinv2 = rinv * rinv;
-    o.df = o.in ? eqq * num * rinv2 * rinv : 0.f;
-    o.du = o.in ? qq * en * rinv : 0.f;
     o.r = r;
+    o.rinv = rinv;
+    o.num = num;
+    o.en = en;
+    return o;
+}
+
+template <bool kKelbg, bool kCheck = true>
+__device__ inline MixPair mix_pair(const MixConst& k, const FixPos& pi, const FixPos& pj,
+                                   float qq, float eqq, bool reg, bool live)
+{
+    const MixRaw m = mix_raw<kKelbg, kCheck>(k, pi, pj, reg, live);
+    MixPair o;
+    o.dx = m.dx;
+    o.dy = m.dy;
+    o.dz = m.dz;
+    o.in = m.in;
+    o.r = m.r;
+    const float rinv2 = m.rinv * m.rinv;
+    o.df = m.in ? eqq * m.num * rinv2 * m.rinv : 0.f;
+    o.du = m.in ? qq * m.en * m.rinv : 0.f;
     return o;
 }
 
@@ -598,6 +624,66 @@ __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* h
     uo = u;
 }
 
+// kNI i tiles against one j tile, off the diagonal, every tile single-species and
+// fully live: each rotation step of the j data and reactions serves kNI pairs
+// per lane, so the nine DPP moves per step are shared by kNI pair evaluations.
+// The charges are uniform over the tile pair, so the sums run over the
+// charge-free terms num/r^3 and en/r and take eps qi qj (forces, reactions)
+// and qi qj (energy) once at the end.
+template <bool kRdf, bool kKelbg, int kNI>
+__device__ inline void n3_tile_multi(const MixConst& k, const PairArgs& a, unsigned* hist,
+                                     const FixPos (&pi)[kNI], int si, int sj, FixPos pj,
+                                     float (&fxo)[kNI], float (&fyo)[kNI], float (&fzo)[kNI],
+                                     float (&uo)[kNI], float& rx, float& ry, float& rz)
+{
+    const float qi = (float)a.q[si];
+    const float eqi = (float)(a.eps * a.q[si]);
+    const float qj = (float)a.q[sj];
+    const bool reg = si != sj || ((a.flags & TCP_REG_EE) != 0 && si == 0 && sj == 0);
+    const float idx = k.inv_dx[si + sj];
+    unsigned* hrow = hist + (si + sj) * kHistStride;
+    float fx[kNI], fy[kNI], fz[kNI], u[kNI];
+#pragma unroll
+    for (int n = 0; n < kNI; ++n) fx[n] = fy[n] = fz[n] = u[n] = 0.f;
+    rx = ry = rz = 0.f;
+    for (int s = 0; s < kTile; ++s) {
+#pragma unroll
+        for (int n = 0; n < kNI; ++n) {
+            const MixRaw m = mix_raw<kKelbg, false>(k, pi[n], pj, reg, true);
+            const float rm = m.in ? m.rinv : 0.f;
+            const float g = m.num * (m.rinv * m.rinv) * rm;
+            fx[n] = fmaf(g, m.dx, fx[n]);
+            fy[n] = fmaf(g, m.dy, fy[n]);
+            fz[n] = fmaf(g, m.dz, fz[n]);
+            u[n] = fmaf(m.en, rm, u[n]);
+            rx = fmaf(-g, m.dx, rx);
+            ry = fmaf(-g, m.dy, ry);
+            rz = fmaf(-g, m.dz, rz);
+            if (kRdf && m.in) atomicAdd(&hrow[min((int)(m.r * idx), kHistStride - 1)], 2u);
+        }
+        pj.ux = rot_u(pj.ux);
+        pj.uy = rot_u(pj.uy);
+        pj.uz = rot_u(pj.uz);
+        pj.rx = rot_f(pj.rx);
+        pj.ry = rot_f(pj.ry);
+        pj.rz = rot_f(pj.rz);
+        rx = rot_f(rx);
+        ry = rot_f(ry);
+        rz = rot_f(rz);
+    }
+    const float eqq = eqi * qj, qq = qi * qj;
+#pragma unroll
+    for (int n = 0; n < kNI; ++n) {
+        fxo[n] = eqq * fx[n];
+        fyo[n] = eqq * fy[n];
+        fzo[n] = eqq * fz[n];
+        uo[n] = qq * u[n];
+    }
+    rx *= eqq;
+    ry *= eqq;
+    rz *= eqq;
+}
+
 // One block pair (row br, column bc) of kBT x kBT tile pairs by kW wavefronts.
 // jpos, racc, hist: the block's LDS (kBT * kTile particles).
 template <bool kRdf, bool kKelbg, int kBT, int kW>
@@ -620,7 +706,81 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
     const MixConst k = mix_const(a);
     const int wave = threadIdx.x / kTile, lane = threadIdx.x % kTile;
     const int ntiles = (n + kTile - 1) / kTile;
-    for (int h = 0; h < kBT / kW; ++h) {
+    // a tile holds one species and only live particles
+    auto uniform_tile = [&](int T) {
+        return (T * kTile >= ne || T * kTile + kTile - 1 < ne) && T * kTile + kTile - 1 < n;
+    };
+    constexpr int kNI = kBT / kW;  // i tiles per wave
+    if (!dblk) {
+        // off the diagonal: all kNI i tiles of the wave walk the j tiles together
+        FixPos pi[kNI];
+        int ip[kNI];
+        bool exists[kNI];
+        double Fx[kNI], Fy[kNI], Fz[kNI], U[kNI];
+        bool all_uni = true;
+        const int s0 = (br * kBT + wave) * kTile >= ne;
+#pragma unroll
+        for (int h = 0; h < kNI; ++h) {
+            const int I = br * kBT + wave + h * kW;
+            exists[h] = I < ntiles;
+            ip[h] = I * kTile + lane;
+            pi[h] = a.ui[min(ip[h], n - 1)];
+            all_uni = all_uni && exists[h] && uniform_tile(I) && ((I * kTile >= ne) == (s0 != 0));
+            Fx[h] = Fy[h] = Fz[h] = U[h] = 0.0;
+        }
+        for (int tj = 0; tj < kBT; ++tj) {
+            const int J = bc * kBT + tj;
+            if (J >= ntiles) break;
+            const FixPos pj = jpos[tj * kTile + lane];
+            const int sj = J * kTile >= ne;
+            const bool juni = uniform_tile(J);
+            const int q = tj * kTile + lane;
+            if (all_uni && juni) {
+                float fx[kNI], fy[kNI], fz[kNI], u[kNI], rx, ry, rz;
+                n3_tile_multi<kRdf, kKelbg, kNI>(k, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+#pragma unroll
+                for (int h = 0; h < kNI; ++h) {
+                    Fx[h] += (double)fx[h];
+                    Fy[h] += (double)fy[h];
+                    Fz[h] += (double)fz[h];
+                    U[h] += (double)u[h];
+                }
+                atomicAdd(&racc[0][q], (double)rx);
+                atomicAdd(&racc[1][q], (double)ry);
+                atomicAdd(&racc[2][q], (double)rz);
+            } else {
+#pragma unroll
+                for (int h = 0; h < kNI; ++h) {
+                    if (!exists[h]) continue;
+                    const int I = br * kBT + wave + h * kW;
+                    const bool ilive = ip[h] < n;
+                    const int si = ip[h] >= ne;
+                    float fx, fy, fz, u, rx, ry, rz;
+                    if (uniform_tile(I) && juni)
+                        n3_tile<kRdf, kKelbg, false, true>(k, a, hist, pi[h], si, ilive, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                    else
+                        n3_tile<kRdf, kKelbg, false, false>(k, a, hist, pi[h], si, ilive, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                    Fx[h] += (double)fx;
+                    Fy[h] += (double)fy;
+                    Fz[h] += (double)fz;
+                    U[h] += (double)u;
+                    atomicAdd(&racc[0][q], (double)rx);
+                    atomicAdd(&racc[1][q], (double)ry);
+                    atomicAdd(&racc[2][q], (double)rz);
+                }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < kNI; ++h) {
+            if (!exists[h] || ip[h] >= n) continue;
+            double* pf = a.pf + (size_t)bc * 3 * n;
+            pf[ip[h]] = Fx[h];
+            pf[n + ip[h]] = Fy[h];
+            pf[2 * n + ip[h]] = Fz[h];
+            a.pu[(size_t)bc * n + ip[h]] = U[h];
+        }
+    }
+    for (int h = 0; h < (dblk ? kBT / kW : 0); ++h) {
         const int ti = wave + h * kW;
         const int I = br * kBT + ti;
         if (I >= ntiles) continue;
