@@ -182,11 +182,13 @@ int tcp_batch_get_histograms(tcp_batch* b, int rep, double* rdf, double* vdf, in
  * TCP_NOTEMP); trace (may be NULL) receives [nsteps][nrep][6] doubles. */
 int tcp_batch_md_steps(tcp_batch* b, int nsteps, int thermo, double dt, const double* Te,
                        const double* Ti, double tau, double fric, uint64_t seed, double* trace);
-/* Block shape of the persistent pair kernel (TCP_MIXED: 0..3 = 16x16 tiles by
- * 8 waves, 8x8 by 8, 8x8 by 4, 4x4 by 4; TCP_FP64: 0 only). */
+/* Block shape of the persistent pair kernel (TCP_MIXED: 0..5 = 8x8 tiles by
+ * 4 waves (the single-context kernel's), 8x8 by 8, 16x16 by 8, 4x4 by 4,
+ * 16x16 by 4, 8x8 by 2, a wave taking tiles/waves i tiles at a time;
+ * TCP_FP64: 0 only). */
 int tcp_batch_set_shape(tcp_batch* b, int shape);
 /* Times `reps` pair launches per shape on the current positions, keeps the
- * fastest; *best = its index, ms[4] (may be NULL) = time per launch and shape. */
+ * fastest; *best = its index, ms[6] (may be NULL) = time per launch and shape. */
 int tcp_batch_autotune(tcp_batch* b, int reps, int* best, double* ms);
 /* pair_ms: best autotuned pair launch; steps_ms: last tcp_batch_md_steps call. */
 int tcp_batch_last_timings(tcp_batch* b, double* pair_ms, double* steps_ms);

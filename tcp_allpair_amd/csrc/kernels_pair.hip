@@ -519,9 +519,10 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
 // ------------------------------------------- MIXED, Newton's third law ---
 //
 // Self interaction of one particle set, each unordered pair once.  Particles
-// form tiles of 64 (one wavefront); a block owns a 16 x 16 square of tile
-// pairs (I, J) of the upper triangle (blockmap) with 8 wavefronts, each taking
-// two I tiles.  For a tile pair the wavefront holds the 64 i in registers and
+// form tiles of 64 (one wavefront); a block owns a kBlkTiles x kBlkTiles square
+// of tile pairs (I, J) of the upper triangle (blockmap) with kN3Waves
+// wavefronts, each taking two I tiles (8 x 8 by 4 waves: 32896 blocks at
+// N = 131072, a short tail on 256 CUs).  For a tile pair the wavefront holds the 64 i in registers and
 // the 64 j (FixPos, species and live flags) plus their reaction sums in
 // registers as well, and rotates the j data and reactions one lane per step
 // with DPP (wave_ror:1): after 64 steps every i has met every j once and each
@@ -535,8 +536,8 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
 // order is the full force and energy.
 
 constexpr int kTile = 64;
-constexpr int kBlkTiles = 16;
-constexpr int kN3Waves = 8;
+constexpr int kBlkTiles = 8;
+constexpr int kN3Waves = 4;
 
 // wave_ror:1 -- every lane takes the value of the lane below it (lane 0 that of lane 63)
 __device__ inline uint32_t rot_u(uint32_t v)
@@ -849,26 +850,6 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
     }
 }
 
-template <bool kRdf, bool kKelbg>
-__global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArgs a,
-                                                                         const int2* blkmap)
-{
-    __shared__ FixPos jpos[kBlkTiles * kTile];
-    __shared__ double racc[4][kBlkTiles * kTile];
-    __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
-    const int2 rc = blkmap[blockIdx.x];
-    n3_block<kRdf, kKelbg, kBlkTiles, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist);
-}
-
-// ------------------------------------------------ persistent (batches) ---
-//
-// A fixed grid of resident blocks takes work items (replica, block pair) from
-// a device queue in list order (the host sorts the list by decreasing cost),
-// so replicas of very different sizes share one launch.  The item body is the
-// single-context kernel's, so a replica's partial slots hold exactly what its
-// own launch would write.  The last block to finish resets the queue, which
-// keeps the launch free of memset nodes when it is captured in a graph.
-
 __device__ inline int queue_next(int* queue, int* slot)
 {
     __syncthreads();  // every thread has read the previous item, LDS is free again
@@ -887,6 +868,35 @@ __device__ inline void queue_done(int* queue)
         }
     }
 }
+
+// Persistent: resident blocks take the block pairs of blkmap (full blocks
+// first, the half-cost diagonal ones last) from a device queue.
+template <bool kRdf, bool kKelbg>
+__global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArgs a,
+                                                                         const int2* blkmap,
+                                                                         int nblk, int* queue)
+{
+    __shared__ FixPos jpos[kBlkTiles * kTile];
+    __shared__ double racc[4][kBlkTiles * kTile];
+    __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
+    __shared__ int slot;
+    for (;;) {
+        const int it = queue_next(queue, &slot);
+        if (it >= nblk) break;
+        const int2 rc = blkmap[it];
+        n3_block<kRdf, kKelbg, kBlkTiles, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist);
+    }
+    queue_done(queue);
+}
+
+// ------------------------------------------------ persistent (batches) ---
+//
+// A fixed grid of resident blocks takes work items (replica, block pair) from
+// a device queue in list order (the host sorts the list by decreasing cost),
+// so replicas of very different sizes share one launch.  The item body is the
+// single-context kernel's, so a replica's partial slots hold exactly what its
+// own launch would write.  The last block to finish resets the queue, which
+// keeps the launch free of memset nodes when it is captured in a graph.
 
 template <bool kRdf, bool kKelbg, int kBT, int kW>
 __global__ __launch_bounds__(kW * kTile) void pair_mixed_n3_batch_kernel(const BatchRep* reps,
@@ -952,16 +962,28 @@ int n3_block_rows(int n)
     return (ntiles + kBlkTiles - 1) / kBlkTiles;
 }
 
-void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, hipStream_t s)
+int pair_n3_occupancy(bool rdf, bool kelbg)
 {
-    const dim3 g(nblk), b(kN3Waves * kTile);
+    const void* f = rdf ? (kelbg ? (const void*)pair_mixed_n3_kernel<true, true>
+                                 : (const void*)pair_mixed_n3_kernel<true, false>)
+                        : (kelbg ? (const void*)pair_mixed_n3_kernel<false, true>
+                                 : (const void*)pair_mixed_n3_kernel<false, false>);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kN3Waves * kTile, 0) != hipSuccess) nb = 1;
+    return nb < 1 ? 1 : nb;
+}
+
+void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
+                          hipStream_t s)
+{
+    const dim3 g(grid), b(kN3Waves * kTile);
     const bool kel = (a.flags & TCP_KELBG) != 0;
     if (a.rdf_weight) {
-        if (kel) hipLaunchKernelGGL((pair_mixed_n3_kernel<true, true>), g, b, 0, s, a, blkmap);
-        else hipLaunchKernelGGL((pair_mixed_n3_kernel<true, false>), g, b, 0, s, a, blkmap);
+        if (kel) hipLaunchKernelGGL((pair_mixed_n3_kernel<true, true>), g, b, 0, s, a, blkmap, nblk, queue);
+        else hipLaunchKernelGGL((pair_mixed_n3_kernel<true, false>), g, b, 0, s, a, blkmap, nblk, queue);
     } else {
-        if (kel) hipLaunchKernelGGL((pair_mixed_n3_kernel<false, true>), g, b, 0, s, a, blkmap);
-        else hipLaunchKernelGGL((pair_mixed_n3_kernel<false, false>), g, b, 0, s, a, blkmap);
+        if (kel) hipLaunchKernelGGL((pair_mixed_n3_kernel<false, true>), g, b, 0, s, a, blkmap, nblk, queue);
+        else hipLaunchKernelGGL((pair_mixed_n3_kernel<false, false>), g, b, 0, s, a, blkmap, nblk, queue);
     }
 }
 
@@ -980,7 +1002,8 @@ void launch_pair_mixed(const PairArgs& a, int nchunk, hipStream_t s)
 
 N3Shape n3_shape(int shape)
 {
-    static const N3Shape shapes[kNumN3Shapes] = {{kBlkTiles, kN3Waves}, {8, 8}, {8, 4}, {4, 4}};
+    static const N3Shape shapes[kNumN3Shapes] = {{kBlkTiles, kN3Waves}, {8, 8}, {16, 8}, {4, 4},
+                                                 {16, 4}, {8, 2}};
     return shapes[shape < 0 || shape >= kNumN3Shapes ? 0 : shape];
 }
 
@@ -998,8 +1021,10 @@ const void* n3_batch_fn(int shape)
 {
     switch (shape) {
         case 1: return (const void*)pair_mixed_n3_batch_kernel<kRdf, kKelbg, 8, 8>;
-        case 2: return (const void*)pair_mixed_n3_batch_kernel<kRdf, kKelbg, 8, 4>;
+        case 2: return (const void*)pair_mixed_n3_batch_kernel<kRdf, kKelbg, 16, 8>;
         case 3: return (const void*)pair_mixed_n3_batch_kernel<kRdf, kKelbg, 4, 4>;
+        case 4: return (const void*)pair_mixed_n3_batch_kernel<kRdf, kKelbg, 16, 4>;
+        case 5: return (const void*)pair_mixed_n3_batch_kernel<kRdf, kKelbg, 8, 2>;
         default: return (const void*)pair_mixed_n3_batch_kernel<kRdf, kKelbg, kBlkTiles, kN3Waves>;
     }
 }

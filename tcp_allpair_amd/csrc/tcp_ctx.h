@@ -27,6 +27,8 @@ struct tcp_ctx {
     int nchunk = 1, jchunk = 1;
     int n3_rows = 0, n3_blocks = 0;  // Newton's-third-law kernel: block-row slots and blocks
     int2* blkmap = nullptr;
+    int* n3_queue = nullptr;  // work queue of the persistent Newton kernel
+    int n3_grid = 1;          // its resident blocks
     size_t pf_cap = 0;  // doubles
     double *pf = nullptr, *pu = nullptr;
     // reciprocal part

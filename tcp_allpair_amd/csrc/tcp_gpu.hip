@@ -177,7 +177,7 @@ int direct_all(tcp_ctx* c, int rdf_weight, const double* frx, const double* fry,
     if (ev_slot >= 0) HIPCK(c, hipEventRecord(c->ev_pair[0][ev_slot], c->stream));
     int slots = c->nchunk;
     if (c->precision == TCP_MIXED) {
-        launch_pair_mixed_n3(a, c->blkmap, c->n3_blocks, c->stream);
+        launch_pair_mixed_n3(a, c->blkmap, c->n3_blocks, c->n3_queue, c->n3_grid, c->stream);
         slots = c->n3_rows;
     } else {
         launch_pair_f64(a, c->nchunk, c->stream);
@@ -483,6 +483,13 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
         if ((rc = dalloc(c, &c->blkmap, map.size()))) return bail(rc);
         if (hipMemcpy(c->blkmap, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(c, TCP_ERR_HIP, "hipMemcpy of the block map failed"));
+        if ((rc = dalloc(c, &c->n3_queue, (size_t)2))) return bail(rc);
+        if (hipMemset(c->n3_queue, 0, 2 * sizeof(int)) != hipSuccess)
+            return bail(fail(c, TCP_ERR_HIP, "hipMemset of the work queue failed"));
+        int cus = 1;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
+            return bail(fail(c, TCP_ERR_HIP, "no multiprocessor count"));
+        c->n3_grid = cus * pair_n3_occupancy(true, (p->flags & TCP_KELBG) != 0);
         if (c->n3_rows > slots) slots = c->n3_rows;
     }
     c->pf_cap = (size_t)slots * 3 * n;

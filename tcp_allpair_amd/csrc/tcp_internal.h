@@ -59,7 +59,11 @@ void launch_pair_mixed(const PairArgs& a, int nchunk, hipStream_t s);
 // of partials, pf[n3_block_rows(n)][3][n] and pu[n3_block_rows(n)][n];
 // blkmap lists the (row, column) block pairs with row <= column.
 int n3_block_rows(int n);
-void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, hipStream_t s);
+// persistent launch of grid resident blocks over the nblk block pairs of blkmap;
+// queue[2] is zero before the first launch and left zero by every launch
+int pair_n3_occupancy(bool rdf, bool kelbg);
+void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
+                          hipStream_t s);
 
 // SPME / Ewald reciprocal part
 struct RecipArgs {
@@ -218,7 +222,7 @@ void launch_ewald_batch(const BatchRep* reps, int nrep, int kmax, int maxn, hipS
 
 // Pair-kernel shapes of the persistent Newton's-third-law kernel: tiles per
 // block side and wavefronts per block.  Shape 0 is the single-context kernel's.
-constexpr int kNumN3Shapes = 4;
+constexpr int kNumN3Shapes = 6;
 struct N3Shape {
     int tiles, waves;
 };

@@ -245,7 +245,7 @@ class Batch:
     """Independent replicas stepped together (tcp_batch_*): one persistent pair
     launch per step for all replicas, the step captured in a hipGraph."""
 
-    NSHAPES = 4
+    NSHAPES = 6
 
     def __init__(self, params, precision: int = MIXED):
         self.lib = load()

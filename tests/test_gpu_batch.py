@@ -98,7 +98,7 @@ def test_mixed_batch_shape0_bitwise():
     assert_bitwise(replicas(), MIXED, 5, NOTEMP, reduced_dt(0.005), shape=0)
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
 def test_mixed_batch_shapes_vs_fp64(shape):
     say("case batch mixed shape", shape)
     reps = replicas(flags=KELBG)
