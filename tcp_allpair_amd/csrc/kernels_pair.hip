@@ -232,18 +232,37 @@ __global__ __launch_bounds__(256) void adjoint_reduce_kernel(const double* part,
 // ----------------------------------------------------------------- MIXED ---
 //
 // Separations come from FixPos: the wrapped 32-bit difference of the fixed-point
-// coordinates is the minimum-image separation exactly, and the FP32 residuals
-// restore the FP64 position to FP32 rounding, dx = float(int(u_i - u_j)) s +
-// (r_i - r_j): no rint and no cancellation even for pairs 0.01 A apart in a
-// 50 A box.  erfc(x) = exp(-x^2) P(w), w = (x/2)/(1 + x/2), with P of degree 9
-// and P(0) = 1 fitted here (weighted least squares plus Lawson iterations
-// against exp(x^2) erfc(x) on [0, 4.5]; relative error 5.6e-9 in exact
-// arithmetic, 3e-8 near x = 0 and below 7.2e-7 up to x = 3.2 once evaluated in
-// FP32).  exp(-x^2) is shared with the 2x exp(-x^2)/sqrt(pi) force term.
+// coordinates is the minimum-image separation exactly (no rint), dx =
+// float(int(u_i - u_j)) s; for pairs closer than L 2^-8 the FP32 residuals
+// restore the FP64 positions to FP32 rounding, dx = float(int(u_i - u_j)) s +
+// (r_i - r_j), so there is no cancellation even for pairs 0.01 A apart in a
+// 50 A box.  erfc(x) = exp(-x^2) P(w), w = (x/2)/(1 + x/2), with P a
+// polynomial with P(0) = 1 fitted here (weighted least squares plus Lawson
+// iterations against exp(x^2) erfc(x) on [0, 4.5]): degree 7 for the Ewald
+// term (erfcx_poly7), degree 9 (relative error 5.6e-9 in exact arithmetic, 3e-8
+// near x = 0 and below 7.2e-7 up to x = 3.2 once evaluated in FP32) for the
+// Kelbg erfc(r/lambda).  exp(-x^2) is shared with the 2x exp(-x^2)/sqrt(pi)
+// force term.
 // Electron-ion pairs closer than 2 lambda take a rare, divergent branch
 // (close_ei) where the Deutsch and Ewald parts are evaluated without their
 // mutual cancellation.
 // Per-lane sums run in FP32 over tiles of 32 j and go to FP64 after each tile.
+
+// e^{x^2} erfc(x) = P7(w) on the Ewald range: degree 7 with P7(0) = 1, fitted
+// here like erfcx_poly on x in [0, 4.5] (relative error 2.2e-7 in exact
+// arithmetic, 2.3e-7 RMS and 7.5e-7 at most on [0, 3.1] evaluated in FP32).
+__device__ inline float erfcx_poly7(float w)
+{
+    float p = -8.387557e-02f;
+    p = fmaf(p, w, 2.5590158e-01f);
+    p = fmaf(p, w, -1.0481428e-01f);
+    p = fmaf(p, w, -2.7200407e-01f);
+    p = fmaf(p, w, -2.820583e-01f);
+    p = fmaf(p, w, 1.7439e+00f);
+    p = fmaf(p, w, -2.2567792e+00f);
+    p = fmaf(p, w, 1.0f);
+    return p;
+}
 
 __device__ inline float erfcx_poly(float w)
 {
@@ -360,6 +379,7 @@ struct MixConst {
     float nl2rs;               // -log2(e)/lambda
     float rclose;              // 2 lambda (Deutsch) or lambda (Kelbg)
     float inv_dx[3];           // RDF bin widths (reciprocal)
+    float rfix2;               // below this r^2 the FP32 residuals join the separation
     bool kelbg;                // Kelbg instead of Deutsch regularisation
 };
 
@@ -378,6 +398,10 @@ __device__ inline MixConst mix_const(const PairArgs& a)
     k.kelbg = (a.flags & TCP_KELBG) != 0;
     k.rclose = (float)((k.kelbg ? 1.0 : 2.0) * a.rs);
     for (int t = 0; t < 3; ++t) k.inv_dx[t] = (float)a.inv_dx[t];
+    // the fixed-point separation alone is off by at most L 2^-33 per axis; beyond
+    // r = L 2^-8 that is below FP32 rounding of the separation (2^-24 r)
+    const double lmax = fmax(fmax(a.box[0], a.box[1]), a.box[2]);
+    k.rfix2 = (float)(lmax * lmax * 0x1p-16);
     return k;
 }
 
@@ -402,17 +426,23 @@ __device__ inline MixRaw mix_raw(const MixConst& k, const FixPos& pi, const FixP
                                  bool live)
 {
     MixRaw o;
-    o.dx = fmaf((float)(int)(pi.ux - pj.ux), k.sx, pi.rx - pj.rx);
-    o.dy = fmaf((float)(int)(pi.uy - pj.uy), k.sy, pi.ry - pj.ry);
-    o.dz = fmaf((float)(int)(pi.uz - pj.uz), k.sz, pi.rz - pj.rz);
-    const float r2 = fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz));
+    o.dx = (float)(int)(pi.ux - pj.ux) * k.sx;
+    o.dy = (float)(int)(pi.uy - pj.uy) * k.sy;
+    o.dz = (float)(int)(pi.uz - pj.uz) * k.sz;
+    float r2 = fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz));
+    if (r2 < k.rfix2) {  // close pair (rare): add the FP32 residuals
+        o.dx = fmaf((float)(int)(pi.ux - pj.ux), k.sx, pi.rx - pj.rx);
+        o.dy = fmaf((float)(int)(pi.uy - pj.uy), k.sy, pi.ry - pj.ry);
+        o.dz = fmaf((float)(int)(pi.uz - pj.uz), k.sz, pi.rz - pj.rz);
+        r2 = fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz));
+    }
     o.in = kCheck ? (r2 < k.rc2) & (r2 > 0.f) & live : r2 < k.rc2;
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
     const float e2 = __builtin_amdgcn_exp2f(k.na2l * r2);
     const float px = k.half_alpha * r;
     const float t = __builtin_amdgcn_rcpf(px + 1.0f);
-    const float ec = e2 * erfcx_poly(px * t);
+    const float ec = e2 * erfcx_poly7(px * t);
     float num = fmaf(k.two_a_sqrtpi * r, e2, ec);
     float en = ec;
     if (reg) {
