@@ -381,6 +381,7 @@ struct MixConst {
     float inv_dx[3];           // RDF bin widths (reciprocal)
     float rfix2;               // below this r^2 the FP32 residuals join the separation
     bool kelbg;                // Kelbg instead of Deutsch regularisation
+    bool rdf_clamp;            // a pair inside rc can land beyond the padded histogram row
 };
 
 __device__ inline MixConst mix_const(const PairArgs& a)
@@ -402,6 +403,8 @@ __device__ inline MixConst mix_const(const PairArgs& a)
     // r = L 2^-8 that is below FP32 rounding of the separation (2^-24 r)
     const double lmax = fmax(fmax(a.box[0], a.box[1]), a.box[2]);
     k.rfix2 = (float)(lmax * lmax * 0x1p-16);
+    const double idx_max = fmax(fmax(a.inv_dx[0], a.inv_dx[1]), a.inv_dx[2]);
+    k.rdf_clamp = sqrt(a.rc2) * idx_max >= (double)(kHistStride - 2);
     return k;
 }
 
@@ -661,7 +664,7 @@ __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* h
 // The charges are uniform over the tile pair, so the sums run over the
 // charge-free terms num/r^3 and en/r and take eps qi qj (forces, reactions)
 // and qi qj (energy) once at the end.
-template <bool kRdf, bool kKelbg, int kNI>
+template <bool kRdf, bool kKelbg, int kNI, bool kClamp>
 __device__ inline void n3_tile_multi(const MixConst& k, const PairArgs& a, unsigned* hist,
                                      const FixPos (&pi)[kNI], int si, int sj, FixPos pj,
                                      float (&fxo)[kNI], float (&fyo)[kNI], float (&fzo)[kNI],
@@ -690,7 +693,12 @@ __device__ inline void n3_tile_multi(const MixConst& k, const PairArgs& a, unsig
             rx = fmaf(-g, m.dx, rx);
             ry = fmaf(-g, m.dy, ry);
             rz = fmaf(-g, m.dz, rz);
-            if (kRdf && m.in) atomicAdd(&hrow[min((int)(m.r * idx), kHistStride - 1)], 2u);
+            // bins past Ndist+1 land in the padding bins 501..511, never read;
+            // without kClamp every pair inside rc has its bin below 511
+            if (kRdf && m.in) {
+                const int b = (int)(m.r * idx);
+                atomicAdd(&hrow[kClamp ? min(b, kHistStride - 1) : b], 2u);
+            }
         }
         pj.ux = rot_u(pj.ux);
         pj.uy = rot_u(pj.uy);
@@ -768,7 +776,10 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
             const int q = tj * kTile + lane;
             if (all_uni && juni) {
                 float fx[kNI], fy[kNI], fz[kNI], u[kNI], rx, ry, rz;
-                n3_tile_multi<kRdf, kKelbg, kNI>(k, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                if (k.rdf_clamp)
+                    n3_tile_multi<kRdf, kKelbg, kNI, true>(k, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                else
+                    n3_tile_multi<kRdf, kKelbg, kNI, false>(k, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
 #pragma unroll
                 for (int h = 0; h < kNI; ++h) {
                     Fx[h] += (double)fx[h];
