@@ -192,7 +192,7 @@ def main():
         if len(etot) > 1 else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
-                     "traffic": traffic, "kernel": "pair_mixed_n3_kernel<true>",
+                     "traffic": traffic, "kernel": "pair_mixed_n3_kernel<true, false>",
                      "kernel_ms": mx["pair_ms"], "flop_per_launch": flops},
         "cpu_baseline": cpu,
         "batch": batch,

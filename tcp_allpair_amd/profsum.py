@@ -16,7 +16,7 @@ import collections
 import csv
 import sys
 
-PAIR_KERNEL = "pair_mixed_n3_kernel<true>"
+PAIR_KERNEL = "pair_mixed_n3_kernel<true, false>"
 STATS_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_stats -o bench "
              "-- python3 bench.py --steps 5 --warmup 1 --fp64-steps 2 --no-cpu-baseline --batch-steps 0")
 BATCH_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_batch -o batch "
