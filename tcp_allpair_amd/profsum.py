@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 CSV output for profiles/ (text, a few KB).
 
-    python -m tcp_allpair_amd.profsum stats <kernel_stats.csv> <out.md> [note]
+    python -m tcp_allpair_amd.profsum stats <kernel_stats.csv> <out.md> [note] [kernel_trace.csv steps]
     python -m tcp_allpair_amd.profsum batchstats <kernel_stats.csv> <out.md> [note]
     python -m tcp_allpair_amd.profsum pmc <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv> <out.md>
 
@@ -38,7 +38,17 @@ def short(name: str) -> str:
     return name.split("(")[0]
 
 
-def stats(src: str, out: str, note: str = "", kind: str = "step") -> None:
+def timed_average(trace: str, kernel: str, steps: int) -> float:
+    """Average duration (ms) of the last `steps` dispatches of `kernel` in a
+    rocprofv3 kernel trace: the launches of bench.py's timed region."""
+    d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+         for r in csv.DictReader(open(trace)) if short(r["Kernel_Name"]) == kernel]
+    tail = d[-steps:]
+    return sum(tail) / len(tail)
+
+
+def stats(src: str, out: str, note: str = "", kind: str = "step", trace: str = "",
+          steps: int = 0) -> None:
     rows = list(csv.DictReader(open(src)))
     rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
     if kind == "batch":
@@ -58,6 +68,10 @@ def stats(src: str, out: str, note: str = "", kind: str = "step") -> None:
         lines.append("| %s | %s | %.4f | %.2f | %.2f |\n" % (
             short(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e6,
             float(r["TotalDurationNs"]) / 1e6, float(r["Percentage"])))
+    if trace and steps:
+        lines.append("\n%s over the %d timed steps (the last %d dispatches of the kernel trace; the "
+                     "earlier ones run before the clocks settle): %.4f ms per launch.\n"
+                     % (PAIR_KERNEL, steps, steps, timed_average(trace, PAIR_KERNEL, steps)))
     if note:
         lines.append("\n" + note + "\n")
     open(out, "w").write("".join(lines))
@@ -96,6 +110,7 @@ def pmc(fetch: str, write: str, cfetch: str, cwrite: str, out: str) -> None:
 if __name__ == "__main__":
     if sys.argv[1] in ("stats", "batchstats"):
         stats(sys.argv[2], sys.argv[3], sys.argv[4] if len(sys.argv) > 4 else "",
-              "batch" if sys.argv[1] == "batchstats" else "step")
+              "batch" if sys.argv[1] == "batchstats" else "step",
+              sys.argv[5] if len(sys.argv) > 5 else "", int(sys.argv[6]) if len(sys.argv) > 6 else 0)
     else:
         pmc(*sys.argv[2:7])
