@@ -7,10 +7,11 @@
 // launch that takes (replica, block pair) items from a device queue, and the
 // whole step is captured once in a hipGraph and replayed.  The kernel bodies
 // are those of the single-context path with the same block decomposition, so
-// in FP64 every replica reproduces its standalone run bit for bit; in mixed
-// precision the same holds for block shape 0, while the autotuner may pick
-// another tile shape for the persistent kernel (a different, equally valid
-// summation order).
+// in FP64 every replica reproduces its standalone run bit for bit.  In mixed
+// precision the reactions of a block meet in LDS FP64 atomics whose order
+// follows the wave schedule, so a replica agrees with its standalone run to
+// rounding, and the autotuner may pick another tile shape for the persistent
+// kernel (a different, equally valid summation order).
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -112,7 +113,7 @@ bool mixed(const tcp_batch* b) { return b->precision == TCP_MIXED; }
 
 int slots_of(const tcp_batch* b, const tcp_ctx* c, int shape)
 {
-    return mixed(b) ? n3_block_rows_shape(c->N, shape) : c->nchunk;
+    return n3_block_rows_shape(c->N, mixed(b) ? shape : 0);
 }
 
 // Work list of the persistent pair launch, largest items first.
@@ -121,26 +122,16 @@ std::vector<PairItem> make_items(const tcp_batch* b, int shape)
     std::vector<PairItem> items;
     for (int r = 0; r < b->nrep; ++r) {
         const tcp_ctx* c = b->reps[r];
-        if (mixed(b)) {
-            const int bt = n3_shape(shape).tiles;
-            const int ntiles = (c->N + 63) / 64;
-            const int rows = n3_block_rows_shape(c->N, shape);
-            for (int br = 0; br < rows; ++br) {
-                const int ti = std::min(bt, ntiles - br * bt);
-                for (int bc = br; bc < rows; ++bc) {
-                    const int tj = std::min(bt, ntiles - bc * bt);
-                    const int cost = bc == br ? ti * (ti + 1) / 2 : ti * tj;
-                    items.push_back(PairItem{r, br, bc, cost});
-                }
-            }
-        } else {
-            const int ib = (c->N + kPairBlock - 1) / kPairBlock;
-            for (int by = 0; by < c->nchunk; ++by) {
-                const int len = std::min(c->jchunk, c->N - by * c->jchunk);
-                for (int bx = 0; bx < ib; ++bx) {
-                    const int li = std::min(kPairBlock, c->N - bx * kPairBlock);
-                    items.push_back(PairItem{r, bx, by, li * len / 64});
-                }
+        // FP64 runs the 8 x 8 tile blocks of shape 0
+        const int bt = n3_shape(mixed(b) ? shape : 0).tiles;
+        const int ntiles = (c->N + 63) / 64;
+        const int rows = n3_block_rows_shape(c->N, mixed(b) ? shape : 0);
+        for (int br = 0; br < rows; ++br) {
+            const int ti = std::min(bt, ntiles - br * bt);
+            for (int bc = br; bc < rows; ++bc) {
+                const int tj = std::min(bt, ntiles - bc * bt);
+                const int cost = bc == br ? ti * (ti + 1) / 2 : ti * tj;
+                items.push_back(PairItem{r, br, bc, cost});
             }
         }
     }
@@ -326,7 +317,7 @@ int tcp_batch_create(tcp_batch** out, int nrep, const tcp_params* p, int precisi
         h.recip = c->p.recip;
         h.kmax = c->p.kmax;
         h.alpha = c->alpha;
-        h.ur_scale = mixed(b) ? 1.0 : 0.5;
+        h.ur_scale = 1.0;
         h.Uo = c->Uo;
         h.eps = c->p.eps;
         const double V = c->p.box[0] * c->p.box[1] * c->p.box[2];

@@ -910,6 +910,239 @@ __device__ inline void queue_done(int* queue)
     }
 }
 
+// ------------------------------------------- FP64, Newton's third law ---
+//
+// The same tiling as the mixed kernel (tiles of 64, 8 x 8 block pairs of the
+// upper triangle, j data and reactions rotated by DPP), one i tile per wave
+// (8 waves), with the pair arithmetic of pair_f64_kernel: the reference's
+// minimum image and r^2 without contraction, erfc/exp of the device library,
+// so each pair's force is the reference's (force.f90:87-178), summed in a
+// different order.  Reactions of the waves meet through a per-wave LDS buffer
+// summed in wave order after each j tile, the i sums of a diagonal block join
+// once every j tile is done, so the result does not depend on scheduling.
+
+constexpr int kF64Waves = 8;
+
+__device__ inline double rot_d(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned long long lo = rot_u((uint32_t)b), hi = rot_u((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)((hi << 32) | lo));
+}
+
+// One tile pair: lane i against the 64 j of the rotating registers.  fj holds
+// the j species (bit 0) and liveness (bit 1).  kDiag: I == J, ordered pairs for
+// the i side only, half energy, RDF weight 1.
+template <bool kRdf, bool kDiag>
+__device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, double xi, double yi,
+                                   double zi, int si, bool ilive, double xj, double yj, double zj,
+                                   int fj, double& fxo, double& fyo, double& fzo, double& uo,
+                                   double& rx, double& ry, double& rz)
+{
+    const double Lx = a.box[0], Ly = a.box[1], Lz = a.box[2];
+    const double iLx = a.inv_box[0], iLy = a.inv_box[1], iLz = a.inv_box[2];
+    const double alpha = a.alpha, a2 = a.alpha * a.alpha, rc2 = a.rc2;
+    const double two_a_sqrtpi = 2.0 * a.alpha / kSqrtPi;
+    const double inv_rs = 1.0 / a.rs;
+    const bool kelbg = (a.flags & TCP_KELBG) != 0;
+    double fx = 0.0, fy = 0.0, fz = 0.0, u = 0.0;
+    rx = ry = rz = 0.0;
+    for (int s = 0; s < kTile; ++s) {
+        const int sj = fj & 1;
+        double xr, yr, zr, r2;
+        {
+#pragma clang fp contract(off)
+            const double dx = xi - xj;
+            const double dy = yi - yj;
+            const double dz = zi - zj;
+            xr = dx - Lx * rint(dx * iLx);
+            yr = dy - Ly * rint(dy * iLy);
+            zr = dz - Lz * rint(dz * iLz);
+            r2 = xr * xr + yr * yr + zr * zr;
+        }
+        if (r2 < rc2 && r2 > 0.0 && ilive && (fj & 2) != 0) {
+            const double r = sqrt(r2);
+            const int ht = si + sj;
+            if (kRdf) {
+                const int b = (int)(r / a.dx[ht]);
+                if (b < kNbin) atomicAdd(&hist[ht * kHistStride + b], kDiag ? 1u : 2u);
+            }
+            const double qq = a.q[si] * a.q[sj];
+            const double eqq = a.eps * qq;
+            const bool reg = si != sj || (si == 0 && sj == 0 && (a.flags & TCP_REG_EE));
+            const double inv_r = 1.0 / r;
+            const double inv_r2 = inv_r * inv_r;
+            const double ec = erfc(alpha * r);
+            const double g = two_a_sqrtpi * r * exp(-(a2 * r2));
+            double num = g + ec, en = ec;
+            if (reg) {
+                const double sr = r * inv_rs;
+                if (kelbg) {
+                    const double y2 = exp(-(sr * sr));
+                    num -= y2;
+                    en += kSqrtPi * sr * erfc(sr) - y2;
+                } else {
+                    const double y = exp(-sr);
+                    num -= y * (1.0 + sr);
+                    en -= y;
+                }
+            }
+            const double df = eqq * num * inv_r2 * inv_r;
+            u += (kDiag ? 0.5 : 1.0) * (qq * en * inv_r);
+            fx += df * xr;
+            fy += df * yr;
+            fz += df * zr;
+            if (!kDiag) {
+                rx -= df * xr;
+                ry -= df * yr;
+                rz -= df * zr;
+            }
+        }
+        xj = rot_d(xj);
+        yj = rot_d(yj);
+        zj = rot_d(zj);
+        fj = (int)rot_u((uint32_t)fj);
+        if (!kDiag) {
+            rx = rot_d(rx);
+            ry = rot_d(ry);
+            rz = rot_d(rz);
+        }
+    }
+    fxo = fx;
+    fyo = fy;
+    fzo = fz;
+    uo = u;
+}
+
+// LDS of one FP64 block: j positions and flags, the block's reaction sums
+// (slot br), the per-wave reaction buffer and the RDF histogram.
+struct F64Lds {
+    double jx[kBlkTiles * kTile], jy[kBlkTiles * kTile], jz[kBlkTiles * kTile];
+    int jf[kBlkTiles * kTile];
+    double racc[4][kBlkTiles * kTile];
+    double wbuf[kF64Waves][3][kTile];
+    unsigned hist[3 * kHistStride];
+};
+
+template <bool kRdf>
+__device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L)
+{
+    constexpr int kBP = kBlkTiles * kTile;
+    const bool dblk = br == bc;
+    const int n = a.n_i, ne = a.ne_i;
+    for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
+        const int jp = bc * kBP + p;
+        const int jc = min(jp, n - 1);
+        L.jx[p] = a.pi.x[jc];
+        L.jy[p] = a.pi.y[jc];
+        L.jz[p] = a.pi.z[jc];
+        L.jf[p] = (jp >= ne ? 1 : 0) | (jp < n ? 2 : 0);
+        L.racc[0][p] = L.racc[1][p] = L.racc[2][p] = L.racc[3][p] = 0.0;
+    }
+    if (kRdf) hist_clear(L.hist);
+    __syncthreads();
+    const int wave = threadIdx.x / kTile, lane = threadIdx.x % kTile;
+    const int ntiles = (n + kTile - 1) / kTile;
+    const int I = br * kBlkTiles + wave;
+    const bool exists = I < ntiles;
+    const int ip = I * kTile + lane;
+    const bool ilive = exists && ip < n;
+    const int ic = min(ip, n - 1);
+    const double xi = a.pi.x[ic], yi = a.pi.y[ic], zi = a.pi.z[ic];
+    const int si = ip >= ne;
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0, U = 0.0;
+    for (int tj = 0; tj < kBlkTiles; ++tj) {
+        const int J = bc * kBlkTiles + tj;
+        double rx = 0.0, ry = 0.0, rz = 0.0;
+        if (exists && J < ntiles && (!dblk || tj >= wave)) {
+            const int q = tj * kTile + lane;
+            double fx, fy, fz, u;
+            if (dblk && tj == wave)
+                n3_tile_f64<kRdf, true>(a, L.hist, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
+                                        L.jf[q], fx, fy, fz, u, rx, ry, rz);
+            else
+                n3_tile_f64<kRdf, false>(a, L.hist, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
+                                         L.jf[q], fx, fy, fz, u, rx, ry, rz);
+            Fx += fx;
+            Fy += fy;
+            Fz += fz;
+            U += u;
+        }
+        L.wbuf[wave][0][lane] = rx;
+        L.wbuf[wave][1][lane] = ry;
+        L.wbuf[wave][2][lane] = rz;
+        __syncthreads();
+        for (int t = threadIdx.x; t < 3 * kTile; t += blockDim.x) {
+            const int c = t / kTile, p = t % kTile;
+            double s = L.racc[c][tj * kTile + p];
+            for (int w = 0; w < kF64Waves; ++w) s += L.wbuf[w][c][p];
+            L.racc[c][tj * kTile + p] = s;
+        }
+        __syncthreads();
+    }
+    if (dblk) {
+        if (exists) {
+            const int q = wave * kTile + lane;
+            L.racc[0][q] += Fx;
+            L.racc[1][q] += Fy;
+            L.racc[2][q] += Fz;
+            L.racc[3][q] += U;
+        }
+    } else if (ilive) {
+        double* pf = a.pf + (size_t)bc * 3 * n;
+        pf[ip] = Fx;
+        pf[n + ip] = Fy;
+        pf[2 * n + ip] = Fz;
+        a.pu[(size_t)bc * n + ip] = U;
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
+        const int jp = bc * kBP + p;
+        if (jp >= n) continue;
+        double* pf = a.pf + (size_t)br * 3 * n;
+        pf[jp] = L.racc[0][p];
+        pf[n + jp] = L.racc[1][p];
+        pf[2 * n + jp] = L.racc[2][p];
+        a.pu[(size_t)br * n + jp] = L.racc[3][p];
+    }
+    if (kRdf) {
+        __syncthreads();
+        hist_flush(L.hist, a.rdf, 1u);
+    }
+}
+
+template <bool kRdf>
+__global__ __launch_bounds__(kF64Waves * kTile) void pair_f64_n3_kernel(PairArgs a,
+                                                                        const int2* blkmap,
+                                                                        int nblk, int* queue)
+{
+    __shared__ F64Lds lds;
+    __shared__ int slot;
+    for (;;) {
+        const int it = queue_next(queue, &slot);
+        if (it >= nblk) break;
+        const int2 rc = blkmap[it];
+        n3_block_f64<kRdf>(a, rc.x, rc.y, lds);
+    }
+    queue_done(queue);
+}
+
+template <bool kRdf>
+__global__ __launch_bounds__(kF64Waves * kTile) void pair_f64_n3_batch_kernel(const BatchRep* reps,
+                                                                              const PairItem* items,
+                                                                              int nitems, int* queue)
+{
+    __shared__ F64Lds lds;
+    __shared__ int slot;
+    for (;;) {
+        const int it = queue_next(queue, &slot);
+        if (it >= nitems) break;
+        const PairItem w = items[it];
+        n3_block_f64<kRdf>(reps[w.rep].pa, w.a, w.b, lds);
+    }
+    queue_done(queue);
+}
+
 // Persistent: resident blocks take the block pairs of blkmap (full blocks
 // first, the half-cost diagonal ones last) from a device queue.
 template <bool kRdf, bool kKelbg>
@@ -957,22 +1190,6 @@ __global__ __launch_bounds__(kW * kTile) void pair_mixed_n3_batch_kernel(const B
     queue_done(queue);
 }
 
-template <bool kRdf>
-__global__ __launch_bounds__(kPairBlock) void pair_f64_batch_kernel(const BatchRep* reps,
-                                                                    const PairItem* items,
-                                                                    int nitems, int* queue)
-{
-    __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
-    __shared__ int slot;
-    for (;;) {
-        const int it = queue_next(queue, &slot);
-        if (it >= nitems) break;
-        const PairItem w = items[it];
-        f64_block<kRdf>(reps[w.rep].pa, w.a, w.b, hist);
-    }
-    queue_done(queue);
-}
-
 }  // namespace
 
 void launch_pair_f64(const PairArgs& a, int nchunk, hipStream_t s)
@@ -1012,6 +1229,24 @@ int pair_n3_occupancy(bool rdf, bool kelbg)
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kN3Waves * kTile, 0) != hipSuccess) nb = 1;
     return nb < 1 ? 1 : nb;
+}
+
+int pair_f64_n3_occupancy(bool rdf)
+{
+    const void* f = rdf ? (const void*)pair_f64_n3_kernel<true> : (const void*)pair_f64_n3_kernel<false>;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kF64Waves * kTile, 0) != hipSuccess) nb = 1;
+    return nb < 1 ? 1 : nb;
+}
+
+void launch_pair_f64_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
+                        hipStream_t s)
+{
+    const dim3 g(grid), b(kF64Waves * kTile);
+    if (a.rdf_weight)
+        hipLaunchKernelGGL(pair_f64_n3_kernel<true>, g, b, 0, s, a, blkmap, nblk, queue);
+    else
+        hipLaunchKernelGGL(pair_f64_n3_kernel<false>, g, b, 0, s, a, blkmap, nblk, queue);
 }
 
 void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
@@ -1090,12 +1325,11 @@ void launch_pair_mixed_n3_batch(const BatchRep* reps, const PairItem* items, int
 void launch_pair_f64_batch(const BatchRep* reps, const PairItem* items, int nitems, int* queue,
                            bool rdf, int grid, hipStream_t s)
 {
+    const dim3 g(grid), b(kF64Waves * kTile);
     if (rdf)
-        hipLaunchKernelGGL(pair_f64_batch_kernel<true>, dim3(grid), dim3(kPairBlock), 0, s, reps,
-                           items, nitems, queue);
+        hipLaunchKernelGGL(pair_f64_n3_batch_kernel<true>, g, b, 0, s, reps, items, nitems, queue);
     else
-        hipLaunchKernelGGL(pair_f64_batch_kernel<false>, dim3(grid), dim3(kPairBlock), 0, s, reps,
-                           items, nitems, queue);
+        hipLaunchKernelGGL(pair_f64_n3_batch_kernel<false>, g, b, 0, s, reps, items, nitems, queue);
 }
 
 int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg)
@@ -1107,9 +1341,10 @@ int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg)
             hipSuccess)
             nb = 1;
     } else {
-        const void* f = rdf ? (const void*)pair_f64_batch_kernel<true>
-                            : (const void*)pair_f64_batch_kernel<false>;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kPairBlock, 0) != hipSuccess) nb = 1;
+        const void* f = rdf ? (const void*)pair_f64_n3_batch_kernel<true>
+                            : (const void*)pair_f64_n3_batch_kernel<false>;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kF64Waves * kTile, 0) != hipSuccess)
+            nb = 1;
     }
     return nb < 1 ? 1 : nb;
 }

@@ -175,18 +175,15 @@ int direct_all(tcp_ctx* c, int rdf_weight, const double* frx, const double* fry,
     a.pf = c->pf;
     a.pu = c->pu;
     if (ev_slot >= 0) HIPCK(c, hipEventRecord(c->ev_pair[0][ev_slot], c->stream));
-    int slots = c->nchunk;
-    if (c->precision == TCP_MIXED) {
+    if (c->precision == TCP_MIXED)
         launch_pair_mixed_n3(a, c->blkmap, c->n3_blocks, c->n3_queue, c->n3_grid, c->stream);
-        slots = c->n3_rows;
-    } else {
-        launch_pair_f64(a, c->nchunk, c->stream);
-    }
+    else
+        launch_pair_f64_n3(a, c->blkmap, c->n3_blocks, c->n3_queue, c->n3_grid, c->stream);
     LAUNCHCK(c);
     if (ev_slot >= 0) HIPCK(c, hipEventRecord(c->ev_pair[1][ev_slot], c->stream));
-    // Ur = sum u / 2 for the ordered-pair FP64 kernel, sum u for the Newton kernel
-    c->ur_scale = c->precision == TCP_MIXED ? 1.0 : 0.5;
-    launch_combine(md_args(c), c->pf, c->pu, slots, frx, fry, frz, c->part, false, c->stream);
+    // the Newton kernels give each unordered pair's energy once: Ur = sum u
+    c->ur_scale = 1.0;
+    launch_combine(md_args(c), c->pf, c->pu, c->n3_rows, frx, fry, frz, c->part, false, c->stream);
     LAUNCHCK(c);
     launch_reduce_parts(c->part, md_blocks(c->N), 4, c->sc + SC_FSUM, c->stream);
     LAUNCHCK(c);
@@ -472,7 +469,7 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
     if ((rc = dalloc(c, &c->upos, n))) return bail(rc);
     choose_chunks(c->N, c->N, &c->nchunk, &c->jchunk);
     int slots = c->nchunk;
-    if (precision == TCP_MIXED) {
+    {
         // upper triangle of block pairs, off-diagonal (full) blocks first
         c->n3_rows = n3_block_rows(c->N);
         std::vector<int2> map;
@@ -489,7 +486,8 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
         int cus = 1;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
             return bail(fail(c, TCP_ERR_HIP, "no multiprocessor count"));
-        c->n3_grid = cus * pair_n3_occupancy(true, (p->flags & TCP_KELBG) != 0);
+        c->n3_grid = cus * (precision == TCP_MIXED ? pair_n3_occupancy(true, (p->flags & TCP_KELBG) != 0)
+                                                   : pair_f64_n3_occupancy(true));
         if (c->n3_rows > slots) slots = c->n3_rows;
     }
     c->pf_cap = (size_t)slots * 3 * n;

@@ -55,7 +55,7 @@ struct PairArgs {
 
 void launch_pair_f64(const PairArgs& a, int nchunk, hipStream_t s);
 void launch_pair_mixed(const PairArgs& a, int nchunk, hipStream_t s);
-// Self interaction with Newton's third law (mixed precision): block-row slots
+// Self interaction with Newton's third law (both precisions): block-row slots
 // of partials, pf[n3_block_rows(n)][3][n] and pu[n3_block_rows(n)][n];
 // blkmap lists the (row, column) block pairs with row <= column.
 int n3_block_rows(int n);
@@ -64,6 +64,9 @@ int n3_block_rows(int n);
 int pair_n3_occupancy(bool rdf, bool kelbg);
 void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
                           hipStream_t s);
+int pair_f64_n3_occupancy(bool rdf);
+void launch_pair_f64_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
+                        hipStream_t s);
 
 // SPME / Ewald reciprocal part
 struct RecipArgs {
