@@ -166,8 +166,8 @@ void launch_pair(tcp_batch* b)
         launch_pair_mixed_n3_batch(b->d_reps, b->d_items, b->nitems, b->d_queue, b->shape, true,
                                    b->kelbg, b->grid, b->stream);
     else
-        launch_pair_f64_batch(b->d_reps, b->d_items, b->nitems, b->d_queue, true, b->grid,
-                              b->stream);
+        launch_pair_f64_batch(b->d_reps, b->d_items, b->nitems, b->d_queue, true, b->kelbg,
+                              b->grid, b->stream);
 }
 
 // One MD step of every replica, in the order of tcp_md_steps (main.f90:87-153).

@@ -12,6 +12,7 @@
 // atomics and the per-(chunk,i) partials are summed in a fixed order later:
 // FP64 results do not depend on scheduling.  RDF counts go to a per-block LDS
 // histogram (3 x 512 uint32) flushed with one global 64-bit atomic per bin.
+#include "erfcx_table.h"
 #include "tcp_internal.h"
 
 namespace tcp {
@@ -930,23 +931,70 @@ __device__ inline double rot_d(double v)
     return __longlong_as_double((long long)((hi << 32) | lo));
 }
 
+// erfcx(x) = exp(x^2) erfc(x), x >= 0, from the LDS copy of kErfcxTab: the
+// interval holding x (width 1/8 below 8, 1/2 from 8 to 28, the last one past
+// 28) and the degree-9 polynomial in t = (x - midpoint) / half-width.
+__device__ inline double erfcx_tab(double x, const double* tab)
+{
+    const bool lo = x < 8.0;
+    const double u = lo ? x * 16.0 : (x - 8.0) * 4.0;  // (x - start) / half-width
+    const int k = min((int)(u * 0.5), lo ? kErfcxLo - 1 : kErfcxHi - 1);
+    const double t = fmin(u - (double)(2 * k + 1), 1.0);
+    const double* c = tab + (k + (lo ? 0 : kErfcxLo)) * (kErfcxDeg + 1);
+    double p = c[kErfcxDeg];
+#pragma unroll
+    for (int j = kErfcxDeg - 1; j >= 0; --j) p = fma(p, t, c[j]);
+    return p;
+}
+
+// exp(-x^2) with x^2 carried as hi + lo, so the result keeps the accuracy of
+// exp (the rounding of x^2 would cost up to 2 x^2 ulp)
+__device__ inline double exp_mx2(double x)
+{
+    const double hi = x * x;
+    const double lo = fma(x, x, -hi);
+    const double e = exp(-hi);
+    return fma(-e, lo, e);
+}
+
+// 1/r to about one ulp: the hardware estimate and two Newton steps
+__device__ inline double rcp_f64(double r)
+{
+    double y = __builtin_amdgcn_rcp(r);
+    double e = fma(-r, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-r, y, 1.0);
+    return fma(y, e, y);
+}
+
 // One tile pair: lane i against the 64 j of the rotating registers.  fj holds
 // the j species (bit 0) and liveness (bit 1).  kDiag: I == J, ordered pairs for
-// the i side only, half energy, RDF weight 1.
-template <bool kRdf, bool kDiag>
-__device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, double xi, double yi,
-                                   double zi, int si, bool ilive, double xj, double yj, double zj,
-                                   int fj, double& fxo, double& fyo, double& fzo, double& uo,
-                                   double& rx, double& ry, double& rz)
+// the i side only, half energy, RDF weight 1.  The cut-off test, r and the RDF
+// bin are the reference's bit for bit (minimum image and r^2 without
+// contraction, correctly rounded sqrt, bin = int(r / dx): r * (1/dx) decides
+// unless it lies within 1e-12 of a bin edge, where the division does).
+// erfc(x) = exp(-x^2) erfcx(x) shares exp(-x^2) with the Gaussian term.
+template <bool kRdf, bool kKelbg, bool kDiag>
+__device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const double* etab,
+                                   double xi, double yi, double zi, int si, bool ilive, double xj,
+                                   double yj, double zj, int fj, double& fxo, double& fyo,
+                                   double& fzo, double& uo, double& rx, double& ry, double& rz)
 {
     const double Lx = a.box[0], Ly = a.box[1], Lz = a.box[2];
     const double iLx = a.inv_box[0], iLy = a.inv_box[1], iLz = a.inv_box[2];
-    const double alpha = a.alpha, a2 = a.alpha * a.alpha, rc2 = a.rc2;
-    const double two_a_sqrtpi = 2.0 * a.alpha / kSqrtPi;
+    const double alpha = a.alpha, rc2 = a.rc2;
+    const double two_sqrtpi = 2.0 / kSqrtPi;
     const double inv_rs = 1.0 / a.rs;
-    const bool kelbg = (a.flags & TCP_KELBG) != 0;
+    // per-i constants for a j electron (0) / a j ion (1)
+    const double qi = si ? a.q[1] : a.q[0];
+    const double eqi = a.eps * qi;
+    const double qj0 = a.q[0], qj1 = a.q[1];
+    const double dx0 = si ? a.dx[1] : a.dx[0], dx1 = si ? a.dx[2] : a.dx[1];
+    const double idx0 = si ? a.inv_dx[1] : a.inv_dx[0], idx1 = si ? a.inv_dx[2] : a.inv_dx[1];
+    const bool reg0 = si != 0 || (a.flags & TCP_REG_EE) != 0, reg1 = si == 0;
     double fx = 0.0, fy = 0.0, fz = 0.0, u = 0.0;
     rx = ry = rz = 0.0;
+#pragma unroll 1
     for (int s = 0; s < kTile; ++s) {
         const int sj = fj & 1;
         double xr, yr, zr, r2;
@@ -962,32 +1010,35 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, double xi,
         }
         if (r2 < rc2 && r2 > 0.0 && ilive && (fj & 2) != 0) {
             const double r = sqrt(r2);
-            const int ht = si + sj;
             if (kRdf) {
-                const int b = (int)(r / a.dx[ht]);
-                if (b < kNbin) atomicAdd(&hist[ht * kHistStride + b], kDiag ? 1u : 2u);
+                const double q = r * (sj ? idx1 : idx0);
+                int b = (int)q;
+                const double fr = q - (double)b;
+                if (fr < 1e-12 || fr > 1.0 - 1e-12) b = (int)(r / (sj ? dx1 : dx0));
+                if (b < kNbin) atomicAdd(&hist[(si + sj) * kHistStride + b], kDiag ? 1u : 2u);
             }
-            const double qq = a.q[si] * a.q[sj];
-            const double eqq = a.eps * qq;
-            const bool reg = si != sj || (si == 0 && sj == 0 && (a.flags & TCP_REG_EE));
-            const double inv_r = 1.0 / r;
+            const double qq = qi * (sj ? qj1 : qj0);
+            const double inv_r = rcp_f64(r);
             const double inv_r2 = inv_r * inv_r;
-            const double ec = erfc(alpha * r);
-            const double g = two_a_sqrtpi * r * exp(-(a2 * r2));
-            double num = g + ec, en = ec;
-            if (reg) {
+            const double x = alpha * r;
+            const double e = exp_mx2(x);
+            const double ec = e * erfcx_tab(x, etab);
+            double num = fma(two_sqrtpi * x, e, ec), en = ec;
+            if (sj ? reg1 : reg0) {
                 const double sr = r * inv_rs;
-                if (kelbg) {
-                    const double y2 = exp(-(sr * sr));
+                if (kKelbg) {
+                    // R = -e^{-s^2} + sqrt(pi) s erfc(s)
+                    const double y2 = exp_mx2(sr);
                     num -= y2;
-                    en += kSqrtPi * sr * erfc(sr) - y2;
+                    en += y2 * fma(kSqrtPi * sr, erfcx_tab(sr, etab), -1.0);
                 } else {
+                    // R = -e^{-s} (Deutsch, force.f90:136-139)
                     const double y = exp(-sr);
                     num -= y * (1.0 + sr);
                     en -= y;
                 }
             }
-            const double df = eqq * num * inv_r2 * inv_r;
+            const double df = eqi * (sj ? qj1 : qj0) * num * inv_r2 * inv_r;
             u += (kDiag ? 0.5 : 1.0) * (qq * en * inv_r);
             fx += df * xr;
             fy += df * yr;
@@ -1022,9 +1073,16 @@ struct F64Lds {
     double racc[4][kBlkTiles * kTile];
     double wbuf[kF64Waves][3][kTile];
     unsigned hist[3 * kHistStride];
+    double etab[kErfcxN * (kErfcxDeg + 1)] __attribute__((aligned(16)));
 };
 
-template <bool kRdf>
+__device__ inline void load_erfcx(double* etab)
+{
+    const double* src = &kErfcxTab[0][0];
+    for (int i = threadIdx.x; i < kErfcxN * (kErfcxDeg + 1); i += blockDim.x) etab[i] = src[i];
+}
+
+template <bool kRdf, bool kKelbg>
 __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L)
 {
     constexpr int kBP = kBlkTiles * kTile;
@@ -1058,10 +1116,10 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L
             const int q = tj * kTile + lane;
             double fx, fy, fz, u;
             if (dblk && tj == wave)
-                n3_tile_f64<kRdf, true>(a, L.hist, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
+                n3_tile_f64<kRdf, kKelbg, true>(a, L.hist, L.etab, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
                                         L.jf[q], fx, fy, fz, u, rx, ry, rz);
             else
-                n3_tile_f64<kRdf, false>(a, L.hist, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
+                n3_tile_f64<kRdf, kKelbg, false>(a, L.hist, L.etab, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
                                          L.jf[q], fx, fy, fz, u, rx, ry, rz);
             Fx += fx;
             Fy += fy;
@@ -1111,34 +1169,36 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L
     }
 }
 
-template <bool kRdf>
+template <bool kRdf, bool kKelbg>
 __global__ __launch_bounds__(kF64Waves * kTile) void pair_f64_n3_kernel(PairArgs a,
                                                                         const int2* blkmap,
                                                                         int nblk, int* queue)
 {
     __shared__ F64Lds lds;
     __shared__ int slot;
+    load_erfcx(lds.etab);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nblk) break;
         const int2 rc = blkmap[it];
-        n3_block_f64<kRdf>(a, rc.x, rc.y, lds);
+        n3_block_f64<kRdf, kKelbg>(a, rc.x, rc.y, lds);
     }
     queue_done(queue);
 }
 
-template <bool kRdf>
+template <bool kRdf, bool kKelbg>
 __global__ __launch_bounds__(kF64Waves * kTile) void pair_f64_n3_batch_kernel(const BatchRep* reps,
                                                                               const PairItem* items,
                                                                               int nitems, int* queue)
 {
     __shared__ F64Lds lds;
     __shared__ int slot;
+    load_erfcx(lds.etab);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nitems) break;
         const PairItem w = items[it];
-        n3_block_f64<kRdf>(reps[w.rep].pa, w.a, w.b, lds);
+        n3_block_f64<kRdf, kKelbg>(reps[w.rep].pa, w.a, w.b, lds);
     }
     queue_done(queue);
 }
@@ -1231,9 +1291,12 @@ int pair_n3_occupancy(bool rdf, bool kelbg)
     return nb < 1 ? 1 : nb;
 }
 
-int pair_f64_n3_occupancy(bool rdf)
+int pair_f64_n3_occupancy(bool rdf, bool kelbg)
 {
-    const void* f = rdf ? (const void*)pair_f64_n3_kernel<true> : (const void*)pair_f64_n3_kernel<false>;
+    const void* f = rdf ? (kelbg ? (const void*)pair_f64_n3_kernel<true, true>
+                                 : (const void*)pair_f64_n3_kernel<true, false>)
+                        : (kelbg ? (const void*)pair_f64_n3_kernel<false, true>
+                                 : (const void*)pair_f64_n3_kernel<false, false>);
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kF64Waves * kTile, 0) != hipSuccess) nb = 1;
     return nb < 1 ? 1 : nb;
@@ -1243,10 +1306,14 @@ void launch_pair_f64_n3(const PairArgs& a, const int2* blkmap, int nblk, int* qu
                         hipStream_t s)
 {
     const dim3 g(grid), b(kF64Waves * kTile);
-    if (a.rdf_weight)
-        hipLaunchKernelGGL(pair_f64_n3_kernel<true>, g, b, 0, s, a, blkmap, nblk, queue);
-    else
-        hipLaunchKernelGGL(pair_f64_n3_kernel<false>, g, b, 0, s, a, blkmap, nblk, queue);
+    const bool kel = (a.flags & TCP_KELBG) != 0;
+    if (a.rdf_weight) {
+        if (kel) hipLaunchKernelGGL((pair_f64_n3_kernel<true, true>), g, b, 0, s, a, blkmap, nblk, queue);
+        else hipLaunchKernelGGL((pair_f64_n3_kernel<true, false>), g, b, 0, s, a, blkmap, nblk, queue);
+    } else {
+        if (kel) hipLaunchKernelGGL((pair_f64_n3_kernel<false, true>), g, b, 0, s, a, blkmap, nblk, queue);
+        else hipLaunchKernelGGL((pair_f64_n3_kernel<false, false>), g, b, 0, s, a, blkmap, nblk, queue);
+    }
 }
 
 void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
@@ -1323,13 +1390,16 @@ void launch_pair_mixed_n3_batch(const BatchRep* reps, const PairItem* items, int
 }
 
 void launch_pair_f64_batch(const BatchRep* reps, const PairItem* items, int nitems, int* queue,
-                           bool rdf, int grid, hipStream_t s)
+                           bool rdf, bool kelbg, int grid, hipStream_t s)
 {
     const dim3 g(grid), b(kF64Waves * kTile);
-    if (rdf)
-        hipLaunchKernelGGL(pair_f64_n3_batch_kernel<true>, g, b, 0, s, reps, items, nitems, queue);
-    else
-        hipLaunchKernelGGL(pair_f64_n3_batch_kernel<false>, g, b, 0, s, reps, items, nitems, queue);
+    if (rdf) {
+        if (kelbg) hipLaunchKernelGGL((pair_f64_n3_batch_kernel<true, true>), g, b, 0, s, reps, items, nitems, queue);
+        else hipLaunchKernelGGL((pair_f64_n3_batch_kernel<true, false>), g, b, 0, s, reps, items, nitems, queue);
+    } else {
+        if (kelbg) hipLaunchKernelGGL((pair_f64_n3_batch_kernel<false, true>), g, b, 0, s, reps, items, nitems, queue);
+        else hipLaunchKernelGGL((pair_f64_n3_batch_kernel<false, false>), g, b, 0, s, reps, items, nitems, queue);
+    }
 }
 
 int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg)
@@ -1341,8 +1411,10 @@ int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg)
             hipSuccess)
             nb = 1;
     } else {
-        const void* f = rdf ? (const void*)pair_f64_n3_batch_kernel<true>
-                            : (const void*)pair_f64_n3_batch_kernel<false>;
+        const void* f = rdf ? (kelbg ? (const void*)pair_f64_n3_batch_kernel<true, true>
+                                     : (const void*)pair_f64_n3_batch_kernel<true, false>)
+                            : (kelbg ? (const void*)pair_f64_n3_batch_kernel<false, true>
+                                     : (const void*)pair_f64_n3_batch_kernel<false, false>);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kF64Waves * kTile, 0) != hipSuccess)
             nb = 1;
     }

@@ -487,7 +487,7 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
             return bail(fail(c, TCP_ERR_HIP, "no multiprocessor count"));
         c->n3_grid = cus * (precision == TCP_MIXED ? pair_n3_occupancy(true, (p->flags & TCP_KELBG) != 0)
-                                                   : pair_f64_n3_occupancy(true));
+                                                   : pair_f64_n3_occupancy(true, (p->flags & TCP_KELBG) != 0));
         if (c->n3_rows > slots) slots = c->n3_rows;
     }
     c->pf_cap = (size_t)slots * 3 * n;

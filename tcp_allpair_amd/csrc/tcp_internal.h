@@ -64,7 +64,7 @@ int n3_block_rows(int n);
 int pair_n3_occupancy(bool rdf, bool kelbg);
 void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
                           hipStream_t s);
-int pair_f64_n3_occupancy(bool rdf);
+int pair_f64_n3_occupancy(bool rdf, bool kelbg);
 void launch_pair_f64_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
                         hipStream_t s);
 
@@ -237,7 +237,7 @@ void launch_pair_mixed_n3_batch(const BatchRep* reps, const PairItem* items, int
                                 int* queue, int shape, bool rdf, bool kelbg, int grid,
                                 hipStream_t s);
 void launch_pair_f64_batch(const BatchRep* reps, const PairItem* items, int nitems, int* queue,
-                           bool rdf, int grid, hipStream_t s);
+                           bool rdf, bool kelbg, int grid, hipStream_t s);
 // resident blocks per CU of the persistent kernels
 int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg);
 
