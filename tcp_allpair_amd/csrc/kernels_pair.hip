@@ -957,22 +957,24 @@ __device__ inline double exp_mx2(double x)
     return fma(-e, lo, e);
 }
 
-// 1/r to about one ulp: the hardware estimate and two Newton steps
-__device__ inline double rcp_f64(double r)
+// 1/sqrt(r2) to about one ulp: the hardware estimate and two Newton steps
+__device__ inline double rsq_f64(double r2)
 {
-    double y = __builtin_amdgcn_rcp(r);
-    double e = fma(-r, y, 1.0);
+    const double h = 0.5 * r2;
+    double y = __builtin_amdgcn_rsq(r2);
+    double e = fma(-(h * y), y, 0.5);
     y = fma(y, e, y);
-    e = fma(-r, y, 1.0);
+    e = fma(-(h * y), y, 0.5);
     return fma(y, e, y);
 }
 
 // One tile pair: lane i against the 64 j of the rotating registers.  fj holds
 // the j species (bit 0) and liveness (bit 1).  kDiag: I == J, ordered pairs for
-// the i side only, half energy, RDF weight 1.  The cut-off test, r and the RDF
-// bin are the reference's bit for bit (minimum image and r^2 without
-// contraction, correctly rounded sqrt, bin = int(r / dx): r * (1/dx) decides
-// unless it lies within 1e-12 of a bin edge, where the division does).
+// the i side only, half energy, RDF weight 1.  The cut-off test and the RDF bin
+// are the reference's bit for bit: minimum image and r^2 without contraction,
+// and bin = int(sqrt(r2) / dx) with both operations correctly rounded, which
+// r * (1/dx) (r = r2 / sqrt(r2) to an ulp or two) decides unless it lies within
+// 1e-12 of a bin edge; there the exact sqrt and division decide.
 // erfc(x) = exp(-x^2) erfcx(x) shares exp(-x^2) with the Gaussian term.
 template <bool kRdf, bool kKelbg, bool kDiag>
 __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const double* etab,
@@ -1009,16 +1011,16 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
             r2 = xr * xr + yr * yr + zr * zr;
         }
         if (r2 < rc2 && r2 > 0.0 && ilive && (fj & 2) != 0) {
-            const double r = sqrt(r2);
+            const double inv_r = rsq_f64(r2);
+            const double r = r2 * inv_r;
             if (kRdf) {
                 const double q = r * (sj ? idx1 : idx0);
                 int b = (int)q;
                 const double fr = q - (double)b;
-                if (fr < 1e-12 || fr > 1.0 - 1e-12) b = (int)(r / (sj ? dx1 : dx0));
+                if (fr < 1e-12 || fr > 1.0 - 1e-12) b = (int)(sqrt(r2) / (sj ? dx1 : dx0));
                 if (b < kNbin) atomicAdd(&hist[(si + sj) * kHistStride + b], kDiag ? 1u : 2u);
             }
             const double qq = qi * (sj ? qj1 : qj0);
-            const double inv_r = rcp_f64(r);
             const double inv_r2 = inv_r * inv_r;
             const double x = alpha * r;
             const double e = exp_mx2(x);
@@ -1169,8 +1171,9 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L
     }
 }
 
+// four waves per SIMD (at most 128 VGPRs): two resident 8-wave blocks per CU
 template <bool kRdf, bool kKelbg>
-__global__ __launch_bounds__(kF64Waves * kTile) void pair_f64_n3_kernel(PairArgs a,
+__global__ __launch_bounds__(kF64Waves * kTile, 4) void pair_f64_n3_kernel(PairArgs a,
                                                                         const int2* blkmap,
                                                                         int nblk, int* queue)
 {
