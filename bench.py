@@ -12,9 +12,10 @@ and resident in HBM before the timed region; the timed region is K calls of
 the step bracketed by a device synchronisation on both sides.
 
 Prints one JSON line.  value = pair-interactions/s of the mixed-precision path
-(N(N-1)/2 unordered pairs per step x steps/s); the FP64 path, their ratio and
-the RMS relative force error of the mixed path against FP64 are reported
-beside it.  roofline: the pair kernel's algorithmic FP32 flops (68.5 per
+(N(N-1)/2 unordered pairs per step x steps/s); the FP64 path (with its pair
+kernel's rate in the same flop count against the 78.6 TFLOP/s FP64 vector
+peak), their ratio and the RMS relative force error of the mixed path against
+FP64 are reported beside it.  roofline: the pair kernel's algorithmic FP32 flops (68.5 per
 unordered pair: 62.5 per pair plus 6 for the Newton's-third-law reaction) over
 its average launch time from HIP events on its stream, against the
 157.3 TFLOP/s FP32 vector peak.  cpu_baseline: the reference's own
@@ -48,6 +49,7 @@ from tcp_allpair_amd.params import FP64, MIXED, NOTEMP, reduced_dt  # noqa: E402
 # Newton's third law (the reaction accumulate), which the mixed kernel does.
 FLOP_PER_UNORDERED_PAIR_NEWTON = 62.5 + 6.0
 FP32_VECTOR_PEAK_TFLOPS = 157.3
+FP64_VECTOR_PEAK_TFLOPS = 78.6
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pair_mixed_n131072_pmc.md")
 
 
@@ -118,9 +120,9 @@ def cpu_baseline(p, x):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--fp64-steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="steps of the 64-replica batching workload (0 skips it)")
@@ -141,7 +143,7 @@ def main():
     mx = run(p, x, v, f64_0, MIXED, args.steps, args.warmup, dt)
     print(f"[bench] mixed {mx['wall_s'] / args.steps * 1e3:.2f} ms/step; timing fp64", file=sys.stderr,
           flush=True)
-    fp = run(p, x, v, f64_0, FP64, args.fp64_steps, 1, dt)
+    fp = run(p, x, v, f64_0, FP64, args.fp64_steps, 2, dt)
     ms_step = mx["wall_s"] / args.steps * 1e3
     sps = args.steps / mx["wall_s"]
     sps64 = args.fp64_steps / fp["wall_s"]
@@ -185,7 +187,10 @@ def main():
         "pair_interactions_per_s_ordered": 2 * pairs_unordered * sps,
         "fp64": {"steps_per_s": sps64, "pair_interactions_per_s": pairs_unordered * sps64,
                  "ms_per_step": fp["wall_s"] / args.fp64_steps * 1e3,
-                 "pair_kernel_ms": fp["pair_ms"]},
+                 "pair_kernel_ms": fp["pair_ms"],
+                 "pair_kernel_tflops": flops / (fp["pair_ms"] * 1e-3) / 1e12,
+                 "pair_kernel_frac_of_fp64_peak": flops / (fp["pair_ms"] * 1e-3) / 1e12
+                 / FP64_VECTOR_PEAK_TFLOPS},
         "mixed_over_fp64_steps": sps / sps64,
         "force_err_rms_vs_fp64": err,
         "nve_energy_drift_per_step": float(np.polyfit(np.arange(len(etot)), etot, 1)[0] / abs(etot[0]))

@@ -3,12 +3,15 @@
     python -m tcp_allpair_amd.profsum stats <kernel_stats.csv> <out.md> [note] [kernel_trace.csv steps]
     python -m tcp_allpair_amd.profsum batchstats <kernel_stats.csv> <out.md> [note]
     python -m tcp_allpair_amd.profsum pmc <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv> <out.md>
+    python -m tcp_allpair_amd.profsum valu <counter_collection.csv> <out.md>
 
 `stats` tabulates rocprofv3 --stats per kernel.  `pmc` scales FETCH_SIZE and
 WRITE_SIZE (KiB per dispatch) by what the same counters read for copy_probe's
 copy_kernel of known size (512 MiB in, 512 MiB out) and writes the HBM bytes
 per launch of each kernel of the step; the line `hbm_bytes_per_launch:` holds
-the pair kernel's figure that bench.py reports as roofline.traffic.
+the pair kernel's figure that bench.py reports as roofline.traffic.  `valu`
+tabulates the SQ instruction counters of the two pair kernels per launch and
+per pair evaluation.
 """
 from __future__ import annotations
 
@@ -17,20 +20,26 @@ import csv
 import sys
 
 PAIR_KERNEL = "pair_mixed_n3_kernel<true, false>"
+FP64_PAIR_KERNEL = "pair_f64_n3_kernel<true, false>"
+FP64_TIMED_STEPS = 2  # --fp64-steps of STATS_CMD
 STATS_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_stats -o bench "
              "-- python3 bench.py --steps 5 --warmup 1 --fp64-steps 2 --no-cpu-baseline --batch-steps 0")
 BATCH_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_batch -o batch "
              "-- python3 -m tcp_allpair_amd.batchbench --steps 5 --warmup 1")
 PMC_CMDS = [
     "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o bench -- python3 bench.py "
-    "--steps 2 --warmup 0 --fp64-steps 1 --no-cpu-baseline",
+    "--steps 2 --warmup 0 --fp64-steps 1 --batch-steps 0 --no-cpu-baseline",
     "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o bench -- python3 bench.py "
-    "--steps 2 --warmup 0 --fp64-steps 1 --no-cpu-baseline",
+    "--steps 2 --warmup 0 --fp64-steps 1 --batch-steps 0 --no-cpu-baseline",
     "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_copy_fetch -o copy -- "
     "tcp_allpair_amd/copy_probe 512 3",
     "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_copy_write -o copy -- "
     "tcp_allpair_amd/copy_probe 512 3",
 ]
+VALU_CMD = ("rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_SALU SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES "
+            "SQ_INSTS_VALU_TRANS_F32 --output-format csv -d gpurun_out/pmc_valu -o bench -- python3 "
+            "bench.py --steps 2 --warmup 0 --fp64-steps 1 --batch-steps 0 --no-cpu-baseline")
+N_BENCH = 131072
 
 
 def short(name: str) -> str:
@@ -72,6 +81,9 @@ def stats(src: str, out: str, note: str = "", kind: str = "step", trace: str = "
         lines.append("\n%s over the %d timed steps (the last %d dispatches of the kernel trace; the "
                      "earlier ones run before the clocks settle): %.4f ms per launch.\n"
                      % (PAIR_KERNEL, steps, steps, timed_average(trace, PAIR_KERNEL, steps)))
+        lines.append("\n%s over the %d timed FP64 steps: %.4f ms per launch.\n"
+                     % (FP64_PAIR_KERNEL, FP64_TIMED_STEPS,
+                        timed_average(trace, FP64_PAIR_KERNEL, FP64_TIMED_STEPS)))
     if note:
         lines.append("\n" + note + "\n")
     open(out, "w").write("".join(lines))
@@ -107,8 +119,40 @@ def pmc(fetch: str, write: str, cfetch: str, cwrite: str, out: str) -> None:
     open(out, "w").write("".join(lines))
 
 
+def valu(src: str, out: str) -> None:
+    d = collections.defaultdict(lambda: collections.defaultdict(list))
+    dur = collections.defaultdict(list)
+    for r in csv.DictReader(open(src)):
+        k = short(r["Kernel_Name"])
+        if k in (PAIR_KERNEL, FP64_PAIR_KERNEL):
+            d[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+            dur[k].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+    steps = N_BENCH * (N_BENCH - 1) / 2 / 64  # wave-wide pair evaluations (64 lanes)
+    lines = ["# VALU instruction counts of the pair kernels (rocprofv3 --pmc), N = 131072\n\n",
+             "Command (repository root on the GPU box, TMPDIR=/tmp):\n\n    " + VALU_CMD + "\n\n",
+             "Per launch (average over the launches of the run; the counters are summed over the "
+             "chip).  Per pair: SQ_INSTS_VALU / (N(N-1)/2 / 64), the wave instructions one lane "
+             "spends per unordered pair, reactions included.  Cycles per VALU instruction per "
+             "SIMD: launch time x 2.4 GHz x 1024 SIMDs / SQ_INSTS_VALU (the launch times of this "
+             "counter run, nominal clock).\n\n",
+             "| kernel | SQ_INSTS_VALU | of which TRANS_F32 | VALU per pair | SQ_INSTS_SALU | launch ms "
+             "| cycles per VALU per SIMD |\n|---|---|---|---|---|---|---|\n"]
+    for k in (PAIR_KERNEL, FP64_PAIR_KERNEL):
+        if k not in d:
+            continue
+        c = {n: sum(v) / len(v) for n, v in d[k].items()}
+        ms = sorted(dur[k])[len(dur[k]) // 2]
+        v = c.get("SQ_INSTS_VALU", 0.0)
+        lines.append("| %s | %.4e | %.4e | %.1f | %.4e | %.3f | %.2f |\n" % (
+            k, v, c.get("SQ_INSTS_VALU_TRANS_F32", 0.0), v / steps, c.get("SQ_INSTS_SALU", 0.0), ms,
+            ms * 1e-3 * 2.4e9 * 1024 / v if v else 0.0))
+    open(out, "w").write("".join(lines))
+
+
 if __name__ == "__main__":
-    if sys.argv[1] in ("stats", "batchstats"):
+    if sys.argv[1] == "valu":
+        valu(sys.argv[2], sys.argv[3])
+    elif sys.argv[1] in ("stats", "batchstats"):
         stats(sys.argv[2], sys.argv[3], sys.argv[4] if len(sys.argv) > 4 else "",
               "batch" if sys.argv[1] == "batchstats" else "step",
               sys.argv[5] if len(sys.argv) > 5 else "", int(sys.argv[6]) if len(sys.argv) > 6 else 0)
