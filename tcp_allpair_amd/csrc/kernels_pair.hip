@@ -665,17 +665,110 @@ __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* h
 // The charges are uniform over the tile pair, so the sums run over the
 // charge-free terms num/r^3 and en/r and take eps qi qj (forces, reactions)
 // and qi qj (energy) once at the end.
-template <bool kRdf, bool kKelbg, int kNI, bool kClamp>
-__device__ inline void n3_tile_multi(const MixConst& k, const PairArgs& a, unsigned* hist,
+// Constants of the uniform tile path in its working length unit u.  For a
+// cubic box u = L 2^-32: the separation is the fixed-point difference itself
+// converted to float, with no per-axis scaling, and every constant that meets
+// a length carries the matching power of u (the FP32 sums of a tile pair take
+// eps qi qj u^-2 and qi qj u^-1 in FP64 as they join the FP64 accumulators,
+// so no FP32 rounding of those factors is common to all pairs).  Otherwise
+// u = 1 and the
+// separation is scaled per axis as in mix_raw.
+struct MixUnits {
+    float sx, sy, sz;        // fixed-point difference to u per axis (u = 1 only)
+    float inv_u;             // FP32 residuals (physical) to u
+    float rc2, rfix2;        // u^2
+    float half_alpha, two_a_sqrtpi, inv_rs, nl2rs;  // x u
+    float na2l;              // x u^2
+    float rclose;            // / u
+    float inv_dx[3];         // x u
+    double f_scale, e_scale; // u^-2 (num d / r^3), u^-1 (en / r)
+};
+
+__device__ inline MixUnits mix_units(const PairArgs& a, bool cubic)
+{
+    const double u = cubic ? a.box[0] * 0x1p-32 : 1.0;
+    MixUnits w;
+    w.sx = (float)(a.box[0] * 0x1p-32);
+    w.sy = (float)(a.box[1] * 0x1p-32);
+    w.sz = (float)(a.box[2] * 0x1p-32);
+    w.inv_u = (float)(1.0 / u);
+    w.rc2 = (float)(a.rc2 / (u * u));
+    const double lmax = fmax(fmax(a.box[0], a.box[1]), a.box[2]);
+    w.rfix2 = (float)(lmax * lmax * 0x1p-16 / (u * u));
+    w.half_alpha = (float)(0.5 * a.alpha * u);
+    w.two_a_sqrtpi = (float)(2.0 * a.alpha / kSqrtPi * u);
+    w.inv_rs = (float)(u / a.rs);
+    w.nl2rs = (float)(-1.4426950408889634 * u / a.rs);
+    w.na2l = (float)(-a.alpha * a.alpha * 1.4426950408889634 * u * u);
+    const bool kelbg = (a.flags & TCP_KELBG) != 0;
+    w.rclose = (float)((kelbg ? 1.0 : 2.0) * a.rs / u);
+    for (int t = 0; t < 3; ++t) w.inv_dx[t] = (float)(a.inv_dx[t] * u);
+    w.f_scale = 1.0 / (u * u);
+    w.e_scale = 1.0 / u;
+    return w;
+}
+
+// mix_raw for the uniform tile path, in the unit u of MixUnits.
+template <bool kKelbg, bool kCubic>
+__device__ inline MixRaw mix_raw_u(const MixUnits& w, const FixPos& pi, const FixPos& q, bool reg)
+{
+    MixRaw o;
+    const float ix = (float)(int)(pi.ux - q.ux), iy = (float)(int)(pi.uy - q.uy),
+                iz = (float)(int)(pi.uz - q.uz);
+    o.dx = kCubic ? ix : ix * w.sx;
+    o.dy = kCubic ? iy : iy * w.sy;
+    o.dz = kCubic ? iz : iz * w.sz;
+    float r2 = fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz));
+    if (r2 < w.rfix2) {  // close pair (rare): add the FP32 residuals
+        o.dx = kCubic ? fmaf(pi.rx - q.rx, w.inv_u, ix) : fmaf(ix, w.sx, pi.rx - q.rx);
+        o.dy = kCubic ? fmaf(pi.ry - q.ry, w.inv_u, iy) : fmaf(iy, w.sy, pi.ry - q.ry);
+        o.dz = kCubic ? fmaf(pi.rz - q.rz, w.inv_u, iz) : fmaf(iz, w.sz, pi.rz - q.rz);
+        r2 = fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz));
+    }
+    o.in = r2 < w.rc2;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    const float e2 = __builtin_amdgcn_exp2f(w.na2l * r2);
+    const float px = w.half_alpha * r;
+    const float t = __builtin_amdgcn_rcpf(px + 1.0f);
+    const float ec = e2 * erfcx_poly7(px * t);
+    const float g = w.two_a_sqrtpi * r;  // 2 x / sqrt(pi)
+    float num = fmaf(g, e2, ec);
+    float en = ec;
+    if (reg) {
+        const float sr = r * w.inv_rs;
+        if (kKelbg) {
+            const float y2 = __builtin_amdgcn_exp2f(-1.4426950408889634f * sr * sr);
+            const float hs = 0.5f * sr;
+            const float ecs = y2 * erfcx_poly(hs * __builtin_amdgcn_rcpf(hs + 1.0f));  // erfc(s)
+            num -= y2;
+            en = fmaf(kSqrtPiF * sr, ecs, en - y2);
+            if (o.in && r < w.rclose) close_kelbg(sr, 2.0f * px, ecs, g * e2, ec, &num, &en);
+        } else {
+            const float y = __builtin_amdgcn_exp2f(w.nl2rs * r);
+            num = fmaf(-y, sr + 1.0f, num);
+            en -= y;
+            if (o.in && r < w.rclose) close_ei(r, sr, 2.0f * px, y, ec, g * e2, &num, &en);
+        }
+    }
+    o.r = r;
+    o.rinv = rinv;
+    o.num = num;
+    o.en = en;
+    return o;
+}
+
+// kNI i tiles against one j tile, all live and of one species each: the
+// charge-free sums of num d / r^3 (i side, and the reactions rx, ry, rz) and
+// en / r in the unit of w; the caller applies the charges and powers of u.
+template <bool kRdf, bool kKelbg, int kNI, bool kClamp, bool kCubic>
+__device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsigned* hist,
                                      const FixPos (&pi)[kNI], int si, int sj, FixPos pj,
                                      float (&fxo)[kNI], float (&fyo)[kNI], float (&fzo)[kNI],
                                      float (&uo)[kNI], float& rx, float& ry, float& rz)
 {
-    const float qi = (float)a.q[si];
-    const float eqi = (float)(a.eps * a.q[si]);
-    const float qj = (float)a.q[sj];
     const bool reg = si != sj || ((a.flags & TCP_REG_EE) != 0 && si == 0 && sj == 0);
-    const float idx = k.inv_dx[si + sj];
+    const float idx = w.inv_dx[si + sj];
     unsigned* hrow = hist + (si + sj) * kHistStride;
     float fx[kNI], fy[kNI], fz[kNI], u[kNI];
 #pragma unroll
@@ -684,7 +777,7 @@ __device__ inline void n3_tile_multi(const MixConst& k, const PairArgs& a, unsig
     for (int s = 0; s < kTile; ++s) {
 #pragma unroll
         for (int n = 0; n < kNI; ++n) {
-            const MixRaw m = mix_raw<kKelbg, false>(k, pi[n], pj, reg, true);
+            const MixRaw m = mix_raw_u<kKelbg, kCubic>(w, pi[n], pj, reg);
             const float rm = m.in ? m.rinv : 0.f;
             const float g = m.num * (m.rinv * m.rinv) * rm;
             fx[n] = fmaf(g, m.dx, fx[n]);
@@ -711,17 +804,13 @@ __device__ inline void n3_tile_multi(const MixConst& k, const PairArgs& a, unsig
         ry = rot_f(ry);
         rz = rot_f(rz);
     }
-    const float eqq = eqi * qj, qq = qi * qj;
 #pragma unroll
     for (int n = 0; n < kNI; ++n) {
-        fxo[n] = eqq * fx[n];
-        fyo[n] = eqq * fy[n];
-        fzo[n] = eqq * fz[n];
-        uo[n] = qq * u[n];
+        fxo[n] = fx[n];
+        fyo[n] = fy[n];
+        fzo[n] = fz[n];
+        uo[n] = u[n];
     }
-    rx *= eqq;
-    ry *= eqq;
-    rz *= eqq;
 }
 
 // One block pair (row br, column bc) of kBT x kBT tile pairs by kW wavefronts.
@@ -744,6 +833,8 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
     if (kRdf) hist_clear(hist);
     __syncthreads();
     const MixConst k = mix_const(a);
+    const bool cubic = a.box[0] == a.box[1] && a.box[1] == a.box[2];
+    const MixUnits w = mix_units(a, cubic);
     const int wave = threadIdx.x / kTile, lane = threadIdx.x % kTile;
     const int ntiles = (n + kTile - 1) / kTile;
     // a tile holds one species and only live particles
@@ -777,20 +868,30 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
             const int q = tj * kTile + lane;
             if (all_uni && juni) {
                 float fx[kNI], fy[kNI], fz[kNI], u[kNI], rx, ry, rz;
-                if (k.rdf_clamp)
-                    n3_tile_multi<kRdf, kKelbg, kNI, true>(k, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
-                else
-                    n3_tile_multi<kRdf, kKelbg, kNI, false>(k, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                if (cubic) {
+                    if (k.rdf_clamp)
+                        n3_tile_multi<kRdf, kKelbg, kNI, true, true>(w, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                    else
+                        n3_tile_multi<kRdf, kKelbg, kNI, false, true>(w, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                } else {
+                    if (k.rdf_clamp)
+                        n3_tile_multi<kRdf, kKelbg, kNI, true, false>(w, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                    else
+                        n3_tile_multi<kRdf, kKelbg, kNI, false, false>(w, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                }
+                // the charge-free FP32 sums take eps qi qj u^-2 and qi qj u^-1 in FP64
+                const double qq = a.q[s0] * a.q[sj];
+                const double sf = a.eps * qq * w.f_scale, su = qq * w.e_scale;
 #pragma unroll
                 for (int h = 0; h < kNI; ++h) {
-                    Fx[h] += (double)fx[h];
-                    Fy[h] += (double)fy[h];
-                    Fz[h] += (double)fz[h];
-                    U[h] += (double)u[h];
+                    Fx[h] += sf * (double)fx[h];
+                    Fy[h] += sf * (double)fy[h];
+                    Fz[h] += sf * (double)fz[h];
+                    U[h] += su * (double)u[h];
                 }
-                atomicAdd(&racc[0][q], (double)rx);
-                atomicAdd(&racc[1][q], (double)ry);
-                atomicAdd(&racc[2][q], (double)rz);
+                atomicAdd(&racc[0][q], sf * (double)rx);
+                atomicAdd(&racc[1][q], sf * (double)ry);
+                atomicAdd(&racc[2][q], sf * (double)rz);
             } else {
 #pragma unroll
                 for (int h = 0; h < kNI; ++h) {
@@ -1209,7 +1310,7 @@ __global__ __launch_bounds__(kF64Waves * kTile) void pair_f64_n3_batch_kernel(co
 // Persistent: resident blocks take the block pairs of blkmap (full blocks
 // first, the half-cost diagonal ones last) from a device queue.
 template <bool kRdf, bool kKelbg>
-__global__ __launch_bounds__(kN3Waves * kTile) void pair_mixed_n3_kernel(PairArgs a,
+__global__ __launch_bounds__(kN3Waves * kTile, 4) void pair_mixed_n3_kernel(PairArgs a,
                                                                          const int2* blkmap,
                                                                          int nblk, int* queue)
 {
