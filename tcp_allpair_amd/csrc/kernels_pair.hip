@@ -12,7 +12,7 @@
 // atomics and the per-(chunk,i) partials are summed in a fixed order later:
 // FP64 results do not depend on scheduling.  RDF counts go to a per-block LDS
 // histogram (3 x 512 uint32) flushed with one global 64-bit atomic per bin.
-#include "erfcx_table.h"
+#include "f64_tables.h"
 #include "tcp_internal.h"
 
 namespace tcp {
@@ -1033,28 +1033,49 @@ __device__ inline double rot_d(double v)
 }
 
 // erfcx(x) = exp(x^2) erfc(x), x >= 0, from the LDS copy of kErfcxTab: the
-// interval holding x (width 1/8 below 8, 1/2 from 8 to 28, the last one past
-// 28) and the degree-9 polynomial in t = (x - midpoint) / half-width.
+// interval of width 1/8 holding x (the last one past 28) and the degree-9
+// polynomial in t = 16 x - (2k + 1).  The row offset k (kErfcxDeg + 1) is a
+// 24-bit multiply (full rate; a 32-bit one issues at quarter rate).
 __device__ inline double erfcx_tab(double x, const double* tab)
 {
-    const bool lo = x < 8.0;
-    const double u = lo ? x * 16.0 : (x - 8.0) * 4.0;  // (x - start) / half-width
-    const int k = min((int)(u * 0.5), lo ? kErfcxLo - 1 : kErfcxHi - 1);
-    const double t = fmin(u - (double)(2 * k + 1), 1.0);
-    const double* c = tab + (k + (lo ? 0 : kErfcxLo)) * (kErfcxDeg + 1);
+    const int k = min((int)(x * 8.0), kErfcxN - 1);
+    const double t = fmin(fma(x, 16.0, -(double)(2 * k + 1)), 1.0);
+    const double* c = tab + __umul24((unsigned)k, kErfcxDeg + 1);
     double p = c[kErfcxDeg];
 #pragma unroll
     for (int j = kErfcxDeg - 1; j >= 0; --j) p = fma(p, t, c[j]);
     return p;
 }
 
+// e^y for y <= 0 by Tang's table-driven reduction (ACM TOMS 15, 1989):
+// y = (64 n + j) ln2/64 + r with |r| <= ln2/128, e^y = 2^n 2^(j/64) e^r,
+// 2^(j/64) from the LDS copy of kExp2Tab and e^r - 1 by its Taylor polynomial
+// to r^5 (remainder below 4e-17); ln2/64 as a sum of two doubles keeps r to an
+// ulp.  About 1.5 ulp; underflows to 0 past -745.
+__device__ inline double exp_tab(double y, const double* e2tab)
+{
+    constexpr double k64_ln2 = 92.332482616893656;              // 64 / ln 2
+    constexpr double ln2_64_hi = 0x1.62e42fefa39efp-7;          // ln 2 / 64
+    constexpr double ln2_64_lo = 0x1.abc9e3b39803fp-62;         // ln 2 / 64 - ln2_64_hi
+    const double k = rint(y * k64_ln2);
+    const double r = fma(-k, ln2_64_lo, fma(-k, ln2_64_hi, y));
+    const int ki = (int)k;
+    double q = fma(r, 1.0 / 120.0, 1.0 / 24.0);
+    q = fma(q, r, 1.0 / 6.0);
+    q = fma(q, r, 0.5);
+    q = fma(q, r, 1.0);
+    q *= r;  // e^r - 1
+    const double t = e2tab[ki & (kExp2N - 1)];
+    return ldexp(fma(t, q, t), ki >> 6);
+}
+
 // exp(-x^2) with x^2 carried as hi + lo, so the result keeps the accuracy of
 // exp (the rounding of x^2 would cost up to 2 x^2 ulp)
-__device__ inline double exp_mx2(double x)
+__device__ inline double exp_mx2(double x, const double* e2tab)
 {
     const double hi = x * x;
     const double lo = fma(x, x, -hi);
-    const double e = exp(-hi);
+    const double e = exp_tab(-hi, e2tab);
     return fma(-e, lo, e);
 }
 
@@ -1079,6 +1100,7 @@ __device__ inline double rsq_f64(double r2)
 // erfc(x) = exp(-x^2) erfcx(x) shares exp(-x^2) with the Gaussian term.
 template <bool kRdf, bool kKelbg, bool kDiag>
 __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const double* etab,
+                                   const double* e2tab,
                                    double xi, double yi, double zi, int si, bool ilive, double xj,
                                    double yj, double zj, int fj, double& fxo, double& fyo,
                                    double& fzo, double& uo, double& rx, double& ry, double& rz)
@@ -1124,19 +1146,19 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
             const double qq = qi * (sj ? qj1 : qj0);
             const double inv_r2 = inv_r * inv_r;
             const double x = alpha * r;
-            const double e = exp_mx2(x);
+            const double e = exp_mx2(x, e2tab);
             const double ec = e * erfcx_tab(x, etab);
             double num = fma(two_sqrtpi * x, e, ec), en = ec;
             if (sj ? reg1 : reg0) {
                 const double sr = r * inv_rs;
                 if (kKelbg) {
                     // R = -e^{-s^2} + sqrt(pi) s erfc(s)
-                    const double y2 = exp_mx2(sr);
+                    const double y2 = exp_mx2(sr, e2tab);
                     num -= y2;
                     en += y2 * fma(kSqrtPi * sr, erfcx_tab(sr, etab), -1.0);
                 } else {
                     // R = -e^{-s} (Deutsch, force.f90:136-139)
-                    const double y = exp(-sr);
+                    const double y = exp_tab(-sr, e2tab);
                     num -= y * (1.0 + sr);
                     en -= y;
                 }
@@ -1177,12 +1199,14 @@ struct F64Lds {
     double wbuf[kF64Waves][3][kTile];
     unsigned hist[3 * kHistStride];
     double etab[kErfcxN * (kErfcxDeg + 1)] __attribute__((aligned(16)));
+    double e2tab[kExp2N];
 };
 
-__device__ inline void load_erfcx(double* etab)
+__device__ inline void load_tables(F64Lds& L)
 {
     const double* src = &kErfcxTab[0][0];
-    for (int i = threadIdx.x; i < kErfcxN * (kErfcxDeg + 1); i += blockDim.x) etab[i] = src[i];
+    for (int i = threadIdx.x; i < kErfcxN * (kErfcxDeg + 1); i += blockDim.x) L.etab[i] = src[i];
+    for (int i = threadIdx.x; i < kExp2N; i += blockDim.x) L.e2tab[i] = kExp2Tab[i];
 }
 
 template <bool kRdf, bool kKelbg>
@@ -1219,10 +1243,10 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L
             const int q = tj * kTile + lane;
             double fx, fy, fz, u;
             if (dblk && tj == wave)
-                n3_tile_f64<kRdf, kKelbg, true>(a, L.hist, L.etab, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
+                n3_tile_f64<kRdf, kKelbg, true>(a, L.hist, L.etab, L.e2tab, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
                                         L.jf[q], fx, fy, fz, u, rx, ry, rz);
             else
-                n3_tile_f64<kRdf, kKelbg, false>(a, L.hist, L.etab, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
+                n3_tile_f64<kRdf, kKelbg, false>(a, L.hist, L.etab, L.e2tab, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
                                          L.jf[q], fx, fy, fz, u, rx, ry, rz);
             Fx += fx;
             Fy += fy;
@@ -1280,7 +1304,7 @@ __global__ __launch_bounds__(kF64Waves * kTile, 4) void pair_f64_n3_kernel(PairA
 {
     __shared__ F64Lds lds;
     __shared__ int slot;
-    load_erfcx(lds.etab);  // the first queue_next orders it before use
+    load_tables(lds);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nblk) break;
@@ -1297,7 +1321,7 @@ __global__ __launch_bounds__(kF64Waves * kTile) void pair_f64_n3_batch_kernel(co
 {
     __shared__ F64Lds lds;
     __shared__ int slot;
-    load_erfcx(lds.etab);  // the first queue_next orders it before use
+    load_tables(lds);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nitems) break;
@@ -1337,7 +1361,7 @@ __global__ __launch_bounds__(kN3Waves * kTile, 4) void pair_mixed_n3_kernel(Pair
 // keeps the launch free of memset nodes when it is captured in a graph.
 
 template <bool kRdf, bool kKelbg, int kBT, int kW>
-__global__ __launch_bounds__(kW * kTile) void pair_mixed_n3_batch_kernel(const BatchRep* reps,
+__global__ __launch_bounds__(kW * kTile, 4) void pair_mixed_n3_batch_kernel(const BatchRep* reps,
                                                                          const PairItem* items,
                                                                          int nitems, int* queue)
 {

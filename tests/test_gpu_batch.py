@@ -140,9 +140,11 @@ def test_autotune_and_graph_reuse():
         t2 = b.md_steps(3, NOTEMP, dt)  # replays the captured step
         s1 = [b.get_state(r) for r in range(len(reps))]
     tr, st, _ = run_batch(reps, MIXED, 6, NOTEMP, dt, shape=0)
-    np.testing.assert_array_equal(np.concatenate([t1, t2]), tr)
+    # the mixed kernel adds the reactions of its waves with LDS FP64 atomics, in
+    # the order the waves run: replays agree with a fresh run to rounding
+    np.testing.assert_allclose(np.concatenate([t1, t2]), tr, rtol=1e-11, atol=0)
     for r in range(len(reps)):
-        np.testing.assert_array_equal(s1[r][0], st[r][0])
+        np.testing.assert_allclose(s1[r][0], st[r][0], rtol=1e-11, atol=1e-12 * np.abs(st[r][0]).max())
 
 
 def test_batch_errors():

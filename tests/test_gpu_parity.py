@@ -347,7 +347,7 @@ def test_fullsize_mixed_step_properties():
     """131072 particles, SPME 128^3 order 6, mixed: after a step the mean force
     is removed exactly and the energies are finite; two calls of one step each
     equal one call of two steps (the state carries over on the device) up to
-    the order of the FP64 atomic adds of the SPME charge spreading."""
+    the order of the mixed kernel's LDS FP64 atomic adds of reactions."""
     say("case fullsize")
     p, x, v = make_plasma(65536, 65536, seed=7, K=(128, 128, 128), order=6)
     dt = reduced_dt(0.002)
