@@ -370,6 +370,15 @@ __device__ inline void close_kelbg(float s, float x, float erfc_s, float g, floa
 constexpr int kMixTile = 32;
 
 // Loop-invariant FP32 constants of the mixed pair arithmetic.
+// One of three values by an index, as a select of values: the empty asm makes
+// them values first, so the compiler cannot turn the select into a load from a
+// selected address (which would keep the struct holding them in scratch).
+__device__ inline float pick3(float a, float b, float c, int i)
+{
+    asm volatile("" : "+v"(a), "+v"(b), "+v"(c));
+    return i == 0 ? a : (i == 1 ? b : c);
+}
+
 struct MixConst {
     float sx, sy, sz;          // L_d 2^-32
     float rc2;
@@ -379,7 +388,7 @@ struct MixConst {
     float inv_rs;              // 1/lambda
     float nl2rs;               // -log2(e)/lambda
     float rclose;              // 2 lambda (Deutsch) or lambda (Kelbg)
-    float inv_dx[3];           // RDF bin widths (reciprocal)
+    float idx_ee, idx_ei, idx_ii;  // RDF bin widths (reciprocal) of e-e, e-i, i-i pairs
     float rfix2;               // below this r^2 the FP32 residuals join the separation
     bool kelbg;                // Kelbg instead of Deutsch regularisation
     bool rdf_clamp;            // a pair inside rc can land beyond the padded histogram row
@@ -399,7 +408,9 @@ __device__ inline MixConst mix_const(const PairArgs& a)
     k.nl2rs = (float)(-1.4426950408889634 / a.rs);
     k.kelbg = (a.flags & TCP_KELBG) != 0;
     k.rclose = (float)((k.kelbg ? 1.0 : 2.0) * a.rs);
-    for (int t = 0; t < 3; ++t) k.inv_dx[t] = (float)a.inv_dx[t];
+    k.idx_ee = (float)a.inv_dx[0];
+    k.idx_ei = (float)a.inv_dx[1];
+    k.idx_ii = (float)a.inv_dx[2];
     // the fixed-point separation alone is off by at most L 2^-33 per axis; beyond
     // r = L 2^-8 that is below FP32 rounding of the separation (2^-24 r)
     const double lmax = fmax(fmax(a.box[0], a.box[1]), a.box[2]);
@@ -515,6 +526,7 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
         const float eqq = (float)(a.eps * a.q[si] * a.q[sj]);
         const bool reg = si != sj || (si == 0 && sj == 0 && (a.flags & TCP_REG_EE));
         const int ht = si + sj;
+        const float idx = pick3(k.idx_ee, k.idx_ei, k.idx_ii, ht);
         for (int jt = lo; jt < hi; jt += kMixTile) {
             const int je = min(hi, jt + kMixTile);
             float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
@@ -526,7 +538,7 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
                 fz = fmaf(o.df, o.dz, fz);
                 u += o.du;
                 if (kRdf && o.in) {
-                    const int b = (int)(o.r * k.inv_dx[ht]);
+                    const int b = (int)(o.r * idx);
                     if (b < kNbin) atomicAdd(&hist[ht * kHistStride + b], 1u);
                 }
             }
@@ -597,8 +609,8 @@ __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* h
     const float qi = (float)a.q[si];
     const float eqi = (float)(a.eps * a.q[si]);
     // RDF bin widths for a j electron / a j ion, seen from this i
-    const float idx0 = si ? k.inv_dx[1] : k.inv_dx[0];
-    const float idx1 = si ? k.inv_dx[2] : k.inv_dx[1];
+    const float idx0 = pick3(k.idx_ee, k.idx_ei, k.idx_ii, si);
+    const float idx1 = pick3(k.idx_ee, k.idx_ei, k.idx_ii, si + 1);
     const float qj_uni = (float)a.q[sj_uni];
     const bool regee = (a.flags & TCP_REG_EE) != 0;
     const bool reg_uni = si != sj_uni || (regee && si == 0 && sj_uni == 0);
@@ -680,7 +692,7 @@ struct MixUnits {
     float half_alpha, two_a_sqrtpi, inv_rs, nl2rs;  // x u
     float na2l;              // x u^2
     float rclose;            // / u
-    float inv_dx[3];         // x u
+    float idx_ee, idx_ei, idx_ii;  // RDF bin widths (reciprocal) x u
     double f_scale, e_scale; // u^-2 (num d / r^3), u^-1 (en / r)
 };
 
@@ -702,7 +714,9 @@ __device__ inline MixUnits mix_units(const PairArgs& a, bool cubic)
     w.na2l = (float)(-a.alpha * a.alpha * 1.4426950408889634 * u * u);
     const bool kelbg = (a.flags & TCP_KELBG) != 0;
     w.rclose = (float)((kelbg ? 1.0 : 2.0) * a.rs / u);
-    for (int t = 0; t < 3; ++t) w.inv_dx[t] = (float)(a.inv_dx[t] * u);
+    w.idx_ee = (float)(a.inv_dx[0] * u);
+    w.idx_ei = (float)(a.inv_dx[1] * u);
+    w.idx_ii = (float)(a.inv_dx[2] * u);
     w.f_scale = 1.0 / (u * u);
     w.e_scale = 1.0 / u;
     return w;
@@ -768,7 +782,8 @@ __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsig
                                      float (&uo)[kNI], float& rx, float& ry, float& rz)
 {
     const bool reg = si != sj || ((a.flags & TCP_REG_EE) != 0 && si == 0 && sj == 0);
-    const float idx = w.inv_dx[si + sj];
+    const int ht = si + sj;
+    const float idx = pick3(w.idx_ee, w.idx_ei, w.idx_ii, ht);
     unsigned* hrow = hist + (si + sj) * kHistStride;
     float fx[kNI], fy[kNI], fz[kNI], u[kNI];
 #pragma unroll
