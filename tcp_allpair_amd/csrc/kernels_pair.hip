@@ -722,13 +722,16 @@ __device__ inline MixUnits mix_units(const PairArgs& a, bool cubic)
     return w;
 }
 
-// mix_raw for the uniform tile path, in the unit u of MixUnits.
+// mix_raw for the uniform tile path, in the unit u of MixUnits: j's
+// fixed-point coordinates in qu, its FP32 residuals (read only for a pair
+// closer than L 2^-8) in q.
 template <bool kKelbg, bool kCubic>
-__device__ inline MixRaw mix_raw_u(const MixUnits& w, const FixPos& pi, const FixPos& q, bool reg)
+__device__ inline MixRaw mix_raw_u(const MixUnits& w, const FixPos& pi, uint4 qu, const FixPos& q,
+                                   bool reg)
 {
     MixRaw o;
-    const float ix = (float)(int)(pi.ux - q.ux), iy = (float)(int)(pi.uy - q.uy),
-                iz = (float)(int)(pi.uz - q.uz);
+    const float ix = (float)(int)(pi.ux - qu.x), iy = (float)(int)(pi.uy - qu.y),
+                iz = (float)(int)(pi.uz - qu.z);
     o.dx = kCubic ? ix : ix * w.sx;
     o.dy = kCubic ? iy : iy * w.sy;
     o.dz = kCubic ? iz : iz * w.sz;
@@ -777,10 +780,18 @@ __device__ inline MixRaw mix_raw_u(const MixUnits& w, const FixPos& pi, const Fi
 // en / r in the unit of w; the caller applies the charges and powers of u.
 template <bool kRdf, bool kKelbg, int kNI, bool kClamp, bool kCubic>
 __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsigned* hist,
-                                     const FixPos (&pi)[kNI], int si, int sj, FixPos pj,
-                                     float (&fxo)[kNI], float (&fyo)[kNI], float (&fzo)[kNI],
-                                     float (&uo)[kNI], float& rx, float& ry, float& rz)
+                                     const FixPos (&pi)[kNI], int si, int sj, const FixPos* jpos,
+                                     int tj, float (&fxo)[kNI], float (&fyo)[kNI],
+                                     float (&fzo)[kNI], float (&uo)[kNI], float& rx, float& ry,
+                                     float& rz)
 {
+    // The j of a lane comes from the block's LDS copy: lane l meets at step s
+    // the j of lane l - s of the tile, whose byte offset lane l - 1 held at
+    // step s - 1, so the offset rotates with the reactions (one DPP move) and
+    // the next step's coordinates are read while this step's pairs compute.
+    const char* jb = (const char*)jpos;
+    uint32_t off = (uint32_t)(tj * kTile + (int)(threadIdx.x % kTile)) * (uint32_t)sizeof(FixPos);
+    uint4 qu = *(const uint4*)(jb + off);
     const bool reg = si != sj || ((a.flags & TCP_REG_EE) != 0 && si == 0 && sj == 0);
     const int ht = si + sj;
     const float idx = pick3(w.idx_ee, w.idx_ei, w.idx_ii, ht);
@@ -789,10 +800,14 @@ __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsig
 #pragma unroll
     for (int n = 0; n < kNI; ++n) fx[n] = fy[n] = fz[n] = u[n] = 0.f;
     rx = ry = rz = 0.f;
-    for (int s = 0; s < kTile; ++s) {
+#pragma unroll 2
+    for (int s = 0; s < kTile; ++s) {  // by two: the carried offset and coordinates alternate registers
+        const uint32_t noff = rot_u(off);
+        const uint4 nqu = *(const uint4*)(jb + noff);
+        const FixPos& q = *(const FixPos*)(jb + off);
 #pragma unroll
         for (int n = 0; n < kNI; ++n) {
-            const MixRaw m = mix_raw_u<kKelbg, kCubic>(w, pi[n], pj, reg);
+            const MixRaw m = mix_raw_u<kKelbg, kCubic>(w, pi[n], qu, q, reg);
             const float rm = m.in ? m.rinv : 0.f;
             const float g = m.num * (m.rinv * m.rinv) * rm;
             fx[n] = fmaf(g, m.dx, fx[n]);
@@ -809,12 +824,8 @@ __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsig
                 atomicAdd(&hrow[kClamp ? min(b, kHistStride - 1) : b], 2u);
             }
         }
-        pj.ux = rot_u(pj.ux);
-        pj.uy = rot_u(pj.uy);
-        pj.uz = rot_u(pj.uz);
-        pj.rx = rot_f(pj.rx);
-        pj.ry = rot_f(pj.ry);
-        pj.rz = rot_f(pj.rz);
+        off = noff;
+        qu = nqu;
         rx = rot_f(rx);
         ry = rot_f(ry);
         rz = rot_f(rz);
@@ -885,14 +896,14 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
                 float fx[kNI], fy[kNI], fz[kNI], u[kNI], rx, ry, rz;
                 if (cubic) {
                     if (k.rdf_clamp)
-                        n3_tile_multi<kRdf, kKelbg, kNI, true, true>(w, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                        n3_tile_multi<kRdf, kKelbg, kNI, true, true>(w, a, hist, pi, s0, sj, jpos, tj, fx, fy, fz, u, rx, ry, rz);
                     else
-                        n3_tile_multi<kRdf, kKelbg, kNI, false, true>(w, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                        n3_tile_multi<kRdf, kKelbg, kNI, false, true>(w, a, hist, pi, s0, sj, jpos, tj, fx, fy, fz, u, rx, ry, rz);
                 } else {
                     if (k.rdf_clamp)
-                        n3_tile_multi<kRdf, kKelbg, kNI, true, false>(w, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                        n3_tile_multi<kRdf, kKelbg, kNI, true, false>(w, a, hist, pi, s0, sj, jpos, tj, fx, fy, fz, u, rx, ry, rz);
                     else
-                        n3_tile_multi<kRdf, kKelbg, kNI, false, false>(w, a, hist, pi, s0, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                        n3_tile_multi<kRdf, kKelbg, kNI, false, false>(w, a, hist, pi, s0, sj, jpos, tj, fx, fy, fz, u, rx, ry, rz);
                 }
                 // the charge-free FP32 sums take eps qi qj u^-2 and qi qj u^-1 in FP64
                 const double qq = a.q[s0] * a.q[sj];
