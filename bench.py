@@ -23,7 +23,9 @@ Force_direct_ext (compiled from source, oracle/_ref) timed on the host cores on
 a bounded row slice of the same workload.  batch: the 64-replica batching
 workload of tcp_allpair_amd/batchbench.py (one persistent pair launch and a
 captured step for all replicas) against the same replicas stepped one
-context after another.
+context after another.  dropin_host_buffers: the step driven as the Fortran
+drop-in drives it, each call copying the host arrays it reads and writes (the
+PCIe-inclusive rate; never value).
 """
 from __future__ import annotations
 
@@ -79,6 +81,51 @@ def run(p, x, v, f0, precision, steps, warmup, dt):
     return dict(wall_s=t1 - t0, dev_ms=steps_ms, pair_ms=pair_ms, trace=tr)
 
 
+def dropin_run(p, x, v, f0, precision, steps, dt):
+    """The step as the Fortran drop-in drives it with one direct rank
+    (tcp_allpair_amd/fortran/tcp_force_gpu.f90): every call uploads the host
+    arrays it reads and downloads what it writes, the host state (AoS x, v, f)
+    staying authoritative.  Per step: vverlet1 (x, v, f in; x, v out),
+    force_direct (x in; f out), force_recip (x, f in; f out), remove_rigidmtn
+    (f in and out), vverlet2 (v, f in; v out): 9 uploads and 6 downloads of
+    3N doubles.  The shim's own copies between the PT records and these arrays
+    are not included."""
+    import ctypes
+    with Context(p, precision) as c:
+        lib, h = c.lib, c.h
+        hx, hv, hf = (np.ascontiguousarray(a, dtype=np.float64).copy() for a in (x, v, f0))
+        px, pv, pf = (a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in (hx, hv, hf))
+        ur, um, a, b = (ctypes.c_double() for _ in range(4))
+
+        def ck(rc, what):
+            if rc != 0:
+                raise RuntimeError(f"{what}: {lib.tcp_last_error(h).decode()}")
+
+        def step():
+            ck(lib.tcp_set_state(h, px, pv, pf), "set_state")
+            ck(lib.tcp_vverlet1(h, NOTEMP, dt, 0.0), "vverlet1")
+            ck(lib.tcp_get_state(h, px, pv, None), "get_state")
+            ck(lib.tcp_set_state(h, px, None, None), "set_state")
+            ck(lib.tcp_force_direct(h, ctypes.byref(ur)), "force_direct")
+            ck(lib.tcp_get_state(h, None, None, pf), "get_state")
+            ck(lib.tcp_set_state(h, px, None, pf), "set_state")
+            ck(lib.tcp_force_recip(h, ctypes.byref(um)), "force_recip")
+            ck(lib.tcp_get_state(h, None, None, pf), "get_state")
+            ck(lib.tcp_set_state(h, None, None, pf), "set_state")
+            ck(lib.tcp_remove_rigidmtn(h), "remove_rigidmtn")
+            ck(lib.tcp_get_state(h, None, None, pf), "get_state")
+            ck(lib.tcp_set_state(h, None, pv, pf), "set_state")
+            ck(lib.tcp_vverlet2(h, NOTEMP, dt, p.Te, p.Ti, p.tau, 0.0, 1, ctypes.byref(a),
+                                ctypes.byref(b)), "vverlet2")
+            ck(lib.tcp_get_state(h, None, pv, None), "get_state")
+
+        step()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        return (time.perf_counter() - t0) / steps * 1e3
+
+
 def cpu_baseline(p, x):
     """The reference's Force_direct_ext on host cores: each worker times its own
     512-particle row slice against all N particles (about 10 s per worker)."""
@@ -126,6 +173,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="steps of the 64-replica batching workload (0 skips it)")
+    ap.add_argument("--dropin-steps", type=int, default=5,
+                    help="steps driven as the Fortran drop-in drives them (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -156,6 +205,13 @@ def main():
         for line in open(PROFILE_TRAFFIC):
             if line.startswith("hbm_bytes_per_launch:"):
                 traffic = float(line.split(":")[1])
+    dropin = None
+    if args.dropin_steps > 0:
+        print("[bench] host-buffer steps (the drop-in's call pattern)", file=sys.stderr, flush=True)
+        dropin = {"steps": args.dropin_steps,
+                  "mixed_ms_per_step": dropin_run(p, x, v, f64_0, MIXED, args.dropin_steps, dt),
+                  "fp64_ms_per_step": dropin_run(p, x, v, f64_0, FP64, args.dropin_steps, dt),
+                  "transfers_per_step": "9 uploads + 6 downloads of 3N doubles (AoS host arrays)"}
     batch = None
     if args.batch_steps > 0:
         print("[bench] batching workload (64 replicas)", file=sys.stderr, flush=True)
@@ -200,6 +256,7 @@ def main():
                      "traffic": traffic, "kernel": "pair_mixed_n3_kernel<true, false>",
                      "kernel_ms": mx["pair_ms"], "flop_per_launch": flops},
         "cpu_baseline": cpu,
+        "dropin_host_buffers": dropin,
         "batch": batch,
     }
     print(json.dumps(out))

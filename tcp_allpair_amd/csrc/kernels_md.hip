@@ -474,6 +474,33 @@ __global__ void step_advance_kernel(uint64_t* counter)
     }
 }
 
+// Host-state transposes: one thread per particle, the AoS triple read or
+// written as three consecutive doubles (the block's lanes cover a contiguous
+// 24 B x 256 span).
+__global__ __launch_bounds__(256) void aos_to_soa_kernel(const double* __restrict__ aos,
+                                                         double* __restrict__ x,
+                                                         double* __restrict__ y,
+                                                         double* __restrict__ z, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    x[i] = aos[3 * (size_t)i];
+    y[i] = aos[3 * (size_t)i + 1];
+    z[i] = aos[3 * (size_t)i + 2];
+}
+
+__global__ __launch_bounds__(256) void soa_to_aos_kernel(const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ z,
+                                                         double* __restrict__ aos, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    aos[3 * (size_t)i] = x[i];
+    aos[3 * (size_t)i + 1] = y[i];
+    aos[3 * (size_t)i + 2] = z[i];
+}
+
 }  // namespace
 
 int md_blocks(int n) { return (n + kRedBlock - 1) / kRedBlock; }
@@ -481,6 +508,19 @@ int md_blocks(int n) { return (n + kRedBlock - 1) / kRedBlock; }
 void launch_vv1(const MdArgs& m, int thermo, double dt, double fric, hipStream_t s)
 {
     hipLaunchKernelGGL(vv1_kernel, dim3(md_blocks(m.n)), dim3(kRedBlock), 0, s, m, thermo, dt, fric);
+}
+
+void launch_aos_to_soa(const double* aos, double* x, double* y, double* z, int n, hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(aos_to_soa_kernel, dim3((n + 255) / 256), dim3(256), 0, s, aos, x, y, z, n);
+}
+
+void launch_soa_to_aos(const double* x, const double* y, const double* z, double* aos, int n,
+                       hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(soa_to_aos_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, y, z, aos, n);
 }
 
 void launch_make_upos(const MdArgs& m, hipStream_t s)

@@ -56,6 +56,11 @@ struct tcp_ctx {
     tcp::FixPos* gupos = nullptr;
     double *gpf = nullptr, *gpu = nullptr;
     size_t gpf_cap = 0;
+    // host-state transfers: AoS arrays staged in pinned host memory, moved by one
+    // DMA per call and transposed to/from the SoA state on the device
+    double* h_stage = nullptr;  // hipHostMalloc
+    double* d_stage = nullptr;  // hipMalloc
+    size_t stage_cap = 0;       // doubles in each
     // energy derivatives (allocated by the first tcp_energy_adjoint call)
     double *adj_part = nullptr, *adj_out = nullptr, *adj_q = nullptr, *adj_pot = nullptr;
     // timing

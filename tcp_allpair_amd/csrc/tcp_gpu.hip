@@ -259,40 +259,73 @@ int ensure_ext(tcp_ctx* c, int ng)
     return 0;
 }
 
-void aos_to_soa(const double* a, int n, std::vector<double>& s)
+// Host state in AoS (x0 y0 z0 x1 ...) moves through one pinned staging buffer
+// and one device buffer, each large enough for the arrays of one call: a host
+// copy into pinned memory, one DMA, and the transposes on the device (and the
+// reverse for downloads).
+int ensure_stage(tcp_ctx* c, size_t count)
 {
-    s.resize(3 * (size_t)n);
-    for (int i = 0; i < n; ++i)
-        for (int k = 0; k < 3; ++k) s[(size_t)k * n + i] = a[3 * (size_t)i + k];
+    if (count <= c->stage_cap) return 0;
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->d_stage) (void)hipFree(c->d_stage);
+    c->h_stage = nullptr;
+    c->d_stage = nullptr;
+    c->stage_cap = 0;
+    if (hipHostMalloc((void**)&c->h_stage, count * sizeof(double)) != hipSuccess) {
+        c->h_stage = nullptr;
+        return fail(c, TCP_ERR_HIP, "hipHostMalloc of the host staging buffer failed");
+    }
+    if (hipMalloc((void**)&c->d_stage, count * sizeof(double)) != hipSuccess) {
+        c->d_stage = nullptr;
+        return fail(c, TCP_ERR_HIP, "hipMalloc of the device staging buffer failed");
+    }
+    c->stage_cap = count;
+    return 0;
 }
 
-void soa_to_aos(const std::vector<double>& s, int n, double* a)
+// One AoS host array and its device SoA triple.
+struct Aos3 {
+    double* host;
+    double *x, *y, *z;
+};
+
+int upload_aos(tcp_ctx* c, const Aos3* t, int k, int n)
 {
-    for (int i = 0; i < n; ++i)
-        for (int k = 0; k < 3; ++k) a[3 * (size_t)i + k] = s[(size_t)k * n + i];
+    if (k == 0 || n == 0) return 0;
+    const size_t m = 3 * (size_t)n;
+    TRY(ensure_stage(c, m * k));
+    for (int j = 0; j < k; ++j) std::memcpy(c->h_stage + m * j, t[j].host, m * sizeof(double));
+    HIPCK(c, hipMemcpyAsync(c->d_stage, c->h_stage, m * k * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+    for (int j = 0; j < k; ++j) launch_aos_to_soa(c->d_stage + m * j, t[j].x, t[j].y, t[j].z, n, c->stream);
+    LAUNCHCK(c);
+    return sync(c);
+}
+
+int download_aos(tcp_ctx* c, const Aos3* t, int k, int n)
+{
+    if (k == 0 || n == 0) return 0;
+    const size_t m = 3 * (size_t)n;
+    TRY(ensure_stage(c, m * k));
+    for (int j = 0; j < k; ++j) launch_soa_to_aos(t[j].x, t[j].y, t[j].z, c->d_stage + m * j, n, c->stream);
+    LAUNCHCK(c);
+    HIPCK(c, hipMemcpyAsync(c->h_stage, c->d_stage, m * k * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    TRY(sync(c));
+    for (int j = 0; j < k; ++j) std::memcpy(t[j].host, c->h_stage + m * j, m * sizeof(double));
+    return 0;
 }
 
 int upload3(tcp_ctx* c, const double* aos, double* dx, double* dy, double* dz, int n)
 {
-    std::vector<double> s;
-    aos_to_soa(aos, n, s);
-    double* d[3] = {dx, dy, dz};
-    for (int k = 0; k < 3; ++k)
-        HIPCK(c, hipMemcpyAsync(d[k], s.data() + (size_t)k * n, n * sizeof(double),
-                                hipMemcpyHostToDevice, c->stream));
-    return sync(c);
+    const Aos3 t{const_cast<double*>(aos), dx, dy, dz};
+    return upload_aos(c, &t, 1, n);
 }
 
-int download3(tcp_ctx* c, double* aos, const double* dx, const double* dy, const double* dz, int n)
+int download3(tcp_ctx* c, double* aos, double* dx, double* dy, double* dz, int n)
 {
-    std::vector<double> s(3 * (size_t)n);
-    const double* d[3] = {dx, dy, dz};
-    for (int k = 0; k < 3; ++k)
-        HIPCK(c, hipMemcpyAsync(s.data() + (size_t)k * n, d[k], n * sizeof(double),
-                                hipMemcpyDeviceToHost, c->stream));
-    TRY(sync(c));
-    soa_to_aos(s, n, aos);
-    return 0;
+    const Aos3 t{aos, dx, dy, dz};
+    return download_aos(c, &t, 1, n);
 }
 
 int setup_recip(tcp_ctx* c)
@@ -516,6 +549,8 @@ int tcp_destroy(tcp_ctx* c)
     if (!c) return 0;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void* a : c->allocs) (void)hipFree(a);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->d_stage) (void)hipFree(c->d_stage);
     if (c->events) {
         for (int e = 0; e < kEvPool; ++e)
             for (int k = 0; k < 2; ++k) (void)hipEventDestroy(c->ev_pair[k][e]);
@@ -558,9 +593,12 @@ int tcp_ewald_self(tcp_ctx* c, double* alpha, double* rc2, double* Uo, double* p
 int tcp_set_state(tcp_ctx* c, const double* x, const double* v, const double* f)
 {
     if (!c) return TCP_ERR_ARG;
-    if (x) TRY(upload3(c, x, c->x, c->y, c->z, c->N));
-    if (v) TRY(upload3(c, v, c->vx, c->vy, c->vz, c->N));
-    if (f) TRY(upload3(c, f, c->fx, c->fy, c->fz, c->N));
+    Aos3 t[3];
+    int k = 0;
+    if (x) t[k++] = Aos3{const_cast<double*>(x), c->x, c->y, c->z};
+    if (v) t[k++] = Aos3{const_cast<double*>(v), c->vx, c->vy, c->vz};
+    if (f) t[k++] = Aos3{const_cast<double*>(f), c->fx, c->fy, c->fz};
+    TRY(upload_aos(c, t, k, c->N));
     if (x) {
         launch_make_upos(md_args(c), c->stream);
         LAUNCHCK(c);
@@ -572,9 +610,12 @@ int tcp_set_state(tcp_ctx* c, const double* x, const double* v, const double* f)
 int tcp_get_state(tcp_ctx* c, double* x, double* v, double* f)
 {
     if (!c) return TCP_ERR_ARG;
-    if (x) TRY(download3(c, x, c->x, c->y, c->z, c->N));
-    if (v) TRY(download3(c, v, c->vx, c->vy, c->vz, c->N));
-    if (f) TRY(download3(c, f, c->fx, c->fy, c->fz, c->N));
+    Aos3 t[3];
+    int k = 0;
+    if (x) t[k++] = Aos3{x, c->x, c->y, c->z};
+    if (v) t[k++] = Aos3{v, c->vx, c->vy, c->vz};
+    if (f) t[k++] = Aos3{f, c->fx, c->fy, c->fz};
+    TRY(download_aos(c, t, k, c->N));
     return 0;
 }
 

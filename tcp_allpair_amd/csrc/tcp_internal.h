@@ -109,6 +109,10 @@ void launch_fft3d_forward_convolve(const RecipArgs& a, hipStream_t s);
 void launch_fft3d_inverse(const RecipArgs& a, hipStream_t s);
 void launch_spme_gather(const RecipArgs& a, hipStream_t s);
 void launch_fill(double* p, size_t n, double v, hipStream_t s);
+// AoS (x0 y0 z0 x1 ...) <-> SoA transposes of n particles
+void launch_aos_to_soa(const double* aos, double* x, double* y, double* z, int n, hipStream_t s);
+void launch_soa_to_aos(const double* x, const double* y, const double* z, double* aos, int n,
+                       hipStream_t s);
 int fft_epart_blocks(const int* K);
 
 // Direct Ewald k-space sum (small systems, reference-independent extension):
