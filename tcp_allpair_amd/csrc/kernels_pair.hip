@@ -1041,13 +1041,15 @@ __device__ inline void queue_done(int* queue)
 // ------------------------------------------- FP64, Newton's third law ---
 //
 // The same tiling as the mixed kernel (tiles of 64, 8 x 8 block pairs of the
-// upper triangle, j data and reactions rotated by DPP), one i tile per wave
-// (8 waves), with the pair arithmetic of pair_f64_kernel: the reference's
-// minimum image and r^2 without contraction, erfc/exp of the device library,
-// so each pair's force is the reference's (force.f90:87-178), summed in a
-// different order.  Reactions of the waves meet through a per-wave LDS buffer
-// summed in wave order after each j tile, the i sums of a diagonal block join
-// once every j tile is done, so the result does not depend on scheduling.
+// upper triangle, reactions rotated by DPP, j read from LDS at a rotating
+// offset), one i tile per wave (8 waves).  The pair arithmetic is the
+// reference's formula (force.f90:87-178) in FP64: its minimum image and r^2
+// without contraction, so the cut-off test and the RDF bins are the
+// reference's bit for bit, with erfc and exp from the tables below, so each
+// pair's force agrees with the reference's to a few ulp, summed in a different
+// order.  Reactions of the waves meet through a per-wave LDS buffer summed in
+// wave order after each j tile, the i sums of a diagonal block join once every
+// j tile is done, so the result does not depend on scheduling.
 
 constexpr int kF64Waves = 8;
 
@@ -1116,8 +1118,16 @@ __device__ inline double rsq_f64(double r2)
     return fma(y, e, y);
 }
 
-// One tile pair: lane i against the 64 j of the rotating registers.  fj holds
-// the j species (bit 0) and liveness (bit 1).  kDiag: I == J, ordered pairs for
+// A j of the FP64 block's LDS copy: position and flags (species bit 0, live bit 1).
+struct __align__(32) F64J {
+    double x, y, z;
+    int f, pad;
+};
+
+// One tile pair: lane i against the 64 j of the tile at jt (LDS).  Lane l meets
+// at step s the j of lane l - s, whose byte offset lane l - 1 held at step
+// s - 1: the offset rotates with the reactions (one DPP move) and the next j is
+// read while this pair computes.  kDiag: I == J, ordered pairs for
 // the i side only, half energy, RDF weight 1.  The cut-off test and the RDF bin
 // are the reference's bit for bit: minimum image and r^2 without contraction,
 // and bin = int(sqrt(r2) / dx) with both operations correctly rounded, which
@@ -1126,11 +1136,13 @@ __device__ inline double rsq_f64(double r2)
 // erfc(x) = exp(-x^2) erfcx(x) shares exp(-x^2) with the Gaussian term.
 template <bool kRdf, bool kKelbg, bool kDiag>
 __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const double* etab,
-                                   const double* e2tab,
-                                   double xi, double yi, double zi, int si, bool ilive, double xj,
-                                   double yj, double zj, int fj, double& fxo, double& fyo,
+                                   const double* e2tab, double xi, double yi, double zi, int si,
+                                   bool ilive, const F64J* jt, double& fxo, double& fyo,
                                    double& fzo, double& uo, double& rx, double& ry, double& rz)
 {
+    const char* jb = (const char*)jt;
+    uint32_t off = (uint32_t)(threadIdx.x % kTile) * (uint32_t)sizeof(F64J);
+    F64J q = *(const F64J*)(jb + off);
     const double Lx = a.box[0], Ly = a.box[1], Lz = a.box[2];
     const double iLx = a.inv_box[0], iLy = a.inv_box[1], iLz = a.inv_box[2];
     const double alpha = a.alpha, rc2 = a.rc2;
@@ -1145,15 +1157,18 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
     const bool reg0 = si != 0 || (a.flags & TCP_REG_EE) != 0, reg1 = si == 0;
     double fx = 0.0, fy = 0.0, fz = 0.0, u = 0.0;
     rx = ry = rz = 0.0;
-#pragma unroll 1
-    for (int s = 0; s < kTile; ++s) {
+#pragma unroll 2
+    for (int s = 0; s < kTile; ++s) {  // by two: the carried offset and j alternate registers
+        const uint32_t noff = rot_u(off);
+        const F64J nq = *(const F64J*)(jb + noff);
+        const int fj = q.f;
         const int sj = fj & 1;
         double xr, yr, zr, r2;
         {
 #pragma clang fp contract(off)
-            const double dx = xi - xj;
-            const double dy = yi - yj;
-            const double dz = zi - zj;
+            const double dx = xi - q.x;
+            const double dy = yi - q.y;
+            const double dz = zi - q.z;
             xr = dx - Lx * rint(dx * iLx);
             yr = dy - Ly * rint(dy * iLy);
             zr = dz - Lz * rint(dz * iLz);
@@ -1200,10 +1215,8 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
                 rz -= df * zr;
             }
         }
-        xj = rot_d(xj);
-        yj = rot_d(yj);
-        zj = rot_d(zj);
-        fj = (int)rot_u((uint32_t)fj);
+        off = noff;
+        q = nq;
         if (!kDiag) {
             rx = rot_d(rx);
             ry = rot_d(ry);
@@ -1219,8 +1232,7 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
 // LDS of one FP64 block: j positions and flags, the block's reaction sums
 // (slot br), the per-wave reaction buffer and the RDF histogram.
 struct F64Lds {
-    double jx[kBlkTiles * kTile], jy[kBlkTiles * kTile], jz[kBlkTiles * kTile];
-    int jf[kBlkTiles * kTile];
+    F64J jp[kBlkTiles * kTile];
     double racc[4][kBlkTiles * kTile];
     double wbuf[kF64Waves][3][kTile];
     unsigned hist[3 * kHistStride];
@@ -1244,10 +1256,7 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L
     for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
         const int jp = bc * kBP + p;
         const int jc = min(jp, n - 1);
-        L.jx[p] = a.pi.x[jc];
-        L.jy[p] = a.pi.y[jc];
-        L.jz[p] = a.pi.z[jc];
-        L.jf[p] = (jp >= ne ? 1 : 0) | (jp < n ? 2 : 0);
+        L.jp[p] = F64J{a.pi.x[jc], a.pi.y[jc], a.pi.z[jc], (jp >= ne ? 1 : 0) | (jp < n ? 2 : 0), 0};
         L.racc[0][p] = L.racc[1][p] = L.racc[2][p] = L.racc[3][p] = 0.0;
     }
     if (kRdf) hist_clear(L.hist);
@@ -1266,14 +1275,14 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L
         const int J = bc * kBlkTiles + tj;
         double rx = 0.0, ry = 0.0, rz = 0.0;
         if (exists && J < ntiles && (!dblk || tj >= wave)) {
-            const int q = tj * kTile + lane;
+            const F64J* jt = L.jp + tj * kTile;
             double fx, fy, fz, u;
             if (dblk && tj == wave)
-                n3_tile_f64<kRdf, kKelbg, true>(a, L.hist, L.etab, L.e2tab, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
-                                        L.jf[q], fx, fy, fz, u, rx, ry, rz);
+                n3_tile_f64<kRdf, kKelbg, true>(a, L.hist, L.etab, L.e2tab, xi, yi, zi, si, ilive,
+                                                jt, fx, fy, fz, u, rx, ry, rz);
             else
-                n3_tile_f64<kRdf, kKelbg, false>(a, L.hist, L.etab, L.e2tab, xi, yi, zi, si, ilive, L.jx[q], L.jy[q], L.jz[q],
-                                         L.jf[q], fx, fy, fz, u, rx, ry, rz);
+                n3_tile_f64<kRdf, kKelbg, false>(a, L.hist, L.etab, L.e2tab, xi, yi, zi, si, ilive,
+                                                 jt, fx, fy, fz, u, rx, ry, rz);
             Fx += fx;
             Fy += fy;
             Fz += fz;
@@ -1341,7 +1350,7 @@ __global__ __launch_bounds__(kF64Waves * kTile, 4) void pair_f64_n3_kernel(PairA
 }
 
 template <bool kRdf, bool kKelbg>
-__global__ __launch_bounds__(kF64Waves * kTile) void pair_f64_n3_batch_kernel(const BatchRep* reps,
+__global__ __launch_bounds__(kF64Waves * kTile, 4) void pair_f64_n3_batch_kernel(const BatchRep* reps,
                                                                               const PairItem* items,
                                                                               int nitems, int* queue)
 {
