@@ -1396,7 +1396,9 @@ __global__ __launch_bounds__(kN3Waves * kTile, 4) void pair_mixed_n3_kernel(Pair
 // keeps the launch free of memset nodes when it is captured in a graph.
 
 template <bool kRdf, bool kKelbg, int kBT, int kW>
-__global__ __launch_bounds__(kW * kTile, 4) void pair_mixed_n3_batch_kernel(const BatchRep* reps,
+// four waves per SIMD (128 VGPRs) where two i tiles per wave fit in them, two
+// where four do not
+__global__ __launch_bounds__(kW * kTile, (kBT / kW) > 2 ? 2 : 4) void pair_mixed_n3_batch_kernel(const BatchRep* reps,
                                                                          const PairItem* items,
                                                                          int nitems, int* queue)
 {

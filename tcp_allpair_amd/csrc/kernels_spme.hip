@@ -194,7 +194,7 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(RecipArgs a, int nb)
     part[threadIdx.x] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        const int v = (int)threadIdx.x >= off ? part[threadIdx.x - off] : 0;
         __syncthreads();
         part[threadIdx.x] += v;
         __syncthreads();
@@ -408,7 +408,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
         red[threadIdx.x] = esum;
         __syncthreads();
         for (int w = kFftThreads / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
         if (threadIdx.x == 0) f.epart[blockIdx.x] = red[0];
@@ -637,7 +637,7 @@ __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const doub
         red[threadIdx.x] = e;
         __syncthreads();
         for (int w = 128; w > 0; w >>= 1) {
-            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
         if (threadIdx.x == 0) a.epart[0] = 2.0 * red[0];
