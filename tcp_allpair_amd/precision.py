@@ -9,9 +9,9 @@ identical positions: every `every` steps of the FP64 run, the mixed path
 evaluates the full force (real space + SPME, mean force removed) and Ur on the
 FP64 run's positions, against the FP64 forces and Ur there (Ur relative to
 itself, a sum that a near-neutral plasma mostly cancels, and eps dUr relative
-to E_tot).  Energy drift is
-each path's own: the slope of E_tot = KE + eps (Ur + Uo) + Um per step over its
-run, divided by |E_tot(0)|, and the largest |E_tot(t) - E_tot(0)| / |E_tot(0)|.
+to E_tot).  Energy drift is each path's own: the slope of E_tot = KE +
+eps (Ur + Uo) + Um per step over its run, divided by |E_tot(0)|, and the
+largest |E_tot(t) - E_tot(0)| / |E_tot(0)|.
 
     python -m tcp_allpair_amd.precision [--steps 5000] [--every 1000]
 
