@@ -567,16 +567,34 @@ __device__ inline void ewald_sk_reduce_body(const RecipArgs& a, int kmax, double
 // Forces F_j = (2 eps q_j / V) sum_m G m/L Im(conj(S) e^{2 pi i m.r_j}) (all m),
 // potential dUm/dq_j = eps/(pi V) sum_m G Re(conj(S) e^{2 pi i m.r_j}), and in
 // block 0 the energy sum_m G |S|^2 into epart[0]; all as twice the half space.
-// Every lane reads the same m at the same time, so the half-space table is
-// staged in LDS when it fits (kStaged) and read from memory otherwise.
-constexpr int kSkLds = 2048;  // half-space entries staged in LDS (kmax <= 7)
+// A block holds 64 particles and kKsumWaves waves; wave w walks its contiguous
+// share of the half space for all 64 (phases by recurrences from the share's
+// first m), and the shares' sums join in wave order, so the result does not
+// depend on scheduling.  Every lane reads the same m at the same time, so the
+// half-space table is staged in LDS when it fits (kStaged) and read from
+// memory otherwise.
+constexpr int kSkLds = 2048;   // half-space entries staged in LDS (kmax <= 7)
+constexpr int kKsumWaves = 8;  // shares of the half space per particle
+
+__device__ inline double2 conjd(double2 a) { return make_double2(a.x, -a.y); }
+
+// u^k for an integer k (|k| small): repeated products, u^-k = conj(u)^k on the unit circle
+__device__ inline double2 cpowi(double2 u, int k)
+{
+    const double2 b = k < 0 ? conjd(u) : u;
+    double2 e = make_double2(1.0, 0.0);
+    for (int t = k < 0 ? -k : k; t > 0; --t) e = cmul(e, b);
+    return e;
+}
 
 template <bool kStaged>
 __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const double* skg, int blk)
 {
-    const int j = blk * 256 + threadIdx.x;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int j = blk * 64 + lane;
     const int nk = 2 * kmax + 1, nh = (nk * nk * nk - 1) / 2;
     __shared__ double sks[kStaged ? 3 * kSkLds : 1];
+    __shared__ double red[kKsumWaves][4][64];
     if (kStaged) {
         // entry mi of the half space at sks[3 (mi - nh - 1) ..]
         for (int t = threadIdx.x; t < 3 * nh; t += blockDim.x) sks[t] = skg[3 * (nh + 1) + t];
@@ -584,41 +602,61 @@ __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const doub
     }
     // half-space entry h = mi - nh - 1
     const double* sk = kStaged ? sks : skg + 3 * (nh + 1);
-    if (j < a.n) {
+    const int h0 = wave * nh / kKsumWaves, h1 = (wave + 1) * nh / kKsumWaves;
+    double fx = 0.0, fy = 0.0, fz = 0.0, pt = 0.0;
+    if (j < a.n && h0 < h1) {
         const double2 ux = cexpi(2.0 * kPi * a.x[j] / a.box[0]);
         const double2 uy = cexpi(2.0 * kPi * a.y[j] / a.box[1]);
         const double2 uz = cexpi(2.0 * kPi * a.z[j] / a.box[2]);
-        // e^{-i kmax theta} for x and y
-        double2 xk = make_double2(1.0, 0.0), yk = make_double2(1.0, 0.0);
-        for (int m = 0; m < kmax; ++m) {
-            xk = cmul(xk, make_double2(ux.x, -ux.y));
-            yk = cmul(yk, make_double2(uy.x, -uy.y));
-        }
-        double fx = 0.0, fy = 0.0, fz = 0.0, pt = 0.0;
-        double2 ez = make_double2(1.0, 0.0);
-        for (int mz = 0; mz <= kmax; ++mz) {
-            const int my0 = mz == 0 ? 0 : -kmax;
-            double2 ey = mz == 0 ? make_double2(1.0, 0.0) : yk;
-            for (int my = my0; my <= kmax; ++my) {
-                const double2 eyz = cmul(ey, ez);
-                const int mx0 = (mz == 0 && my == 0) ? 1 : -kmax;
-                double2 e = cmul(eyz, mx0 == 1 ? ux : xk);
-                int h = ((mz + kmax) * nk + (my + kmax)) * nk + (mx0 + kmax) - (nh + 1);
-                double gx = 0.0, gi = 0.0;
-                for (int mx = mx0; mx <= kmax; ++mx, ++h) {
-                    const double Sr = sk[3 * h], Si = sk[3 * h + 1], G = sk[3 * h + 2];
-                    const double im = G * (Sr * e.y - Si * e.x);  // G Im(conj(S) e)
-                    gx += im * (double)mx;
-                    gi += im;
-                    pt += G * (Sr * e.x + Si * e.y);               // G Re(conj(S) e)
-                    e = cmul(e, ux);
-                }
+        const double2 xk = cpowi(ux, -kmax), yk = cpowi(uy, -kmax);  // row starts
+        int mx, my, mz;
+        decode_m(nh + 1 + h0, kmax, &mx, &my, &mz);
+        double2 ez = cpowi(uz, mz);
+        double2 eyz = cmul(cpowi(uy, my), ez);
+        double2 e = cmul(eyz, cpowi(ux, mx));
+        double gx = 0.0, gi = 0.0;  // the current row (my, mz)
+        for (int h = h0; h < h1; ++h) {
+            const double Sr = sk[3 * h], Si = sk[3 * h + 1], G = sk[3 * h + 2];
+            const double im = G * (Sr * e.y - Si * e.x);  // G Im(conj(S) e)
+            gx += im * (double)mx;
+            gi += im;
+            pt += G * (Sr * e.x + Si * e.y);               // G Re(conj(S) e)
+            if (mx < kmax) {
+                ++mx;
+                e = cmul(e, ux);
+            } else {  // end of the row: its sums, then the next row's first m
                 fx += gx;
                 fy += gi * (double)my;
                 fz += gi * (double)mz;
-                ey = cmul(ey, uy);
+                gx = gi = 0.0;
+                mx = -kmax;
+                if (my < kmax) {
+                    ++my;
+                    eyz = cmul(eyz, uy);
+                } else {
+                    my = -kmax;
+                    ++mz;
+                    ez = cmul(ez, uz);
+                    eyz = cmul(yk, ez);
+                }
+                e = cmul(eyz, xk);
             }
-            ez = cmul(ez, uz);
+        }
+        fx += gx;
+        fy += gi * (double)my;
+        fz += gi * (double)mz;
+    }
+    red[wave][0][lane] = fx;
+    red[wave][1][lane] = fy;
+    red[wave][2][lane] = fz;
+    red[wave][3][lane] = pt;
+    __syncthreads();
+    if (wave == 0 && j < a.n) {
+        for (int w = 1; w < kKsumWaves; ++w) {
+            fx += red[w][0][lane];
+            fy += red[w][1][lane];
+            fz += red[w][2][lane];
+            pt += red[w][3][lane];
         }
         const double V = a.box[0] * a.box[1] * a.box[2];
         if (a.pot) a.pot[j] = 2.0 * a.eps / (kPi * V) * pt;
@@ -628,19 +666,19 @@ __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const doub
         a.frz[j] = pre * fz / a.box[2];
     }
     if (blk == 0) {
-        __shared__ double red[256];
-        double e = 0.0;
+        __shared__ double esum[kKsumWaves * 64];
+        double en = 0.0;
         for (int h = threadIdx.x; h < nh; h += blockDim.x) {
             const int mi = nh + 1 + h;
-            e += skg[3 * mi + 2] * (skg[3 * mi] * skg[3 * mi] + skg[3 * mi + 1] * skg[3 * mi + 1]);
+            en += skg[3 * mi + 2] * (skg[3 * mi] * skg[3 * mi] + skg[3 * mi + 1] * skg[3 * mi + 1]);
         }
-        red[threadIdx.x] = e;
+        esum[threadIdx.x] = en;
         __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        for (int w = kKsumWaves * 32; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) esum[threadIdx.x] += esum[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0) a.epart[0] = 2.0 * red[0];
+        if (threadIdx.x == 0) a.epart[0] = 2.0 * esum[0];
     }
 }
 
@@ -657,7 +695,8 @@ __global__ __launch_bounds__(256) void ewald_sk_reduce_kernel(RecipArgs a, int k
 }
 
 template <bool kStaged>
-__global__ __launch_bounds__(256) void ewald_force_kernel(RecipArgs a, int kmax, const double* sk)
+__global__ __launch_bounds__(kKsumWaves * 64) void ewald_force_kernel(RecipArgs a, int kmax,
+                                                                      const double* sk)
 {
     ewald_force_body<kStaged>(a, kmax, sk, blockIdx.x);
 }
@@ -680,10 +719,10 @@ __global__ __launch_bounds__(256) void ewald_sk_reduce_batch_kernel(const BatchR
 }
 
 template <bool kStaged>
-__global__ __launch_bounds__(256) void ewald_force_batch_kernel(const BatchRep* reps)
+__global__ __launch_bounds__(kKsumWaves * 64) void ewald_force_batch_kernel(const BatchRep* reps)
 {
     const BatchRep& r = reps[blockIdx.y];
-    if (r.recip != TCP_RECIP_EWALD || (int)blockIdx.x >= (r.ra.n + 255) / 256) return;
+    if (r.recip != TCP_RECIP_EWALD || (int)blockIdx.x >= (r.ra.n + 63) / 64) return;
     ewald_force_body<kStaged>(r.ra, r.kmax, r.sk, blockIdx.x);
 }
 
@@ -801,12 +840,11 @@ void launch_ewald_kspace(const RecipArgs& a, int kmax, double alpha, double* sk,
     hipLaunchKernelGGL(ewald_sk_part_kernel, dim3(nchunk), dim3(256), lds, s, a, kmax, chunk);
     hipLaunchKernelGGL(ewald_sk_reduce_kernel, dim3((ewald_half(kmax) + 255) / 256), dim3(256), 0,
                        s, a, kmax, alpha, nchunk, sk);
+    const dim3 fg((a.n + 63) / 64), fb(kKsumWaves * 64);
     if (ewald_half(kmax) <= kSkLds)
-        hipLaunchKernelGGL(ewald_force_kernel<true>, dim3((a.n + 255) / 256), dim3(256), 0, s, a,
-                           kmax, (const double*)sk);
+        hipLaunchKernelGGL(ewald_force_kernel<true>, fg, fb, 0, s, a, kmax, (const double*)sk);
     else
-        hipLaunchKernelGGL(ewald_force_kernel<false>, dim3((a.n + 255) / 256), dim3(256), 0, s, a,
-                           kmax, (const double*)sk);
+        hipLaunchKernelGGL(ewald_force_kernel<false>, fg, fb, 0, s, a, kmax, (const double*)sk);
 }
 
 void launch_ewald_batch(const BatchRep* reps, int nrep, int kmax, int maxn, hipStream_t s)
@@ -817,12 +855,11 @@ void launch_ewald_batch(const BatchRep* reps, int nrep, int kmax, int maxn, hipS
                        lds, s, reps, chunk);
     hipLaunchKernelGGL(ewald_sk_reduce_batch_kernel, dim3((ewald_half(kmax) + 255) / 256, nrep),
                        dim3(256), 0, s, reps, chunk);
+    const dim3 fg((maxn + 63) / 64, nrep), fb(kKsumWaves * 64);
     if (ewald_half(kmax) <= kSkLds)
-        hipLaunchKernelGGL(ewald_force_batch_kernel<true>, dim3((maxn + 255) / 256, nrep), dim3(256),
-                           0, s, reps);
+        hipLaunchKernelGGL(ewald_force_batch_kernel<true>, fg, fb, 0, s, reps);
     else
-        hipLaunchKernelGGL(ewald_force_batch_kernel<false>, dim3((maxn + 255) / 256, nrep), dim3(256),
-                           0, s, reps);
+        hipLaunchKernelGGL(ewald_force_batch_kernel<false>, fg, fb, 0, s, reps);
     launch_um_batch(reps, nrep, s);
 }
 
