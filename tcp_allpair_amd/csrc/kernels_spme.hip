@@ -348,6 +348,8 @@ __device__ inline double2 cmul(double2 a, double2 b)
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+__device__ inline double2 conjd(double2 a) { return make_double2(a.x, -a.y); }
+
 template <int kMode>
 __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
 {
@@ -542,6 +544,69 @@ __device__ inline void ewald_sk_part_body(const RecipArgs& a, int kmax, int chun
     }
 }
 
+// The same partials by rows: thread t owns the half-space row (my, mz) t and
+// walks its mx (up to kLen = 2 kmax + 1 of them) by the recurrence
+// e <- e e^{i theta_x}, so a particle costs each row two complex products to
+// start and one per m, and the row's S(m) partials stay in registers.  Row 0 is
+// (my, mz) = (0, 0) with mx = 1..kmax; rows 1..kmax are mz = 0, my = 1..kmax;
+// then mz = 1..kmax with my = -kmax..kmax.  ux: e^{i theta_x} of the chunk.
+template <int kLen>
+__device__ inline void ewald_sk_rows_body(const RecipArgs& a, int kmax, int chunk, int c,
+                                          double2* ph, double2* ux)
+{
+    const int K1 = kmax + 1, nk = 2 * kmax + 1, nh = (nk * nk * nk - 1) / 2;
+    for (int t = threadIdx.x; t < 3 * chunk; t += blockDim.x) {
+        const int d = t / chunk, p = t % chunk, j = c * chunk + p;
+        const double* xd = d == 0 ? a.x : (d == 1 ? a.y : a.z);
+        double2 e = make_double2(j < a.n ? (d == 0 ? charge(a, j) : 1.0) : 0.0, 0.0);
+        const double2 u = cexpi(2.0 * kPi * (j < a.n ? xd[j] : 0.0) / a.box[d]);
+        if (d == 0) ux[p] = u;
+        for (int m = 0; m < K1; ++m) {
+            ph[(d * K1 + m) * chunk + p] = e;
+            e = cmul(e, u);
+        }
+    }
+    __syncthreads();
+    const int rows = 1 + kmax + kmax * nk;
+    for (int t = threadIdx.x; t < rows; t += blockDim.x) {
+        int my, mz, mx0;
+        if (t <= kmax) {
+            mz = 0;
+            my = t;
+            mx0 = t == 0 ? 1 : -kmax;
+        } else {
+            mz = 1 + (t - kmax - 1) / nk;
+            my = (t - kmax - 1) % nk - kmax;
+            mx0 = -kmax;
+        }
+        const int len = kmax - mx0 + 1;
+        const double2* Y = ph + (1 * K1 + abs(my)) * chunk;
+        const double2* Z = ph + (2 * K1 + mz) * chunk;
+        const double2* X = ph + (0 * K1 + (t == 0 ? 1 : kmax)) * chunk;  // q u^1, or q u^kmax to conjugate
+        double2 acc[kLen];
+#pragma unroll
+        for (int k = 0; k < kLen; ++k) acc[k] = make_double2(0.0, 0.0);
+        for (int p = 0; p < chunk; ++p) {
+            const double2 y = my < 0 ? conjd(Y[p]) : Y[p];
+            const double2 x = t == 0 ? X[p] : conjd(X[p]);
+            double2 e = cmul(cmul(x, y), Z[p]);
+            const double2 u = ux[p];
+#pragma unroll
+            for (int k = 0; k < kLen; ++k) {
+                if (k < len) {
+                    acc[k].x += e.x;
+                    acc[k].y += e.y;
+                    e = cmul(e, u);
+                }
+            }
+        }
+        const int h0 = ((mz + kmax) * nk + (my + kmax)) * nk + (mx0 + kmax) - (nh + 1);
+#pragma unroll
+        for (int k = 0; k < kLen; ++k)
+            if (k < len) a.skpart[(size_t)c * nh + h0 + k] = acc[k];
+    }
+}
+
 // S(m) and G(m) = exp(-pi^2 m^2/alpha^2)/m^2 of every half-space m into sk[3 mi ..]
 __device__ inline void ewald_sk_reduce_body(const RecipArgs& a, int kmax, double alpha, int nchunk,
                                             double* sk, int h)
@@ -571,12 +636,10 @@ __device__ inline void ewald_sk_reduce_body(const RecipArgs& a, int kmax, double
 // share of the half space for all 64 (phases by recurrences from the share's
 // first m), and the shares' sums join in wave order, so the result does not
 // depend on scheduling.  Every lane reads the same m at the same time, so the
-// half-space table is staged in LDS when it fits (kStaged) and read from
-// memory otherwise.
+// half-space table is staged in (dynamic) LDS when it fits (kStaged) and read
+// from memory otherwise; the same LDS then carries the reductions.
 constexpr int kSkLds = 2048;   // half-space entries staged in LDS (kmax <= 7)
 constexpr int kKsumWaves = 8;  // shares of the half space per particle
-
-__device__ inline double2 conjd(double2 a) { return make_double2(a.x, -a.y); }
 
 // u^k for an integer k (|k| small): repeated products, u^-k = conj(u)^k on the unit circle
 __device__ inline double2 cpowi(double2 u, int k)
@@ -587,21 +650,30 @@ __device__ inline double2 cpowi(double2 u, int k)
     return e;
 }
 
-template <bool kStaged>
-__device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const double* skg, int blk)
+// Dynamic LDS of ewald_force_body, in doubles: the staged table, then reused
+// for the reductions.
+__host__ __device__ inline int ewald_force_lds(int kmax, bool staged)
 {
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int nk = 2 * kmax + 1, nh = (nk * nk * nk - 1) / 2;
+    const int red = kKsumWaves * 4 * 64;
+    return staged && 3 * nh > red ? 3 * nh : red;
+}
+
+template <bool kStaged>
+__device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const double* skg, int blk,
+                                        double* lds)
+{
+    const int lane = threadIdx.x % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int j = blk * 64 + lane;
     const int nk = 2 * kmax + 1, nh = (nk * nk * nk - 1) / 2;
-    __shared__ double sks[kStaged ? 3 * kSkLds : 1];
-    __shared__ double red[kKsumWaves][4][64];
     if (kStaged) {
-        // entry mi of the half space at sks[3 (mi - nh - 1) ..]
-        for (int t = threadIdx.x; t < 3 * nh; t += blockDim.x) sks[t] = skg[3 * (nh + 1) + t];
+        // entry mi of the half space at lds[3 (mi - nh - 1) ..]
+        for (int t = threadIdx.x; t < 3 * nh; t += blockDim.x) lds[t] = skg[3 * (nh + 1) + t];
         __syncthreads();
     }
     // half-space entry h = mi - nh - 1
-    const double* sk = kStaged ? sks : skg + 3 * (nh + 1);
+    const double* sk = kStaged ? lds : skg + 3 * (nh + 1);
     const int h0 = wave * nh / kKsumWaves, h1 = (wave + 1) * nh / kKsumWaves;
     double fx = 0.0, fy = 0.0, fz = 0.0, pt = 0.0;
     if (j < a.n && h0 < h1) {
@@ -646,17 +718,19 @@ __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const doub
         fy += gi * (double)my;
         fz += gi * (double)mz;
     }
-    red[wave][0][lane] = fx;
-    red[wave][1][lane] = fy;
-    red[wave][2][lane] = fz;
-    red[wave][3][lane] = pt;
+    __syncthreads();  // every wave is done with the staged table
+    double* red = lds;  // [kKsumWaves][4][64]
+    red[(wave * 4 + 0) * 64 + lane] = fx;
+    red[(wave * 4 + 1) * 64 + lane] = fy;
+    red[(wave * 4 + 2) * 64 + lane] = fz;
+    red[(wave * 4 + 3) * 64 + lane] = pt;
     __syncthreads();
     if (wave == 0 && j < a.n) {
         for (int w = 1; w < kKsumWaves; ++w) {
-            fx += red[w][0][lane];
-            fy += red[w][1][lane];
-            fz += red[w][2][lane];
-            pt += red[w][3][lane];
+            fx += red[(w * 4 + 0) * 64 + lane];
+            fy += red[(w * 4 + 1) * 64 + lane];
+            fz += red[(w * 4 + 2) * 64 + lane];
+            pt += red[(w * 4 + 3) * 64 + lane];
         }
         const double V = a.box[0] * a.box[1] * a.box[2];
         if (a.pot) a.pot[j] = 2.0 * a.eps / (kPi * V) * pt;
@@ -666,7 +740,8 @@ __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const doub
         a.frz[j] = pre * fz / a.box[2];
     }
     if (blk == 0) {
-        __shared__ double esum[kKsumWaves * 64];
+        __syncthreads();  // wave 0 has read the force shares
+        double* esum = lds;  // [kKsumWaves * 64]
         double en = 0.0;
         for (int h = threadIdx.x; h < nh; h += blockDim.x) {
             const int mi = nh + 1 + h;
@@ -682,10 +757,23 @@ __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const doub
     }
 }
 
+// the row form for kmax <= 8 (2 kmax + 1 <= 17 accumulators), per-m otherwise
+__device__ inline void ewald_sk_part_any(const RecipArgs& a, int kmax, int chunk, int c,
+                                         double2* dyn)
+{
+    double2* ux = dyn + 3 * (kmax + 1) * chunk;
+    if (kmax <= 6)
+        ewald_sk_rows_body<13>(a, kmax, chunk, c, dyn, ux);
+    else if (kmax <= 8)
+        ewald_sk_rows_body<17>(a, kmax, chunk, c, dyn, ux);
+    else
+        ewald_sk_part_body(a, kmax, chunk, c, dyn);
+}
+
 __global__ __launch_bounds__(256) void ewald_sk_part_kernel(RecipArgs a, int kmax, int chunk)
 {
     extern __shared__ double2 ph_dyn[];
-    ewald_sk_part_body(a, kmax, chunk, blockIdx.x, ph_dyn);
+    ewald_sk_part_any(a, kmax, chunk, blockIdx.x, ph_dyn);
 }
 
 __global__ __launch_bounds__(256) void ewald_sk_reduce_kernel(RecipArgs a, int kmax, double alpha,
@@ -698,7 +786,8 @@ template <bool kStaged>
 __global__ __launch_bounds__(kKsumWaves * 64) void ewald_force_kernel(RecipArgs a, int kmax,
                                                                       const double* sk)
 {
-    ewald_force_body<kStaged>(a, kmax, sk, blockIdx.x);
+    extern __shared__ double kf_dyn[];
+    ewald_force_body<kStaged>(a, kmax, sk, blockIdx.x, kf_dyn);
 }
 
 // batched: blockIdx.y selects the replica (see kernels_md.hip)
@@ -707,7 +796,7 @@ __global__ __launch_bounds__(256) void ewald_sk_part_batch_kernel(const BatchRep
     extern __shared__ double2 ph_dyn[];
     const BatchRep& r = reps[blockIdx.y];
     if (r.recip != TCP_RECIP_EWALD || (int)blockIdx.x >= (r.ra.n + chunk - 1) / chunk) return;
-    ewald_sk_part_body(r.ra, r.kmax, chunk, blockIdx.x, ph_dyn);
+    ewald_sk_part_any(r.ra, r.kmax, chunk, blockIdx.x, ph_dyn);
 }
 
 __global__ __launch_bounds__(256) void ewald_sk_reduce_batch_kernel(const BatchRep* reps, int chunk)
@@ -723,7 +812,8 @@ __global__ __launch_bounds__(kKsumWaves * 64) void ewald_force_batch_kernel(cons
 {
     const BatchRep& r = reps[blockIdx.y];
     if (r.recip != TCP_RECIP_EWALD || (int)blockIdx.x >= (r.ra.n + 63) / 64) return;
-    ewald_force_body<kStaged>(r.ra, r.kmax, r.sk, blockIdx.x);
+    extern __shared__ double kf_dyn[];
+    ewald_force_body<kStaged>(r.ra, r.kmax, r.sk, blockIdx.x, kf_dyn);
 }
 
 }  // namespace
@@ -821,9 +911,9 @@ void launch_fft3d_inverse(const RecipArgs& a, hipStream_t s)
 
 int ewald_chunk(int kmax)
 {
-    // LDS phase table 3 (kmax + 1) chunk double2 within 64 KiB
+    // LDS phase table 3 (kmax + 1) chunk double2 plus ux[chunk] within 64 KiB
     int c = 128;
-    while (c > 16 && (size_t)48 * (kmax + 1) * c > 65536) c /= 2;
+    while (c > 16 && (size_t)(48 * (kmax + 1) + 16) * c > 65536) c /= 2;
     return c;
 }
 
@@ -836,30 +926,34 @@ int ewald_half(int kmax)
 void launch_ewald_kspace(const RecipArgs& a, int kmax, double alpha, double* sk, hipStream_t s)
 {
     const int chunk = ewald_chunk(kmax), nchunk = (a.n + chunk - 1) / chunk;
-    const size_t lds = (size_t)48 * (kmax + 1) * chunk;
+    const size_t lds = (size_t)(48 * (kmax + 1) + 16) * chunk;
     hipLaunchKernelGGL(ewald_sk_part_kernel, dim3(nchunk), dim3(256), lds, s, a, kmax, chunk);
     hipLaunchKernelGGL(ewald_sk_reduce_kernel, dim3((ewald_half(kmax) + 255) / 256), dim3(256), 0,
                        s, a, kmax, alpha, nchunk, sk);
     const dim3 fg((a.n + 63) / 64), fb(kKsumWaves * 64);
-    if (ewald_half(kmax) <= kSkLds)
-        hipLaunchKernelGGL(ewald_force_kernel<true>, fg, fb, 0, s, a, kmax, (const double*)sk);
+    const bool staged = ewald_half(kmax) <= kSkLds;
+    const size_t flds = sizeof(double) * ewald_force_lds(kmax, staged);
+    if (staged)
+        hipLaunchKernelGGL(ewald_force_kernel<true>, fg, fb, flds, s, a, kmax, (const double*)sk);
     else
-        hipLaunchKernelGGL(ewald_force_kernel<false>, fg, fb, 0, s, a, kmax, (const double*)sk);
+        hipLaunchKernelGGL(ewald_force_kernel<false>, fg, fb, flds, s, a, kmax, (const double*)sk);
 }
 
 void launch_ewald_batch(const BatchRep* reps, int nrep, int kmax, int maxn, hipStream_t s)
 {
     const int chunk = ewald_chunk(kmax);
-    const size_t lds = (size_t)48 * (kmax + 1) * chunk;
+    const size_t lds = (size_t)(48 * (kmax + 1) + 16) * chunk;
     hipLaunchKernelGGL(ewald_sk_part_batch_kernel, dim3((maxn + chunk - 1) / chunk, nrep), dim3(256),
                        lds, s, reps, chunk);
     hipLaunchKernelGGL(ewald_sk_reduce_batch_kernel, dim3((ewald_half(kmax) + 255) / 256, nrep),
                        dim3(256), 0, s, reps, chunk);
     const dim3 fg((maxn + 63) / 64, nrep), fb(kKsumWaves * 64);
-    if (ewald_half(kmax) <= kSkLds)
-        hipLaunchKernelGGL(ewald_force_batch_kernel<true>, fg, fb, 0, s, reps);
+    const bool staged = ewald_half(kmax) <= kSkLds;
+    const size_t flds = sizeof(double) * ewald_force_lds(kmax, staged);
+    if (staged)
+        hipLaunchKernelGGL(ewald_force_batch_kernel<true>, fg, fb, flds, s, reps);
     else
-        hipLaunchKernelGGL(ewald_force_batch_kernel<false>, fg, fb, 0, s, reps);
+        hipLaunchKernelGGL(ewald_force_batch_kernel<false>, fg, fb, flds, s, reps);
     launch_um_batch(reps, nrep, s);
 }
 
