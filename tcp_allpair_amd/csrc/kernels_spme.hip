@@ -686,37 +686,37 @@ __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const doub
         double2 ez = cpowi(uz, mz);
         double2 eyz = cmul(cpowi(uy, my), ez);
         double2 e = cmul(eyz, cpowi(ux, mx));
-        double gx = 0.0, gi = 0.0;  // the current row (my, mz)
-        for (int h = h0; h < h1; ++h) {
-            const double Sr = sk[3 * h], Si = sk[3 * h + 1], G = sk[3 * h + 2];
-            const double im = G * (Sr * e.y - Si * e.x);  // G Im(conj(S) e)
-            gx += im * (double)mx;
-            gi += im;
-            pt += G * (Sr * e.x + Si * e.y);               // G Re(conj(S) e)
-            if (mx < kmax) {
-                ++mx;
+        int h = h0;
+        for (;;) {
+            // the share's part of row (my, mz): mx up to kmax, branch-free
+            const int hend = min(h1, h + kmax - mx + 1);
+            double gx = 0.0, gi = 0.0, mxd = (double)mx;
+            for (; h < hend; ++h) {
+                const double Sr = sk[3 * h], Si = sk[3 * h + 1], G = sk[3 * h + 2];
+                const double im = G * (Sr * e.y - Si * e.x);  // G Im(conj(S) e)
+                gx = fma(im, mxd, gx);
+                gi += im;
+                pt = fma(G, fma(Sr, e.x, Si * e.y), pt);      // G Re(conj(S) e)
                 e = cmul(e, ux);
-            } else {  // end of the row: its sums, then the next row's first m
-                fx += gx;
-                fy += gi * (double)my;
-                fz += gi * (double)mz;
-                gx = gi = 0.0;
-                mx = -kmax;
-                if (my < kmax) {
-                    ++my;
-                    eyz = cmul(eyz, uy);
-                } else {
-                    my = -kmax;
-                    ++mz;
-                    ez = cmul(ez, uz);
-                    eyz = cmul(yk, ez);
-                }
-                e = cmul(eyz, xk);
+                mxd += 1.0;
             }
+            fx += gx;
+            fy += gi * (double)my;
+            fz += gi * (double)mz;
+            if (h >= h1) break;
+            // the next row's first m
+            mx = -kmax;
+            if (my < kmax) {
+                ++my;
+                eyz = cmul(eyz, uy);
+            } else {
+                my = -kmax;
+                ++mz;
+                ez = cmul(ez, uz);
+                eyz = cmul(yk, ez);
+            }
+            e = cmul(eyz, xk);
         }
-        fx += gx;
-        fy += gi * (double)my;
-        fz += gi * (double)mz;
     }
     __syncthreads();  // every wave is done with the staged table
     double* red = lds;  // [kKsumWaves][4][64]
