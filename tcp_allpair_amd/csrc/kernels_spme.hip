@@ -508,6 +508,14 @@ __device__ inline void decode_m(int mi, int kmax, int* mx, int* my, int* mz)
     *mz = mi / (nk * nk) - kmax;
 }
 
+// half-space rows (my, mz) of the row form, and how many groups of that many
+// threads a 256-thread block holds (1 where the row form is not used)
+__host__ __device__ inline int ewald_sk_rows(int kmax) { return 1 + kmax + kmax * (2 * kmax + 1); }
+__host__ __device__ inline int ewald_sk_groups(int kmax)
+{
+    return kmax <= 8 ? 256 / ewald_sk_rows(kmax) : 1;
+}
+
 // ph: LDS table [3][kmax + 1][chunk] of q^(d == 0) e^{i m theta_d}, m = 0..kmax
 __device__ inline void ewald_sk_part_body(const RecipArgs& a, int kmax, int chunk, int c,
                                           double2* ph)
@@ -544,12 +552,15 @@ __device__ inline void ewald_sk_part_body(const RecipArgs& a, int kmax, int chun
     }
 }
 
-// The same partials by rows: thread t owns the half-space row (my, mz) t and
-// walks its mx (up to kLen = 2 kmax + 1 of them) by the recurrence
-// e <- e e^{i theta_x}, so a particle costs each row two complex products to
-// start and one per m, and the row's S(m) partials stay in registers.  Row 0 is
-// (my, mz) = (0, 0) with mx = 1..kmax; rows 1..kmax are mz = 0, my = 1..kmax;
-// then mz = 1..kmax with my = -kmax..kmax.  ux: e^{i theta_x} of the chunk.
+// The same partials by rows: a row is the half-space line (my, mz), walked
+// along mx (up to kLen = 2 kmax + 1 values) by the recurrence e <- e e^{i theta_x},
+// so a particle costs a row two complex products to start and one per m, and
+// the row's S(m) partials stay in registers.  Row 0 is (my, mz) = (0, 0) with
+// mx = 1..kmax; rows 1..kmax are mz = 0, my = 1..kmax; then mz = 1..kmax with
+// my = -kmax..kmax.  The block's threads form ewald_sk_groups(kmax) groups of
+// `rows` threads; group g walks its own contiguous share of the chunk's
+// particles, and the groups' partials are joined in LDS in group order, so the
+// sum is fixed by kmax and chunk.  ux: e^{i theta_x} of the chunk.
 template <int kLen>
 __device__ inline void ewald_sk_rows_body(const RecipArgs& a, int kmax, int chunk, int c,
                                           double2* ph, double2* ux)
@@ -567,26 +578,29 @@ __device__ inline void ewald_sk_rows_body(const RecipArgs& a, int kmax, int chun
         }
     }
     __syncthreads();
-    const int rows = 1 + kmax + kmax * nk;
-    for (int t = threadIdx.x; t < rows; t += blockDim.x) {
-        int my, mz, mx0;
-        if (t <= kmax) {
-            mz = 0;
-            my = t;
-            mx0 = t == 0 ? 1 : -kmax;
-        } else {
-            mz = 1 + (t - kmax - 1) / nk;
-            my = (t - kmax - 1) % nk - kmax;
-            mx0 = -kmax;
-        }
-        const int len = kmax - mx0 + 1;
+    const int rows = ewald_sk_rows(kmax), groups = ewald_sk_groups(kmax);
+    const int g = threadIdx.x / rows, t = threadIdx.x - g * rows;
+    const bool live = g < groups;
+    int my, mz, mx0;
+    if (t <= kmax) {
+        mz = 0;
+        my = t;
+        mx0 = t == 0 ? 1 : -kmax;
+    } else {
+        mz = 1 + (t - kmax - 1) / nk;
+        my = (t - kmax - 1) % nk - kmax;
+        mx0 = -kmax;
+    }
+    const int len = kmax - mx0 + 1;
+    double2 acc[kLen];
+#pragma unroll
+    for (int k = 0; k < kLen; ++k) acc[k] = make_double2(0.0, 0.0);
+    if (live) {
         const double2* Y = ph + (1 * K1 + abs(my)) * chunk;
         const double2* Z = ph + (2 * K1 + mz) * chunk;
         const double2* X = ph + (0 * K1 + (t == 0 ? 1 : kmax)) * chunk;  // q u^1, or q u^kmax to conjugate
-        double2 acc[kLen];
-#pragma unroll
-        for (int k = 0; k < kLen; ++k) acc[k] = make_double2(0.0, 0.0);
-        for (int p = 0; p < chunk; ++p) {
+        const int p1 = (g + 1) * chunk / groups;
+        for (int p = g * chunk / groups; p < p1; ++p) {
             const double2 y = my < 0 ? conjd(Y[p]) : Y[p];
             const double2 x = t == 0 ? X[p] : conjd(X[p]);
             double2 e = cmul(cmul(x, y), Z[p]);
@@ -600,7 +614,30 @@ __device__ inline void ewald_sk_rows_body(const RecipArgs& a, int kmax, int chun
                 }
             }
         }
-        const int h0 = ((mz + kmax) * nk + (my + kmax)) * nk + (mx0 + kmax) - (nh + 1);
+    }
+    const int h0 = ((mz + kmax) * nk + (my + kmax)) * nk + (mx0 + kmax) - (nh + 1);
+    if (groups > 1) {
+        // groups 1.. park their partials over the phase table, group 0 adds them in order
+        __syncthreads();
+        if (live && g > 0) {
+#pragma unroll
+            for (int k = 0; k < kLen; ++k)
+                if (k < len) ph[(size_t)(g - 1) * nh + h0 + k] = acc[k];
+        }
+        __syncthreads();
+        if (g == 0) {
+            for (int o = 0; o < groups - 1; ++o) {
+#pragma unroll
+                for (int k = 0; k < kLen; ++k)
+                    if (k < len) {
+                        const double2 v = ph[(size_t)o * nh + h0 + k];
+                        acc[k].x += v.x;
+                        acc[k].y += v.y;
+                    }
+            }
+        }
+    }
+    if (g == 0) {
 #pragma unroll
         for (int k = 0; k < kLen; ++k)
             if (k < len) a.skpart[(size_t)c * nh + h0 + k] = acc[k];
@@ -917,6 +954,15 @@ int ewald_chunk(int kmax)
     return c;
 }
 
+// dynamic LDS of the partial kernels: the phase table, or the parked group
+// partials of the row form when they need more
+static size_t ewald_part_lds(int kmax, int chunk)
+{
+    const size_t table = (size_t)(48 * (kmax + 1) + 16) * chunk;
+    const size_t parked = (size_t)(ewald_sk_groups(kmax) - 1) * ewald_half(kmax) * sizeof(double2);
+    return table > parked ? table : parked;
+}
+
 int ewald_half(int kmax)
 {
     const int nk = 2 * kmax + 1;
@@ -926,7 +972,7 @@ int ewald_half(int kmax)
 void launch_ewald_kspace(const RecipArgs& a, int kmax, double alpha, double* sk, hipStream_t s)
 {
     const int chunk = ewald_chunk(kmax), nchunk = (a.n + chunk - 1) / chunk;
-    const size_t lds = (size_t)(48 * (kmax + 1) + 16) * chunk;
+    const size_t lds = ewald_part_lds(kmax, chunk);
     hipLaunchKernelGGL(ewald_sk_part_kernel, dim3(nchunk), dim3(256), lds, s, a, kmax, chunk);
     hipLaunchKernelGGL(ewald_sk_reduce_kernel, dim3((ewald_half(kmax) + 255) / 256), dim3(256), 0,
                        s, a, kmax, alpha, nchunk, sk);
@@ -942,7 +988,7 @@ void launch_ewald_kspace(const RecipArgs& a, int kmax, double alpha, double* sk,
 void launch_ewald_batch(const BatchRep* reps, int nrep, int kmax, int maxn, hipStream_t s)
 {
     const int chunk = ewald_chunk(kmax);
-    const size_t lds = (size_t)(48 * (kmax + 1) + 16) * chunk;
+    const size_t lds = ewald_part_lds(kmax, chunk);
     hipLaunchKernelGGL(ewald_sk_part_batch_kernel, dim3((maxn + chunk - 1) / chunk, nrep), dim3(256),
                        lds, s, reps, chunk);
     hipLaunchKernelGGL(ewald_sk_reduce_batch_kernel, dim3((ewald_half(kmax) + 255) / 256, nrep),
