@@ -63,7 +63,7 @@ def stats(src: str, out: str, note: str = "", kind: str = "step", trace: str = "
     if kind == "batch":
         head = ["# Kernel time summary: 64-replica batch, mixed precision, direct k-sum kmax 6\n\n",
                 "Workload: tcp_allpair_amd/batchbench.py (64 replicas, 526..16238 particles, 279616 in "
-                "all); the autotuner times each of the 4 block shapes, then 1 warm-up and 5 timed "
+                "all); the autotuner times each of the 6 block shapes, then 1 warm-up and 5 timed "
                 "batch steps; the same replicas then run one context after another (the "
                 "single-context kernels below).\n\n", BATCH_CMD]
     else:
