@@ -292,6 +292,50 @@ def test_md_steps_with_spme_vs_oracle():
     assert rel_err(fg, fo).max() <= 1e-8
 
 
+def test_isokin_32k_spme64_vs_oracle():
+    """16384 + 16384 particles at reduced density 1 (L = 32 A), SPME 64^3 order 4 with
+    alpha = 3.1/rc, FP64, isokinetic at T* = 1, dt = 0.005 tau*.  The step-0 forces and
+    energies and three steps against the oracle (OpenMP over the host's cores), then
+    1000 steps on the device holding the isokinetic constraint sum m v^2 = 3 N_s T_s
+    (VVerletIsokin2, vverlet.f90:319-368) at every step."""
+    say("case isokin 32k spme64")
+    p, x, v = make_plasma(16384, 16384, seed=7, K=(64, 64, 64), order=4)
+    dt = reduced_dt(0.005)
+    fo, uro, rdfo = cpu.force_allpair(p, x)
+    fro, umo = cpu.spme(p, x)
+    f0 = cpu.remove_rigidmtn(p, fo + fro)
+    with Context(p, FP64) as c:
+        c.set_state(x=x, f=np.zeros_like(x))
+        ur = c.force_direct()
+        um = c.force_recip()
+        c.remove_rigidmtn()
+        f = c.get_state()[2]
+        rdf, _ = c.histograms(reset=True)
+    say("step 0: Ur", ur, "oracle", uro, "Um", um, "oracle", umo)
+    assert abs(ur - uro) <= 1e-12 * abs(uro)
+    assert abs(um - umo) <= 1e-12 * abs(umo)
+    assert rel_err(f, f0).max() <= 1e-10
+    np.testing.assert_array_equal(rdf, rdfo)
+
+    xo, vo, fs = x.copy(), v.copy(), f0.copy()
+    rows = []
+    for _ in range(3):
+        r = cpu.step(p, ISOKIN, dt, xo, vo, fs, fast=True)
+        rows.append([r["Ur"], r["Um"], r["Epot"], r["mv2el"], r["mv2ion"]])
+    with Context(p, FP64) as c:
+        c.set_state(x=x, v=v, f=f0)
+        tr = c.md_steps(3, ISOKIN, dt)
+        xg, _, fg = c.get_state()
+        np.testing.assert_allclose(tr[:, [0, 2, 3, 4, 5]], np.array(rows), rtol=1e-10)
+        assert rel_err(fg, fs).max() <= 1e-8
+        np.testing.assert_allclose(xg, xo, rtol=0, atol=1e-10)
+        tr = c.md_steps(1000, ISOKIN, dt)
+    say("1000 steps: Epot", tr[0, 3], "->", tr[-1, 3])
+    assert np.isfinite(tr).all()
+    np.testing.assert_allclose(tr[:, 4], 3 * p.ne * p.Te, rtol=1e-12)
+    np.testing.assert_allclose(tr[:, 5], 3 * p.ni * p.Ti, rtol=1e-12)
+
+
 @pytest.mark.parametrize("recip,order", [(RECIP_NONE, 4), (RECIP_SPME, 4), (RECIP_SPME, 8),
                                          (RECIP_EWALD, 4)])
 def test_fp64_deterministic(recip, order):
