@@ -109,6 +109,26 @@ int balloc(tcp_batch* b, T** p, size_t count)
     return 0;
 }
 
+// Grows a batch buffer to at least count elements, freeing the old one once
+// the stream has drained (contents are not kept).  The captured step refers to
+// the old address, so it is dropped and captured again on the next steps.
+template <typename T>
+int bgrow(tcp_batch* b, T** p, size_t count)
+{
+    if (*p) {
+        BHIP(b, hipStreamSynchronize(b->stream));
+        if (b->exec) {
+            BHIP(b, hipGraphExecDestroy(b->exec));
+            b->exec = nullptr;
+        }
+        auto it = std::find(b->allocs.begin(), b->allocs.end(), (void*)*p);
+        if (it != b->allocs.end()) b->allocs.erase(it);
+        BHIP(b, hipFree(*p));
+        *p = nullptr;
+    }
+    return balloc(b, p, count);
+}
+
 bool mixed(const tcp_batch* b) { return b->precision == TCP_MIXED; }
 
 int slots_of(const tcp_batch* b, const tcp_ctx* c, int shape)
@@ -145,7 +165,7 @@ int set_shape(tcp_batch* b, int shape)
 {
     const std::vector<PairItem> items = make_items(b, shape);
     if ((int)items.size() > b->item_cap) {
-        BTRY(balloc(b, &b->d_items, items.size()));
+        BTRY(bgrow(b, &b->d_items, items.size()));
         b->item_cap = (int)items.size();
     }
     BHIP(b, hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(PairItem),
@@ -407,7 +427,7 @@ int tcp_batch_md_steps(tcp_batch* b, int nsteps, int thermo, double dt, const do
     BHIP(b, hipMemcpyAsync(b->d_reps, b->h_reps.data(), b->nrep * sizeof(BatchRep),
                            hipMemcpyHostToDevice, b->stream));
     if (trace && (size_t)nsteps * b->nrep > (size_t)b->trace_cap) {
-        BTRY(balloc(b, &b->d_trace, (size_t)6 * nsteps * b->nrep));
+        BTRY(bgrow(b, &b->d_trace, (size_t)6 * nsteps * b->nrep));
         b->trace_cap = nsteps * b->nrep;
     }
     const uint64_t ctr[2] = {b->counter, 0};

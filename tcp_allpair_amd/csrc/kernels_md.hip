@@ -51,7 +51,7 @@ __device__ inline void vdf_count(unsigned* h, int s, double mv2, double dv)
 {
 #pragma clang fp contract(off)
     const double t = mv2 / 3.0 / dv;
-    if (t < (double)kNbin) atomicAdd(&h[s * kHistStride + (int)t], 1u);
+    if (t >= 0.0 && t < (double)kNbin) atomicAdd(&h[s * kHistStride + (int)t], 1u);
 }
 
 // fractional coordinate in [0, 1] -> 32-bit fixed point (modulo 2^32) and the

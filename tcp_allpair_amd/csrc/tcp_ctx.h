@@ -29,7 +29,8 @@ struct tcp_ctx {
     int2* blkmap = nullptr;
     int* n3_queue = nullptr;  // work queue of the persistent Newton kernel
     int n3_grid = 1;          // its resident blocks
-    size_t pf_cap = 0;  // doubles
+    size_t pf_cap = 0;    // doubles
+    size_t part_cap = 0;  // doubles of part
     double *pf = nullptr, *pu = nullptr;
     // reciprocal part
     double *Q = nullptr, *C = nullptr, *gb = nullptr, *epart = nullptr, *sk = nullptr;

@@ -5,6 +5,7 @@
 // for the mixed pair kernel, per-(chunk, particle) FP64 pair partials, the
 // K1*K2*K3 SPME grids, and a small block of device scalars so that a whole MD
 // step runs without a host round trip.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -57,6 +58,26 @@ int dalloc(tcp_ctx* c, T** p, size_t count)
         int rc_ = (expr);          \
         if (rc_) return rc_;       \
     } while (0)
+
+// Grows a buffer of the context to at least count elements: the old one is
+// freed (after the stream has drained, since queued kernels may read it) and
+// its contents are not kept.  *cap counts elements.
+template <typename T>
+int dgrow(tcp_ctx* c, T** p, size_t* cap, size_t count)
+{
+    if (*p && count <= *cap) return 0;
+    if (*p) {
+        HIPCK(c, hipStreamSynchronize(c->stream));
+        auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)*p);
+        if (it != c->allocs.end()) c->allocs.erase(it);
+        HIPCK(c, hipFree(*p));
+        *p = nullptr;
+        *cap = 0;
+    }
+    TRY(dalloc(c, p, count));
+    *cap = count;
+    return 0;
+}
 
 bool pow2(int k) { return k >= 2 && (k & (k - 1)) == 0; }
 
@@ -253,8 +274,13 @@ int ensure_ext(tcp_ctx* c, int ng)
 {
     if (ng <= c->g_cap) return 0;
     double** arrs[6] = {&c->gx, &c->gy, &c->gz, &c->gfx, &c->gfy, &c->gfz};
-    for (auto a : arrs) TRY(dalloc(c, a, (size_t)ng));
-    TRY(dalloc(c, &c->gupos, (size_t)ng));
+    size_t cap = c->g_cap;
+    for (auto a : arrs) {
+        cap = c->g_cap;
+        TRY(dgrow(c, a, &cap, (size_t)ng));
+    }
+    cap = c->g_cap;
+    TRY(dgrow(c, &c->gupos, &cap, (size_t)ng));
     c->g_cap = ng;
     return 0;
 }
@@ -425,8 +451,15 @@ int validate_params(const tcp_params* p, int precision, std::string* msg)
         *m = text;
         return code;
     };
-    if (p->ne < 0 || p->ni < 0 || p->ne + p->ni < 2)
+    if (p->ne < 0 || p->ni < 0 || (int64_t)p->ne + p->ni < 2)
         return vfail(msg, TCP_ERR_ARG, "need at least two particles");
+    // the block-row partials take N^2/16 bytes (275 GB at 2^21 particles)
+    if ((int64_t)p->ne + p->ni > kMaxParticles)
+        return vfail(msg, TCP_ERR_ARG, "at most 2^22 particles per context");
+    if (!(p->xm[0] > 0) || !(p->xm[1] > 0))
+        return vfail(msg, TCP_ERR_ARG, "masses must be positive");
+    if (!(p->dv_e >= 0) || !(p->dv_i >= 0))
+        return vfail(msg, TCP_ERR_ARG, "VDF bin widths must not be negative");
     if (p->flags & ~(TCP_KELBG | TCP_REG_EE))
         return vfail(msg, TCP_ERR_ARG, "unknown regularisation flags");
     if (precision != TCP_FP64 && precision != TCP_MIXED)
@@ -526,7 +559,7 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
     c->pf_cap = (size_t)slots * 3 * n;
     if ((rc = dalloc(c, &c->pf, c->pf_cap))) return bail(rc);
     if ((rc = dalloc(c, &c->pu, (size_t)slots * n))) return bail(rc);
-    if ((rc = dalloc(c, &c->part, (size_t)4 * md_blocks(c->N)))) return bail(rc);
+    if ((rc = dgrow(c, &c->part, &c->part_cap, (size_t)4 * md_blocks(c->N)))) return bail(rc);
     if ((rc = dalloc(c, &c->sc, (size_t)SC_N))) return bail(rc);
     if ((rc = dalloc(c, &c->rdf, (size_t)3 * kHistStride))) return bail(rc);
     if ((rc = dalloc(c, &c->vdf, (size_t)2 * kHistStride))) return bail(rc);
@@ -655,9 +688,9 @@ int tcp_force_direct_ext(tcp_ctx* c, const double* xg, int ne_g, int ni_g, doubl
     choose_chunks(c->N, ng, &nc, &jc);
     const size_t need = (size_t)nc * 3 * c->N;
     if (need > c->pf_cap) {
-        TRY(dalloc(c, &c->pf, need));
-        TRY(dalloc(c, &c->pu, (size_t)nc * c->N));
-        c->pf_cap = need;
+        size_t ucap = c->pf_cap / 3;
+        TRY(dgrow(c, &c->pu, &ucap, (size_t)nc * c->N));
+        TRY(dgrow(c, &c->pf, &c->pf_cap, need));
     }
     PairArgs a = pair_args(c);
     a.pi = PosSoA{c->x, c->y, c->z};
@@ -682,9 +715,9 @@ int tcp_force_direct_ext(tcp_ctx* c, const double* xg, int ne_g, int ni_g, doubl
     choose_chunks(ng, c->N, &nc2, &jc2);
     const size_t need2 = (size_t)nc2 * 3 * ng;
     if (need2 > c->gpf_cap) {
-        TRY(dalloc(c, &c->gpf, need2));
-        TRY(dalloc(c, &c->gpu, (size_t)nc2 * ng));
-        c->gpf_cap = need2;
+        size_t ucap = c->gpf_cap / 3;
+        TRY(dgrow(c, &c->gpu, &ucap, (size_t)nc2 * ng));
+        TRY(dgrow(c, &c->gpf, &c->gpf_cap, need2));
     }
     PairArgs b = a;
     b.pi = a.pj; b.pj = a.pi;
@@ -697,10 +730,8 @@ int tcp_force_direct_ext(tcp_ctx* c, const double* xg, int ne_g, int ni_g, doubl
     b.pu = c->gpu;
     launch_pair(c, b, nc2);
     LAUNCHCK(c);
-    // part is large enough for md_blocks(max(N, ng)) only if ng <= N; use a dedicated reduction
-    if (md_blocks(ng) > md_blocks(c->N)) {
-        TRY(dalloc(c, &c->part, (size_t)4 * md_blocks(ng)));
-    }
+    // the block partials of the external block's combine: 4 per block of ng
+    TRY(dgrow(c, &c->part, &c->part_cap, (size_t)4 * md_blocks(ng)));
     launch_combine(mg, c->gpf, c->gpu, nc2, nullptr, nullptr, nullptr, c->part, false, c->stream);
     LAUNCHCK(c);
     if (gemit) TRY(download3(c, gemit, c->gfx, c->gfy, c->gfz, ng));
@@ -817,7 +848,8 @@ int tcp_md_steps(tcp_ctx* c, int nsteps, int thermo, double dt, double Te, doubl
     if (!c || nsteps < 0 || thermo < TCP_NOTEMP || thermo > TCP_STOCH) return TCP_ERR_ARG;
     if (nsteps == 0) return 0;
     if (trace && nsteps > c->trace_cap) {
-        TRY(dalloc(c, &c->trace, (size_t)6 * nsteps));
+        size_t cap = (size_t)6 * c->trace_cap;
+        TRY(dgrow(c, &c->trace, &cap, (size_t)6 * nsteps));
         c->trace_cap = nsteps;
     }
     const bool rec = c->p.recip != TCP_RECIP_NONE;

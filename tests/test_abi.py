@@ -75,6 +75,9 @@ def test_cutoff_error_before_device_use():
     (dict(ne=1, ni=0), -1),
     (dict(rs=0.0), -1),
     (dict(flags=4), -1),             # unknown regularisation flag
+    (dict(xm=(1.0, 0.0)), -1),       # masses divide the kicks
+    (dict(Te=-1.0), -1),             # negative VDF bin width (4 Te / 500)
+    (dict(ne=1 << 22, ni=1), -1),    # beyond the per-context particle limit
 ])
 def test_parameter_errors(change, code):
     p, _, _ = make_plasma(10, 10)

@@ -119,6 +119,29 @@ def test_direct_ext_vs_reference():
     np.testing.assert_array_equal(rdf, d["out_rdf"])
 
 
+def test_direct_ext_growing_and_shrinking_blocks():
+    """Force_direct_ext on one context with external blocks larger than the own
+    set, then smaller, then larger again (the device buffers of the external
+    block are regrown and reused between calls), each call against the oracle."""
+    say("case direct_ext sizes")
+    p, x, _ = make_plasma(1500, 1200, seed=71, recip=RECIP_NONE)
+    ne_o, ni_o = 300, 200
+    own = np.concatenate([x[:ne_o], x[p.ne:p.ne + ni_o]])
+    po = p.replace(ne=ne_o, ni=ni_o)
+    with Context(po, FP64) as c:
+        for ne_g, ni_g in [(400, 300), (100, 50), (1200, 1000), (400, 300)]:
+            xg = np.concatenate([x[ne_o:ne_o + ne_g], x[p.ne + ni_o:p.ne + ni_o + ni_g]])
+            c.set_state(x=own, f=np.zeros_like(own))
+            gemit, ur = c.force_direct_ext(xg, ne_g, ni_g)
+            f = c.get_state()[2]
+            fo, go, uo, _ = cpu.force_direct_ext(po, ne_o, ni_o, own, np.zeros_like(own),
+                                                 ne_g, ni_g, xg)
+            say("external block", ne_g, ni_g)
+            assert abs(ur - uo) <= 1e-12 * abs(uo)
+            assert rel_err(f, fo).max() <= 1e-10
+            assert rel_err(gemit, go).max() <= 1e-10
+
+
 def test_remove_rigidmtn_vs_reference():
     say("case rigid")
     p, d = load("rigid")
