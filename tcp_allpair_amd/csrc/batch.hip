@@ -51,7 +51,7 @@ struct tcp_batch {
     uint64_t* d_counter = nullptr;  // [0] noise step counter, [1] trace row
     unsigned long long* d_rdf_scratch = nullptr;
     double* d_trace = nullptr;
-    int trace_cap = 0;
+    size_t trace_cap = 0;  // trace rows (steps x replicas)
     // persistent launch shape
     int shape = 0;
     int grid = 1;
@@ -261,6 +261,7 @@ int tcp_batch_destroy(tcp_batch* b)
 int tcp_batch_create(tcp_batch** out, int nrep, const tcp_params* p, int precision)
 {
     if (!out || !p || nrep < 1) return bfail(nullptr, TCP_ERR_ARG, "need an output pointer and nrep >= 1 parameter sets");
+    if (nrep > 65535) return bfail(nullptr, TCP_ERR_ARG, "at most 65535 replicas (one grid row each)");
     *out = nullptr;
     for (int r = 1; r < nrep; ++r)
         if (p[r].recip != p[0].recip || p[r].flags != p[0].flags ||
@@ -426,9 +427,9 @@ int tcp_batch_md_steps(tcp_batch* b, int nsteps, int thermo, double dt, const do
     }
     BHIP(b, hipMemcpyAsync(b->d_reps, b->h_reps.data(), b->nrep * sizeof(BatchRep),
                            hipMemcpyHostToDevice, b->stream));
-    if (trace && (size_t)nsteps * b->nrep > (size_t)b->trace_cap) {
+    if (trace && (size_t)nsteps * b->nrep > b->trace_cap) {
         BTRY(bgrow(b, &b->d_trace, (size_t)6 * nsteps * b->nrep));
-        b->trace_cap = nsteps * b->nrep;
+        b->trace_cap = (size_t)nsteps * b->nrep;
     }
     const uint64_t ctr[2] = {b->counter, 0};
     BHIP(b, hipMemcpyAsync(b->d_counter, ctr, sizeof(ctr), hipMemcpyHostToDevice, b->stream));

@@ -103,6 +103,9 @@ def test_batch_argument_errors_before_device_use():
     p, _, _ = make_plasma(10, 10, recip=2, kmax=3)
     rc, msg, h = _batch_create([p], nrep=0)
     assert rc == -1 and msg and not h.value
+    # the replica count is checked before any parameter set is read
+    rc, msg, h = _batch_create([p], nrep=70000)
+    assert rc == -1 and "65535" in msg and not h.value
     rc, msg, h = _batch_create([p, p.replace(kmax=4)])
     assert rc == -1 and "share" in msg and not h.value
     rc, msg, h = _batch_create([p, p.replace(flags=1)])
