@@ -1386,6 +1386,27 @@ __global__ __launch_bounds__(kN3Waves * kTile, 4) void pair_mixed_n3_kernel(Pair
     queue_done(queue);
 }
 
+// The same over blocks of 4 x 4 tiles by four waves (one i tile each), for
+// small systems, whose 8 x 8 tile blocks would leave most of the chip idle.
+template <bool kRdf, bool kKelbg>
+__global__ __launch_bounds__(kN3Waves * kTile, 4) void pair_mixed_n3_fine_kernel(PairArgs a,
+                                                                              const int2* blkmap,
+                                                                              int nblk, int* queue)
+{
+    constexpr int kBT = 4;
+    __shared__ FixPos jpos[kBT * kTile];
+    __shared__ double racc[4][kBT * kTile];
+    __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
+    __shared__ int slot;
+    for (;;) {
+        const int it = queue_next(queue, &slot);
+        if (it >= nblk) break;
+        const int2 rc = blkmap[it];
+        n3_block<kRdf, kKelbg, kBT, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist);
+    }
+    queue_done(queue);
+}
+
 // ------------------------------------------------ persistent (batches) ---
 //
 // A fixed grid of resident blocks takes work items (replica, block pair) from
@@ -1439,18 +1460,21 @@ void launch_adjoint_reduce(const double* part, int nchunk, int n, double* out, h
                        part, nchunk, n, out);
 }
 
-int n3_block_rows(int n)
+namespace {
+
+template <bool kRdf, bool kKelbg>
+const void* mixed_n3_fn(int shape)
 {
-    const int ntiles = (n + kTile - 1) / kTile;
-    return (ntiles + kBlkTiles - 1) / kBlkTiles;
+    return shape == kFineN3Shape ? (const void*)pair_mixed_n3_fine_kernel<kRdf, kKelbg>
+                                 : (const void*)pair_mixed_n3_kernel<kRdf, kKelbg>;
 }
 
-int pair_n3_occupancy(bool rdf, bool kelbg)
+}  // namespace
+
+int pair_n3_occupancy(bool rdf, bool kelbg, int shape)
 {
-    const void* f = rdf ? (kelbg ? (const void*)pair_mixed_n3_kernel<true, true>
-                                 : (const void*)pair_mixed_n3_kernel<true, false>)
-                        : (kelbg ? (const void*)pair_mixed_n3_kernel<false, true>
-                                 : (const void*)pair_mixed_n3_kernel<false, false>);
+    const void* f = rdf ? (kelbg ? mixed_n3_fn<true, true>(shape) : mixed_n3_fn<true, false>(shape))
+                        : (kelbg ? mixed_n3_fn<false, true>(shape) : mixed_n3_fn<false, false>(shape));
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kN3Waves * kTile, 0) != hipSuccess) nb = 1;
     return nb < 1 ? 1 : nb;
@@ -1482,10 +1506,20 @@ void launch_pair_f64_n3(const PairArgs& a, const int2* blkmap, int nblk, int* qu
 }
 
 void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
-                          hipStream_t s)
+                          int shape, hipStream_t s)
 {
     const dim3 g(grid), b(kN3Waves * kTile);
     const bool kel = (a.flags & TCP_KELBG) != 0;
+    if (shape == kFineN3Shape) {
+        if (a.rdf_weight) {
+            if (kel) hipLaunchKernelGGL((pair_mixed_n3_fine_kernel<true, true>), g, b, 0, s, a, blkmap, nblk, queue);
+            else hipLaunchKernelGGL((pair_mixed_n3_fine_kernel<true, false>), g, b, 0, s, a, blkmap, nblk, queue);
+        } else {
+            if (kel) hipLaunchKernelGGL((pair_mixed_n3_fine_kernel<false, true>), g, b, 0, s, a, blkmap, nblk, queue);
+            else hipLaunchKernelGGL((pair_mixed_n3_fine_kernel<false, false>), g, b, 0, s, a, blkmap, nblk, queue);
+        }
+        return;
+    }
     if (a.rdf_weight) {
         if (kel) hipLaunchKernelGGL((pair_mixed_n3_kernel<true, true>), g, b, 0, s, a, blkmap, nblk, queue);
         else hipLaunchKernelGGL((pair_mixed_n3_kernel<true, false>), g, b, 0, s, a, blkmap, nblk, queue);

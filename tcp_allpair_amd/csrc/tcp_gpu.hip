@@ -197,7 +197,8 @@ int direct_all(tcp_ctx* c, int rdf_weight, const double* frx, const double* fry,
     a.pu = c->pu;
     if (ev_slot >= 0) HIPCK(c, hipEventRecord(c->ev_pair[0][ev_slot], c->stream));
     if (c->precision == TCP_MIXED)
-        launch_pair_mixed_n3(a, c->blkmap, c->n3_blocks, c->n3_queue, c->n3_grid, c->stream);
+        launch_pair_mixed_n3(a, c->blkmap, c->n3_blocks, c->n3_queue, c->n3_grid, c->n3_shape,
+                             c->stream);
     else
         launch_pair_f64_n3(a, c->blkmap, c->n3_blocks, c->n3_queue, c->n3_grid, c->stream);
     LAUNCHCK(c);
@@ -536,8 +537,22 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
     choose_chunks(c->N, c->N, &c->nchunk, &c->jchunk);
     int slots = c->nchunk;
     {
+        int cus = 1;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
+            return bail(fail(c, TCP_ERR_HIP, "no multiprocessor count"));
+        const bool kel = (p->flags & TCP_KELBG) != 0;
+        // mixed: 4 x 4 tile blocks when the 8 x 8 ones are fewer than twice the
+        // resident blocks (a function of N and the CU count only)
+        c->n3_shape = 0;
+        if (precision == TCP_MIXED) {
+            const int64_t rows0 = n3_block_rows_shape(c->N, 0);
+            if (rows0 * (rows0 + 1) / 2 < 2 * (int64_t)cus * pair_n3_occupancy(true, kel, 0))
+                c->n3_shape = kFineN3Shape;
+        }
+        c->n3_grid = cus * (precision == TCP_MIXED ? pair_n3_occupancy(true, kel, c->n3_shape)
+                                                   : pair_f64_n3_occupancy(true, kel));
         // upper triangle of block pairs, off-diagonal (full) blocks first
-        c->n3_rows = n3_block_rows(c->N);
+        c->n3_rows = n3_block_rows_shape(c->N, c->n3_shape);
         std::vector<int2> map;
         for (int r = 0; r < c->n3_rows; ++r)
             for (int q = r + 1; q < c->n3_rows; ++q) map.push_back(make_int2(r, q));
@@ -549,11 +564,6 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
         if ((rc = dalloc(c, &c->n3_queue, (size_t)2))) return bail(rc);
         if (hipMemset(c->n3_queue, 0, 2 * sizeof(int)) != hipSuccess)
             return bail(fail(c, TCP_ERR_HIP, "hipMemset of the work queue failed"));
-        int cus = 1;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
-            return bail(fail(c, TCP_ERR_HIP, "no multiprocessor count"));
-        c->n3_grid = cus * (precision == TCP_MIXED ? pair_n3_occupancy(true, (p->flags & TCP_KELBG) != 0)
-                                                   : pair_f64_n3_occupancy(true, (p->flags & TCP_KELBG) != 0));
         if (c->n3_rows > slots) slots = c->n3_rows;
     }
     c->pf_cap = (size_t)slots * 3 * n;

@@ -57,14 +57,15 @@ struct PairArgs {
 void launch_pair_f64(const PairArgs& a, int nchunk, hipStream_t s);
 void launch_pair_mixed(const PairArgs& a, int nchunk, hipStream_t s);
 // Self interaction with Newton's third law (both precisions): block-row slots
-// of partials, pf[n3_block_rows(n)][3][n] and pu[n3_block_rows(n)][n];
-// blkmap lists the (row, column) block pairs with row <= column.
-int n3_block_rows(int n);
+// of partials, pf[rows][3][n] and pu[rows][n] with rows =
+// n3_block_rows_shape(n, shape); blkmap lists the (row, column) block pairs
+// with row <= column.
 // persistent launch of grid resident blocks over the nblk block pairs of blkmap;
 // queue[2] is zero before the first launch and left zero by every launch
-int pair_n3_occupancy(bool rdf, bool kelbg);
+// shape: 0 (8 x 8 tile blocks) or kFineN3Shape (4 x 4 tile blocks, small systems)
+int pair_n3_occupancy(bool rdf, bool kelbg, int shape);
 void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
-                          hipStream_t s);
+                          int shape, hipStream_t s);
 int pair_f64_n3_occupancy(bool rdf, bool kelbg);
 void launch_pair_f64_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
                         hipStream_t s);
@@ -236,6 +237,9 @@ struct N3Shape {
 };
 N3Shape n3_shape(int shape);
 int n3_block_rows_shape(int n, int shape);
+// the 4 x 4 tile, four-wave shape that a single mixed context takes when the
+// 8 x 8 tile blocks are fewer than twice its resident blocks
+constexpr int kFineN3Shape = 3;
 // Persistent launches over a work list; queue[2] must be zero before the first
 // launch and is left zero by every launch.  grid: resident blocks to start.
 void launch_pair_mixed_n3_batch(const BatchRep* reps, const PairItem* items, int nitems,
