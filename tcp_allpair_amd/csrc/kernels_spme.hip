@@ -11,7 +11,9 @@
 //            as real input, the last forward (z) pass multiplies by
 //            gb = PSI * |b|^2 and writes the energy partial sum gb*|S|^2 of
 //            each block; the inverse x pass writes only the real part C.
-//   gather   one thread per particle; p^3 reads of C, force from the spline
+//   gather   one wavefront per brick of the spreading's binning: the brick's
+//            (kBrick + p - 1)^3 points of C go to LDS once and each of its
+//            particles reads its p^3 support there; force from the spline
 //            derivatives (force.f90:868-919).
 // With C the convolution of Q with the real, even kernel IDFT(gb):
 //   Um = eps/(2 pi V) sum_m gb(m) |S(m)|^2,  F_i = -eps/(pi V) sum dQ/dx_i C.
@@ -98,17 +100,62 @@ __global__ __launch_bounds__(256) void spread_kernel(RecipArgs a)
     }
 }
 
-template <int P>
-__global__ __launch_bounds__(256) void gather_kernel(RecipArgs a)
+// Force and potential of particle m from the p^3 grid values C around it
+// (force.f90:868-919); cval(ix, iy, iz) returns C at the support's grid point
+// base + (ix, iy, iz), so the same sums serve the global grid and a brick's
+// LDS copy.
+template <int P, typename CVal>
+__device__ inline void gather_one(const RecipArgs& a, int m, const double (&w)[3][P],
+                                  const double (&dw)[3][P], CVal cval)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n) return;
-    // in brick order neighbouring threads read neighbouring grid rows
-    const int m = a.nbrick[0] ? a.sorted[t] : t;
+    double sx = 0.0, sy = 0.0, sz = 0.0, sp = 0.0;
+    // z planes one at a time (p^2 reads in flight, not p^3); the plane's
+    // weights come from the front of wz/dwz, shifted after each plane, so no
+    // register array is indexed by a loop variable
+    double wz[P], dwz[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        wz[k] = w[2][k];
+        dwz[k] = dw[2][k];
+    }
+#pragma unroll 1
+    for (int k = 0; k < P; ++k) {
+        const double w2 = wz[0], dw2 = dwz[0];
+#pragma unroll
+        for (int s = 0; s + 1 < P; ++s) {
+            wz[s] = wz[s + 1];
+            dwz[s] = dwz[s + 1];
+        }
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            double cx = 0.0, c0 = 0.0;
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const double c = cval(i, j, k);
+                cx += dw[0][i] * c;
+                c0 += w[0][i] * c;
+            }
+            sx += cx * w[1][j] * w2;
+            sy += c0 * dw[1][j] * w2;
+            sz += c0 * w[1][j] * dw2;
+            sp += c0 * w[1][j] * w2;
+        }
+    }
     const double q = charge(a, m);
+    const double V = a.box[0] * a.box[1] * a.box[2];
+    const double pre = -a.eps * q / (kPi * V);
+    if (a.pot) a.pot[m] = a.eps / (kPi * V) * sp;  // dUm/dq_m
+    a.frx[m] = pre * sx * ((double)a.K[0] / a.box[0]);
+    a.fry[m] = pre * sy * ((double)a.K[1] / a.box[1]);
+    a.frz[m] = pre * sz * ((double)a.K[2] / a.box[2]);
+}
+
+// spline weights of particle m and the first grid point of its support
+template <int P>
+__device__ inline void spline_of(const RecipArgs& a, int m, double (&w)[3][P], double (&dw)[3][P],
+                                 int (&base)[3])
+{
     const double pos[3] = {a.x[m], a.y[m], a.z[m]};
-    double w[3][P], dw[3][P];
-    int base[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const double u = (double)a.K[d] * (pos[d] + a.box[d] * 0.5) / a.box[d];
@@ -116,32 +163,59 @@ __global__ __launch_bounds__(256) void gather_kernel(RecipArgs a)
         bspline<P>(u - fl, w[d], dw[d]);
         base[d] = (int)fl - P / 2 + 1;
     }
+}
+
+// one thread per particle reading the grid (grids that are not brick-binned)
+template <int P>
+__global__ __launch_bounds__(256) void gather_kernel(RecipArgs a)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.n) return;
+    double w[3][P], dw[3][P];
+    int base[3];
+    spline_of<P>(a, m, w, dw, base);
     const int K1 = a.K[0], K2 = a.K[1], K3 = a.K[2];
-    double sx = 0.0, sy = 0.0, sz = 0.0, sp = 0.0;
-    for (int k = 0; k < P; ++k) {
-        const int iz = wrap(base[2] + k, K3);
-        for (int j = 0; j < P; ++j) {
-            const int iy = wrap(base[1] + j, K2);
-            const double* row = a.C + ((size_t)iz * K2 + iy) * K1;
-            double cx = 0.0, c0 = 0.0;
-#pragma unroll
-            for (int i = 0; i < P; ++i) {
-                const double c = row[wrap(base[0] + i, K1)];
-                cx += dw[0][i] * c;
-                c0 += w[0][i] * c;
-            }
-            sx += cx * w[1][j] * w[2][k];
-            sy += c0 * dw[1][j] * w[2][k];
-            sz += c0 * w[1][j] * dw[2][k];
-            sp += c0 * w[1][j] * w[2][k];
-        }
+    gather_one<P>(a, m, w, dw, [&](int i, int j, int k) {
+        return a.C[((size_t)wrap(base[2] + k, K3) * K2 + wrap(base[1] + j, K2)) * K1 +
+                   wrap(base[0] + i, K1)];
+    });
+}
+
+// One wavefront per brick: the (kBrick + p - 1)^3 grid points its particles'
+// splines reach are copied to LDS once, and each particle of the brick (the
+// binned order of the spreading) reads its support from there at offsets
+// inside the tile.
+template <int P>
+__global__ __launch_bounds__(64) void gather_brick_kernel(RecipArgs a)
+{
+    constexpr int E = kBrick + P - 1;
+    __shared__ double tile[E * E * E];
+    const int bid = blockIdx.x;
+    const int p0 = a.brick_start[bid], p1 = a.brick_start[bid + 1];
+    if (p0 == p1) return;
+    const int borg[3] = {(bid % a.nbrick[0]) * kBrick, ((bid / a.nbrick[0]) % a.nbrick[1]) * kBrick,
+                         (bid / (a.nbrick[0] * a.nbrick[1])) * kBrick};
+    const int K1 = a.K[0], K2 = a.K[1], K3 = a.K[2];
+    for (int t = threadIdx.x; t < E * E * E; t += blockDim.x) {
+        const int lx = t % E, ly = (t / E) % E, lz = t / (E * E);
+        int gx = borg[0] + lx, gy = borg[1] + ly, gz = borg[2] + lz;  // < K + E <= 2K
+        gx -= gx >= K1 ? K1 : 0;
+        gy -= gy >= K2 ? K2 : 0;
+        gz -= gz >= K3 ? K3 : 0;
+        tile[t] = a.C[((size_t)gz * K2 + gy) * K1 + gx];
     }
-    const double V = a.box[0] * a.box[1] * a.box[2];
-    const double pre = -a.eps * q / (kPi * V);
-    if (a.pot) a.pot[m] = a.eps / (kPi * V) * sp;  // dUm/dq_m
-    a.frx[m] = pre * sx * ((double)K1 / a.box[0]);
-    a.fry[m] = pre * sy * ((double)K2 / a.box[1]);
-    a.frz[m] = pre * sz * ((double)K3 / a.box[2]);
+    __syncthreads();
+    for (int t = p0 + (int)threadIdx.x; t < p1; t += blockDim.x) {
+        const int m = a.sorted[t];
+        double w[3][P], dw[3][P];
+        int base[3];
+        spline_of<P>(a, m, w, dw, base);
+        // offset of the support inside the tile, in [0, kBrick) per axis
+        const int o0 = wrap(base[0], K1) - borg[0], o1 = wrap(base[1], K2) - borg[1],
+                  o2 = wrap(base[2], K3) - borg[2];
+        const double* tb = tile + (o2 * E + o1) * E + o0;
+        gather_one<P>(a, m, w, dw, [&](int i, int j, int k) { return tb[(k * E + j) * E + i]; });
+    }
 }
 
 // ------------------------------------------------- brick-binned spreading ---
@@ -904,6 +978,17 @@ void launch_spme_spread(const RecipArgs& a, hipStream_t s)
 
 void launch_spme_gather(const RecipArgs& a, hipStream_t s)
 {
+    if (a.nbrick[0]) {
+        const dim3 g(a.nbrick[0] * a.nbrick[1] * a.nbrick[2]), b(64);
+        switch (a.order) {
+            case 4: hipLaunchKernelGGL(gather_brick_kernel<4>, g, b, 0, s, a); break;
+            case 6: hipLaunchKernelGGL(gather_brick_kernel<6>, g, b, 0, s, a); break;
+            case 8: hipLaunchKernelGGL(gather_brick_kernel<8>, g, b, 0, s, a); break;
+            case 10: hipLaunchKernelGGL(gather_brick_kernel<10>, g, b, 0, s, a); break;
+            default: hipLaunchKernelGGL(gather_brick_kernel<12>, g, b, 0, s, a); break;
+        }
+        return;
+    }
     dim3 grid((a.n + 255) / 256), blk(256);
     switch (a.order) {
         case 4: hipLaunchKernelGGL(gather_kernel<4>, grid, blk, 0, s, a); break;
