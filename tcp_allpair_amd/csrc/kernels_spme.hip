@@ -6,11 +6,12 @@
 //            p^3 hardware FP64 atomic adds into the real grid Q (x fastest,
 //            the Fortran order of Q(ix,iy,iz)).
 //   FFT      batched radix-2 Stockham transforms in LDS along x, y, z; each
-//            block holds 1024 complex points (1024/K lines) so the strided
-//            axes load 16*(1024/K)-byte contiguous runs.  The x pass reads Q
-//            as real input, the last forward (z) pass multiplies by
-//            gb = PSI * |b|^2 and writes the energy partial sum gb*|S|^2 of
-//            each block; the inverse x pass writes only the real part C.
+//            block holds 1024 complex points (1024/K lines), twiddles in LDS.
+//            The x pass reads Q as real input and keeps the Hermitian half
+//            kx <= K1/2; the z pass transforms, multiplies by gb = PSI |b|^2
+//            (writing each block's energy partial gb |S|^2) and transforms
+//            back; the inverse x pass restores the other half by conjugation
+//            and writes only the real part C.
 //   gather   one wavefront per brick of the spreading's binning: the brick's
 //            (kBrick + p - 1)^3 points of C go to LDS once and each of its
 //            particles reads its p^3 support there; force from the spline
@@ -196,13 +197,28 @@ __global__ __launch_bounds__(64) void gather_brick_kernel(RecipArgs a)
     const int borg[3] = {(bid % a.nbrick[0]) * kBrick, ((bid / a.nbrick[0]) % a.nbrick[1]) * kBrick,
                          (bid / (a.nbrick[0] * a.nbrick[1])) * kBrick};
     const int K1 = a.K[0], K2 = a.K[1], K3 = a.K[2];
-    for (int t = threadIdx.x; t < E * E * E; t += blockDim.x) {
-        const int lx = t % E, ly = (t / E) % E, lz = t / (E * E);
-        int gx = borg[0] + lx, gy = borg[1] + ly, gz = borg[2] + lz;  // < K + E <= 2K
-        gx -= gx >= K1 ? K1 : 0;
-        gy -= gy >= K2 ? K2 : 0;
-        gz -= gz >= K3 ? K3 : 0;
-        tile[t] = a.C[((size_t)gz * K2 + gy) * K1 + gx];
+    // eight loads in flight per thread before their LDS writes
+    constexpr int kGroup = 8;
+    for (int t0 = 0; t0 < E * E * E; t0 += 64 * kGroup) {
+        double c[kGroup];
+#pragma unroll
+        for (int r = 0; r < kGroup; ++r) {
+            const int t = t0 + r * 64 + (int)threadIdx.x;
+            c[r] = 0.0;
+            if (t < E * E * E) {
+                const int lx = t % E, ly = (t / E) % E, lz = t / (E * E);
+                int gx = borg[0] + lx, gy = borg[1] + ly, gz = borg[2] + lz;  // < K + E <= 2K
+                gx -= gx >= K1 ? K1 : 0;
+                gy -= gy >= K2 ? K2 : 0;
+                gz -= gz >= K3 ? K3 : 0;
+                c[r] = a.C[((size_t)gz * K2 + gy) * K1 + gx];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kGroup; ++r) {
+            const int t = t0 + r * 64 + (int)threadIdx.x;
+            if (t < E * E * E) tile[t] = c[r];
+        }
     }
     __syncthreads();
     for (int t = p0 + (int)threadIdx.x; t < p1; t += blockDim.x) {
@@ -388,33 +404,44 @@ __global__ __launch_bounds__(256) void grid_from_tiles_kernel(RecipArgs a)
 }
 
 // -------------------------------------------------------------------- FFT ---
+//
+// Q is real, so its transform is Hermitian, S(-m) = conj S(m), and gb is real
+// and even.  The forward x pass keeps kx = 0..K1/2 only (H = K1/2 + 1 columns
+// of the complex work grid A, x fastest), the y and z passes run on that half,
+// and the inverse x pass restores kx > K1/2 by conjugation before writing the
+// real part.  The z pass transforms a line, multiplies it by gb, adds
+// c(kx) gb |S|^2 to the energy partial of its block (c = 1 for kx = 0 and
+// K1/2, 2 for the columns that stand for their mirror too) and transforms it
+// back, so the spectrum makes one round trip through memory less.
 
 constexpr int kFftTile = 1024;   // complex points per block
 constexpr int kFftThreads = 256;
 
-enum { kInComplex = 0, kInReal = 1 };
-enum { kOutComplex = 0, kOutReal = 2, kOutConvolve = 4 };
+enum { kFftFwdX = 0, kFftY = 1, kFftZConv = 2, kFftInvX = 3 };
+
+// line l starts at (l / inner) outer + (l % inner) lstride; its point k is
+// estride further on
+struct FftAddr {
+    int inner, outer, lstride, estride;
+};
 
 struct FftPass {
-    const double2* in;
     const double* in_real;
+    const double2* in;
     double2* out;
     double* out_real;
     const double2* tw;   // exp(+2 pi i m / K), m < K/2
     const double* gb;
     double* epart;
-    int K, logK;
-    int lines_per_block;
-    int inner, outer;    // line l -> base = (l / inner) * outer + (l % inner) * lstride
-    int lstride;
-    int estride;         // distance between points of one line
-    int contiguous;      // 1: a block's lines form one contiguous run (x axis)
-    int sign;            // -1 forward, +1 inverse
+    int K, logK, lines, lines_per_block;
+    int sign;            // -1 forward, +1 inverse (the y pass)
+    FftAddr ia, oa;
+    int H, K1, K2;       // half width and grid sizes for the z pass's gb and weights
 };
 
-__device__ inline size_t line_base(const FftPass& f, int l)
+__device__ inline size_t fft_base(const FftAddr& d, int l)
 {
-    return (size_t)(l / f.inner) * f.outer + (size_t)(l % f.inner) * f.lstride;
+    return (size_t)(l / d.inner) * d.outer + (size_t)(l % d.inner) * d.lstride;
 }
 
 __device__ inline double2 cmul(double2 a, double2 b)
@@ -424,29 +451,13 @@ __device__ inline double2 cmul(double2 a, double2 b)
 
 __device__ inline double2 conjd(double2 a) { return make_double2(a.x, -a.y); }
 
-template <int kMode>
-__global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
+// radix-2 Stockham stages over the block's lines, ping-pong in buf; src
+// names the buffer that holds the result
+__device__ inline void fft_stages(double2 (*buf)[kFftTile], int& src, int K, int logK, int npts,
+                                  const double2* tws, bool inverse)
 {
-    __shared__ double2 buf[2][kFftTile];
-    __shared__ double red[kFftThreads];
-    const int K = f.K, B = f.lines_per_block;
-    const int l0 = blockIdx.x * B;
-    const int npts = B * K;
-    // load: tile index t = lb * K + k
-    for (int e = threadIdx.x; e < npts; e += kFftThreads) {
-        int lb, k;
-        if (f.contiguous) { lb = e / K; k = e % K; }
-        else { k = e / B; lb = e % B; }
-        const size_t g = line_base(f, l0 + lb) + (size_t)k * f.estride;
-        double2 v;
-        if (kMode & kInReal) v = make_double2(f.in_real[g], 0.0);
-        else v = f.in[g];
-        buf[0][lb * K + k] = v;
-    }
-    __syncthreads();
-    int src = 0;
     const int half = K >> 1;
-    for (int s = 0; s < f.logK; ++s) {
+    for (int s = 0; s < logK; ++s) {
         const int Ns = 1 << s;
         for (int b = threadIdx.x; b < (npts >> 1); b += kFftThreads) {
             const int lb = b / half, j = b % half;
@@ -454,8 +465,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
             const double2* A = buf[src] + lb * K;
             double2* Bo = buf[src ^ 1] + lb * K;
             const double2 a0 = A[j];
-            double2 w = f.tw[k << (f.logK - 1 - s)];
-            if (f.sign < 0) w.y = -w.y;
+            double2 w = tws[k << (logK - 1 - s)];
+            if (!inverse) w.y = -w.y;
             const double2 a1 = cmul(A[j + half], w);
             const int d = ((j - k) << 1) + k;
             Bo[d] = make_double2(a0.x + a1.x, a0.y + a1.y);
@@ -464,23 +475,69 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
         src ^= 1;
         __syncthreads();
     }
-    double esum = 0.0;
-    for (int e = threadIdx.x; e < npts; e += kFftThreads) {
-        int lb, k;
-        if (f.contiguous) { lb = e / K; k = e % K; }
-        else { k = e / B; lb = e % B; }
-        const size_t g = line_base(f, l0 + lb) + (size_t)k * f.estride;
-        double2 v = buf[src][lb * K + k];
-        if (kMode & kOutConvolve) {
-            const double gbv = f.gb[g];
-            esum += gbv * (v.x * v.x + v.y * v.y);
-            v.x *= gbv;
-            v.y *= gbv;
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
+{
+    __shared__ double2 buf[2][kFftTile];
+    __shared__ double2 tws[kFftTile / 2];
+    const int K = f.K, B = f.lines_per_block, half = K >> 1;
+    const int l0 = blockIdx.x * B;
+    const int npts = B * K;
+    // x lines: the block's lines are one contiguous run (of H or K1 points each)
+    constexpr bool kX = kMode == kFftFwdX || kMode == kFftInvX;
+    for (int t = threadIdx.x; t < half; t += kFftThreads) tws[t] = f.tw[t];
+    // all of a thread's loads in flight before its LDS writes
+    constexpr int kPer = kFftTile / kFftThreads;
+    const int nin = kMode == kFftInvX ? f.H : K;
+    double2 v[kPer];
+    int slot[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        v[r] = make_double2(0.0, 0.0);
+        slot[r] = -1;
+        if (e < B * nin) {
+            int lb, k;
+            if (kX) { lb = e / nin; k = e % nin; }
+            else { k = e / B; lb = e % B; }
+            const int l = l0 + lb;
+            slot[r] = lb * K + k;
+            if (l < f.lines) {
+                const size_t g = fft_base(f.ia, l) + (size_t)k * f.ia.estride;
+                if (kMode == kFftFwdX) v[r] = make_double2(f.in_real[g], 0.0);
+                else v[r] = f.in[g];
+            }
         }
-        if (kMode & kOutReal) f.out_real[g] = v.x;
-        else f.out[g] = v;
     }
-    if (kMode & kOutConvolve) {
+#pragma unroll
+    for (int r = 0; r < kPer; ++r)
+        if (slot[r] >= 0) buf[0][slot[r]] = v[r];
+    __syncthreads();
+    if (kMode == kFftInvX) {  // X(K1 - k) = conj X(k), k = 1 .. K1/2 - 1
+        for (int e = threadIdx.x; e < B * (half - 1); e += kFftThreads) {
+            const int lb = e / (half - 1), k = 1 + e % (half - 1);
+            buf[0][lb * K + K - k] = conjd(buf[0][lb * K + k]);
+        }
+        __syncthreads();
+    }
+    int src = 0;
+    fft_stages(buf, src, K, f.logK, npts, tws, kMode == kFftInvX || (kMode == kFftY && f.sign > 0));
+    if (kMode == kFftZConv) {
+        double esum = 0.0;
+        for (int e = threadIdx.x; e < npts; e += kFftThreads) {
+            const int lb = e / K, k = e % K, l = l0 + lb;
+            if (l >= f.lines) continue;
+            const int kx = l % f.H, ky = l / f.H;
+            const double gbv = f.gb[((size_t)k * f.K2 + ky) * f.K1 + kx];
+            const double2 v = buf[src][e];
+            const double c = (kx == 0 || 2 * kx == f.K1) ? 1.0 : 2.0;
+            esum += c * gbv * (v.x * v.x + v.y * v.y);
+            buf[src][e] = make_double2(v.x * gbv, v.y * gbv);
+        }
+        // the block's energy partial, summed in a fixed order over the idle buffer
+        double* red = (double*)buf[src ^ 1];
         red[threadIdx.x] = esum;
         __syncthreads();
         for (int w = kFftThreads / 2; w > 0; w >>= 1) {
@@ -488,6 +545,20 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
             __syncthreads();
         }
         if (threadIdx.x == 0) f.epart[blockIdx.x] = red[0];
+        __syncthreads();
+        fft_stages(buf, src, K, f.logK, npts, tws, true);
+    }
+    const int nout = kMode == kFftFwdX ? f.H : K;
+    for (int e = threadIdx.x; e < B * nout; e += kFftThreads) {
+        int lb, k;
+        if (kX) { lb = e / nout; k = e % nout; }
+        else { k = e / B; lb = e % B; }
+        const int l = l0 + lb;
+        if (l >= f.lines) continue;
+        const size_t g = fft_base(f.oa, l) + (size_t)k * f.oa.estride;
+        const double2 v = buf[src][lb * K + k];
+        if (kMode == kFftInvX) f.out_real[g] = v.x;
+        else f.out[g] = v;
     }
 }
 
@@ -498,47 +569,49 @@ int ilog2(int k)
     return l;
 }
 
-// Lines per block: as many as fit 1024 points, and never more than the run of
-// consecutive lines that share one outer index (so a block's loads stay strided
-// by one line).  All K are powers of two, so every count divides.
-int lines_per_block(const int* K, int axis)
-{
-    const int fit = kFftTile / K[axis];
-    const int run = axis == 0 ? K[1] * K[2] : (axis == 1 ? K[0] : K[0] * K[1]);
-    return fit < run ? fit : run;
-}
-
-FftPass make_pass(const RecipArgs& a, int axis, int sign)
+// The four passes over K = (K1, K2, K3); A holds H = K1/2 + 1 columns.
+FftPass make_pass(const RecipArgs& a, int mode, int sign)
 {
     FftPass f{};
-    const int K1 = a.K[0], K2 = a.K[1], K3 = a.K[2];
+    const int K1 = a.K[0], K2 = a.K[1], K3 = a.K[2], H = K1 / 2 + 1;
+    const int axis = mode == kFftY ? 1 : (mode == kFftZConv ? 2 : 0);
     f.K = a.K[axis];
     f.logK = ilog2(f.K);
-    f.lines_per_block = lines_per_block(a.K, axis);
+    f.lines_per_block = kFftTile / f.K;
     f.tw = a.tw[axis];
     f.sign = sign;
     f.gb = a.gb;
     f.epart = a.epart;
+    f.H = H;
+    f.K1 = K1;
+    f.K2 = K2;
     if (axis == 0) {
-        f.inner = 1; f.outer = K1; f.lstride = 0; f.estride = 1; f.contiguous = 1;
+        f.lines = K2 * K3;
+        const FftAddr full{1, K1, 0, 1}, halfx{1, H, 0, 1};
+        f.ia = mode == kFftFwdX ? full : halfx;
+        f.oa = mode == kFftFwdX ? halfx : full;
+        f.in_real = a.Q;
+        f.in = a.A;
+        f.out = a.A;
+        f.out_real = a.C;
     } else if (axis == 1) {
-        f.inner = K1; f.outer = K1 * K2; f.lstride = 1; f.estride = K1; f.contiguous = 0;
+        f.lines = H * K3;
+        f.ia = f.oa = FftAddr{H, K2 * H, 1, H};
+        f.in = a.A;
+        f.out = a.A;
     } else {
-        f.inner = K1 * K2; f.outer = 0; f.lstride = 1; f.estride = K1 * K2; f.contiguous = 0;
+        f.lines = H * K2;
+        f.ia = f.oa = FftAddr{H * K2, 0, 1, H * K2};
+        f.in = a.A;
+        f.out = a.A;
     }
-    (void)K3;
     return f;
 }
 
-int nlines(const int* K, int axis)
-{
-    return K[0] * K[1] * K[2] / K[axis];
-}
-
 template <int kMode>
-void run_pass(const FftPass& f, int lines, hipStream_t s)
+void run_pass(const FftPass& f, hipStream_t s)
 {
-    const int blocks = lines / f.lines_per_block;
+    const int blocks = (f.lines + f.lines_per_block - 1) / f.lines_per_block;
     hipLaunchKernelGGL(fft_pass_kernel<kMode>, dim3(blocks), dim3(kFftThreads), 0, s, f);
 }
 
@@ -931,7 +1004,8 @@ __global__ __launch_bounds__(kKsumWaves * 64) void ewald_force_batch_kernel(cons
 
 int fft_epart_blocks(const int* K)
 {
-    return nlines(K, 2) / lines_per_block(K, 2);
+    const int lines = (K[0] / 2 + 1) * K[1], per = kFftTile / K[2];
+    return (lines + per - 1) / per;
 }
 
 void launch_fill(double* p, size_t n, double v, hipStream_t s)
@@ -999,36 +1073,13 @@ void launch_spme_gather(const RecipArgs& a, hipStream_t s)
     }
 }
 
-void launch_fft3d_forward_convolve(const RecipArgs& a, hipStream_t s)
+void launch_spme_convolve(const RecipArgs& a, hipStream_t s)
 {
-    FftPass f0 = make_pass(a, 0, -1);
-    f0.in_real = a.Q;
-    f0.out = a.A;
-    run_pass<kInReal | kOutComplex>(f0, nlines(a.K, 0), s);
-    FftPass f1 = make_pass(a, 1, -1);
-    f1.in = a.A;
-    f1.out = a.A;
-    run_pass<kInComplex | kOutComplex>(f1, nlines(a.K, 1), s);
-    FftPass f2 = make_pass(a, 2, -1);
-    f2.in = a.A;
-    f2.out = a.A;
-    run_pass<kInComplex | kOutConvolve>(f2, nlines(a.K, 2), s);
-}
-
-void launch_fft3d_inverse(const RecipArgs& a, hipStream_t s)
-{
-    FftPass f2 = make_pass(a, 2, +1);
-    f2.in = a.A;
-    f2.out = a.A;
-    run_pass<kInComplex | kOutComplex>(f2, nlines(a.K, 2), s);
-    FftPass f1 = make_pass(a, 1, +1);
-    f1.in = a.A;
-    f1.out = a.A;
-    run_pass<kInComplex | kOutComplex>(f1, nlines(a.K, 1), s);
-    FftPass f0 = make_pass(a, 0, +1);
-    f0.in = a.A;
-    f0.out_real = a.C;
-    run_pass<kInComplex | kOutReal>(f0, nlines(a.K, 0), s);
+    run_pass<kFftFwdX>(make_pass(a, kFftFwdX, -1), s);
+    run_pass<kFftY>(make_pass(a, kFftY, -1), s);
+    run_pass<kFftZConv>(make_pass(a, kFftZConv, -1), s);
+    run_pass<kFftY>(make_pass(a, kFftY, +1), s);
+    run_pass<kFftInvX>(make_pass(a, kFftInvX, +1), s);
 }
 
 int ewald_chunk(int kmax)

@@ -223,9 +223,7 @@ int recip(tcp_ctx* c, const double* qp = nullptr, double* pot = nullptr)
     if (c->p.recip == TCP_RECIP_SPME) {
         launch_spme_spread(a, c->stream);
         LAUNCHCK(c);
-        launch_fft3d_forward_convolve(a, c->stream);
-        LAUNCHCK(c);
-        launch_fft3d_inverse(a, c->stream);
+        launch_spme_convolve(a, c->stream);
         LAUNCHCK(c);
         launch_spme_gather(a, c->stream);
         LAUNCHCK(c);

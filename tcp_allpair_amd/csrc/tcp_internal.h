@@ -107,8 +107,8 @@ struct RecipArgs {
 constexpr int kBrick = 8;      // grid points per brick edge
 
 void launch_spme_spread(const RecipArgs& a, hipStream_t s);
-void launch_fft3d_forward_convolve(const RecipArgs& a, hipStream_t s);
-void launch_fft3d_inverse(const RecipArgs& a, hipStream_t s);
+// Q -> C = Q convolved with IDFT(gb), energy partials gb |S|^2 per block into epart
+void launch_spme_convolve(const RecipArgs& a, hipStream_t s);
 void launch_spme_gather(const RecipArgs& a, hipStream_t s);
 void launch_fill(double* p, size_t n, double v, hipStream_t s);
 // AoS (x0 y0 z0 x1 ...) <-> SoA transposes of n particles
