@@ -23,14 +23,15 @@ PAIR_KERNEL = "pair_mixed_n3_kernel<true, false>"
 FP64_PAIR_KERNEL = "pair_f64_n3_kernel<true, false>"
 FP64_TIMED_STEPS = 2  # --fp64-steps of STATS_CMD
 STATS_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_stats -o bench "
-             "-- python3 bench.py --steps 5 --warmup 1 --fp64-steps 2 --no-cpu-baseline --batch-steps 0")
+             "-- python3 bench.py --steps 5 --warmup 1 --fp64-steps 2 --no-cpu-baseline --batch-steps 0 "
+             "--dropin-steps 0")
 BATCH_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_batch -o batch "
              "-- python3 -m tcp_allpair_amd.batchbench --steps 5 --warmup 1")
 PMC_CMDS = [
     "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o bench -- python3 bench.py "
-    "--steps 2 --warmup 0 --fp64-steps 1 --batch-steps 0 --no-cpu-baseline",
+    "--steps 2 --warmup 0 --fp64-steps 1 --batch-steps 0 --dropin-steps 0 --no-cpu-baseline",
     "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o bench -- python3 bench.py "
-    "--steps 2 --warmup 0 --fp64-steps 1 --batch-steps 0 --no-cpu-baseline",
+    "--steps 2 --warmup 0 --fp64-steps 1 --batch-steps 0 --dropin-steps 0 --no-cpu-baseline",
     "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_copy_fetch -o copy -- "
     "tcp_allpair_amd/copy_probe 512 3",
     "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_copy_write -o copy -- "
@@ -38,7 +39,7 @@ PMC_CMDS = [
 ]
 VALU_CMD = ("rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_SALU SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES "
             "SQ_INSTS_VALU_TRANS_F32 --output-format csv -d gpurun_out/pmc_valu -o bench -- python3 "
-            "bench.py --steps 2 --warmup 0 --fp64-steps 1 --batch-steps 0 --no-cpu-baseline")
+            "bench.py --steps 2 --warmup 0 --fp64-steps 1 --batch-steps 0 --dropin-steps 0 --no-cpu-baseline")
 N_BENCH = 131072
 
 
