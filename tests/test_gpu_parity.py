@@ -264,6 +264,20 @@ def test_spme_vs_oracle(order, K):
     assert rel_err(f, fo).max() <= 1e-10
 
 
+def test_spme_unbinned_grid_vs_oracle():
+    """K1 = 4 is not a multiple of the 8-point brick, so spreading and gather take
+    the per-particle kernels, and the x transform keeps H = 3 of 4 columns."""
+    say("case spme K=(4,16,8)")
+    p, x, _ = make_plasma(300, 300, seed=34, K=(4, 16, 8), order=4)
+    fo, umo = cpu.spme(p, x)
+    with Context(p, FP64) as c:
+        c.set_state(x=x, f=np.zeros_like(x))
+        um = c.force_recip()
+        f = c.get_state()[2]
+    assert abs(um - umo) <= 1e-12 * abs(umo)
+    assert rel_err(f, fo).max() <= 1e-10
+
+
 def test_spme_anisotropic_vs_oracle():
     say("case spme aniso")
     p, x, _ = make_plasma(300, 500, seed=32, K=(32, 64, 16), order=6)
