@@ -5,8 +5,9 @@
 //   spread   one thread per particle; centred order-p B-spline weights and
 //            p^3 hardware FP64 atomic adds into the real grid Q (x fastest,
 //            the Fortran order of Q(ix,iy,iz)).
-//   FFT      batched radix-2 Stockham transforms in LDS along x, y, z; each
-//            block holds 1024 complex points (1024/K lines), twiddles in LDS.
+//   FFT      batched radix-4 (and one radix-2) Stockham transforms in LDS
+//            along x, y, z; each block holds 1024 complex points (1024/K
+//            lines) and the twiddles.
 //            The x pass reads Q as real input and keeps the Hermitian half
 //            kx <= K1/2; the z pass transforms, multiplies by gb = PSI |b|^2
 //            (writing each block's energy partial gb |S|^2) and transforms
@@ -451,26 +452,62 @@ __device__ inline double2 cmul(double2 a, double2 b)
 
 __device__ inline double2 conjd(double2 a) { return make_double2(a.x, -a.y); }
 
-// radix-2 Stockham stages over the block's lines, ping-pong in buf; src
-// names the buffer that holds the result
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// exp(-+2 pi i m / K) for 0 <= m < K from the half table exp(+2 pi i m / K),
+// m < K/2 (the second half is its negative)
+__device__ inline double2 twiddle(const double2* tws, int m, int half, bool inverse)
+{
+    double2 w = m < half ? tws[m] : make_double2(-tws[m - half].x, -tws[m - half].y);
+    if (!inverse) w.y = -w.y;
+    return w;
+}
+
+// Mixed-radix Stockham stages over the block's lines, ping-pong in buf (a
+// radix-2 stage first when log2 K is odd, then radix-4 stages: four
+// barriers for K = 128); src names the buffer that holds the result.
+// Stage with span Ns and radix R: butterfly j of a line reads points
+// j + r K/R, multiplies point r by w^(r (j mod Ns) K/(R Ns)), transforms the R
+// values and writes them to (j - j mod Ns) R + j mod Ns + r Ns.
 __device__ inline void fft_stages(double2 (*buf)[kFftTile], int& src, int K, int logK, int npts,
                                   const double2* tws, bool inverse)
 {
-    const int half = K >> 1;
-    for (int s = 0; s < logK; ++s) {
-        const int Ns = 1 << s;
+    const int half = K >> 1, quarter = K >> 2;
+    int Ns = 1, lNs = 0;
+    if (logK & 1) {
         for (int b = threadIdx.x; b < (npts >> 1); b += kFftThreads) {
-            const int lb = b / half, j = b % half;
-            const int k = j & (Ns - 1);
+            const int lb = b >> (logK - 1), j = b & (half - 1);
             const double2* A = buf[src] + lb * K;
             double2* Bo = buf[src ^ 1] + lb * K;
-            const double2 a0 = A[j];
-            double2 w = tws[k << (logK - 1 - s)];
-            if (!inverse) w.y = -w.y;
-            const double2 a1 = cmul(A[j + half], w);
-            const int d = ((j - k) << 1) + k;
-            Bo[d] = make_double2(a0.x + a1.x, a0.y + a1.y);
-            Bo[d + Ns] = make_double2(a0.x - a1.x, a0.y - a1.y);
+            const double2 a0 = A[j], a1 = A[j + half];
+            Bo[2 * j] = cadd(a0, a1);
+            Bo[2 * j + 1] = csub(a0, a1);
+        }
+        src ^= 1;
+        __syncthreads();
+        Ns = 2;
+        lNs = 1;
+    }
+    for (; Ns < K; Ns <<= 2, lNs += 2) {
+        for (int b = threadIdx.x; b < (npts >> 2); b += kFftThreads) {
+            const int lb = b >> (logK - 2), j = b & (quarter - 1);
+            const int k = j & (Ns - 1);
+            const int m = k << (logK - 2 - lNs);  // k K / (4 Ns)
+            const double2* A = buf[src] + lb * K;
+            double2* Bo = buf[src ^ 1] + lb * K;
+            const double2 v0 = A[j];
+            const double2 v1 = cmul(A[j + quarter], twiddle(tws, m, half, inverse));
+            const double2 v2 = cmul(A[j + 2 * quarter], twiddle(tws, 2 * m, half, inverse));
+            const double2 v3 = cmul(A[j + 3 * quarter], twiddle(tws, 3 * m, half, inverse));
+            const double2 t0 = cadd(v0, v2), t1 = csub(v0, v2), t2 = cadd(v1, v3), d = csub(v1, v3);
+            // times -i (forward) or +i (inverse)
+            const double2 t3 = inverse ? make_double2(-d.y, d.x) : make_double2(d.y, -d.x);
+            const int o = ((j - k) << 2) + k;
+            Bo[o] = cadd(t0, t2);
+            Bo[o + Ns] = cadd(t1, t3);
+            Bo[o + 2 * Ns] = csub(t0, t2);
+            Bo[o + 3 * Ns] = csub(t1, t3);
         }
         src ^= 1;
         __syncthreads();
