@@ -2,9 +2,13 @@
 // generalised to even B-spline order p) and the direct Ewald k-space sum.
 //
 // Pipeline per force evaluation, all FP64:
-//   spread   one thread per particle; centred order-p B-spline weights and
-//            p^3 hardware FP64 atomic adds into the real grid Q (x fastest,
-//            the Fortran order of Q(ix,iy,iz)).
+//   spread   particles counting-sorted by the 8^3 brick of their first grid
+//            point; one block per brick adds its particles' centred order-p
+//            B-spline weights into an LDS tile (brick plus halo) as 64-bit
+//            fixed-point integers, and a pass over the covering tiles forms
+//            the real grid Q (x fastest, the Fortran order of Q(ix,iy,iz)) in
+//            a fixed order.  Grids whose sizes are not multiples of 8 take
+//            one thread per particle with FP64 atomic adds into Q.
 //   FFT      batched radix-4 (and one radix-2) Stockham transforms in LDS
 //            along x, y, z; each block holds 1024 complex points (1024/K
 //            lines) and the twiddles.
@@ -15,7 +19,8 @@
 //            and writes only the real part C.
 //   gather   one wavefront per brick of the spreading's binning: the brick's
 //            (kBrick + p - 1)^3 points of C go to LDS once and each of its
-//            particles reads its p^3 support there; force from the spline
+//            particles reads its p^3 support there (one thread per particle
+//            reading C where the grid is not binned); force from the spline
 //            derivatives (force.f90:868-919).
 // With C the convolution of Q with the real, even kernel IDFT(gb):
 //   Um = eps/(2 pi V) sum_m gb(m) |S(m)|^2,  F_i = -eps/(pi V) sum dQ/dx_i C.
