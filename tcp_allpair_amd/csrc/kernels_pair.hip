@@ -1,17 +1,24 @@
 // All-pair real-space kernels: Ewald erfc for every pair within rc plus the
-// electron-ion Deutsch term -exp(-r/lambda)/r (force.f90:87-178, 215-341).
+// electron-ion Deutsch term -exp(-r/lambda)/r (force.f90:87-178, 215-341),
+// with the Kelbg form and e-e regularisation as options.
 //
-// Shape (both precisions): one i particle per lane held in registers, 256 i per
-// block (blockIdx.x), the j range split into chunks (blockIdx.y).  j is uniform
-// across the wavefront, so its coordinates are scalar loads broadcast to all
-// 64 lanes and no LDS staging is needed for them; the j loop runs over the
-// electron part and then the ion part of the chunk so the pair species and the
-// Deutsch branch are uniform per wavefront except in the one wavefront that
-// straddles the electron/ion boundary of the i set.  Every ordered pair (i,j)
-// is evaluated (no Newton's third law), so each i sums its own force without
-// atomics and the per-(chunk,i) partials are summed in a fixed order later:
-// FP64 results do not depend on scheduling.  RDF counts go to a per-block LDS
-// histogram (3 x 512 uint32) flushed with one global 64-bit atomic per bin.
+// Self interaction (Force_direct_self, every MD step): Newton's-third-law
+// kernels, pair_mixed_n3_kernel (FP32 pair arithmetic), pair_mixed_n3_fine_kernel
+// (the same on 4 x 4 tile blocks for small systems) and pair_f64_n3_kernel.
+// Tiles of 64 particles; a persistent grid takes blocks of tile pairs of the
+// upper triangle from a device queue; a wavefront holds its i tiles in
+// registers and meets the block's j tiles by rotating the j data and their
+// reaction sums one lane per step with DPP.  Each (block row, particle) partial
+// is written once into its slot, so the fixed-order sum over slots
+// (combine_kernel) does not depend on scheduling.
+//
+// External block (Force_direct_ext) and the adjoint: ordered-pair kernels with
+// one i particle per lane, 256 i per block (blockIdx.x) and the j range split
+// into chunks (blockIdx.y); j is uniform across the wavefront (scalar loads),
+// and the per-(chunk, i) partials are summed in a fixed order later.
+//
+// RDF counts go to a per-block LDS histogram (3 x 512 uint32) flushed with one
+// global 64-bit atomic per non-empty bin.
 #include "f64_tables.h"
 #include "tcp_internal.h"
 
