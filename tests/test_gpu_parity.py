@@ -471,6 +471,47 @@ def test_regularised_forms_vs_oracle(flags):
     assert abs(umx - u64) <= 1e-8 * pair_energy_scale(p, x)
 
 
+@pytest.mark.parametrize("ne,ni,recip", [(2, 0, RECIP_NONE), (0, 2, RECIP_NONE),
+                                         (1, 1, RECIP_SPME), (300, 0, RECIP_SPME),
+                                         (0, 257, RECIP_NONE), (1, 130, RECIP_SPME)])
+def test_edge_systems_vs_oracle(ne, ni, recip):
+    """Two particles, one species only, a lone electron among ions; a few
+    particles sit exactly on the faces of the box (x = -L/2 and just below
+    +L/2).  FP64 against the oracle (real space and SPME), mixed against FP64."""
+    say("case edge", ne, ni, recip)
+    p, x, _ = make_plasma(ne, ni, seed=81, recip=recip, K=(16, 16, 16), order=4)
+    L = p.box[0]
+    if ne + ni == 2:  # the pair across the x face, 0.23 A apart in the minimum image
+        x[:] = [[-0.5 * L, 0.1, 0.2], [0.5 * L - 0.2, 0.0, 0.25]]
+    else:
+        x[0, 0] = -0.5 * L
+        x[-1, 1] = np.nextafter(0.5 * L, 0.0)
+    fo, uro, rdfo = cpu.force_allpair(p, x)
+    if recip == RECIP_SPME:
+        fro, umo = cpu.spme(p, x)
+        fo = fo + fro
+    out = {}
+    for prec in (FP64, MIXED):
+        with Context(p, prec) as c:
+            c.set_state(x=x, f=np.zeros_like(x))
+            ur = c.force_direct()
+            um = c.force_recip()
+            f = c.get_state()[2]
+            rdf, _ = c.histograms()
+        out[prec] = (f, ur, um, rdf)
+    f, ur, um, rdf = out[FP64]
+    assert rel_err(f, fo).max() <= 1e-10
+    assert abs(ur - uro) <= 1e-12 * abs(uro)
+    np.testing.assert_array_equal(rdf, rdfo)
+    if recip == RECIP_SPME:
+        assert abs(um - umo) <= 1e-12 * abs(umo)
+    fm, urm, _, _ = out[MIXED]
+    assert rms_rel(fm, f) <= 1e-6
+    # Ur relative 1e-6 where it does not cancel (one species, a lone pair), else
+    # 1e-8 of the summed pair-energy magnitudes
+    assert abs(urm - ur) <= max(1e-6 * abs(ur), 1e-8 * pair_energy_scale(p, x))
+
+
 def test_regularised_ext_mixed_vs_fp64():
     say("case regularised ext")
     p, x, _ = make_plasma(700, 600, seed=62, recip=RECIP_NONE)
