@@ -9,6 +9,7 @@
 
 namespace tcp {
 constexpr int kEvPool = 64;  // timed pair launches kept per call
+constexpr int kStageChunks = 16;  // pipelined pieces of one host transfer
 }
 
 struct tcp_ctx {
@@ -68,6 +69,7 @@ struct tcp_ctx {
     // timing
     hipEvent_t ev_pair[2][tcp::kEvPool];
     hipEvent_t ev_all[2];
+    hipEvent_t ev_stage[tcp::kStageChunks];  // download chunks of the host staging
     bool events = false;
     double pair_ms = 0, steps_ms = 0;
     double ur_scale = 0.5;

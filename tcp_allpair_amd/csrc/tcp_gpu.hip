@@ -314,14 +314,39 @@ struct Aos3 {
     double *x, *y, *z;
 };
 
+// The k arrays of one call, concatenated, move in at most kStageChunks pieces
+// of at least 128 Ki doubles (1 MiB): the host copy of one piece overlaps the
+// DMA of the previous one.
+size_t stage_chunk(size_t total)
+{
+    const size_t by_count = (total + kStageChunks - 1) / kStageChunks;
+    return by_count > ((size_t)1 << 17) ? by_count : ((size_t)1 << 17);
+}
+
+// doubles [off, off + len) of the concatenated host arrays, to or from dst/src
+void host_piece(const Aos3* t, size_t m, size_t off, size_t len, double* stage, bool in)
+{
+    while (len > 0) {
+        const size_t j = off / m, o = off % m, l = len < m - o ? len : m - o;
+        if (in) std::memcpy(stage, t[j].host + o, l * sizeof(double));
+        else std::memcpy(t[j].host + o, stage, l * sizeof(double));
+        stage += l;
+        off += l;
+        len -= l;
+    }
+}
+
 int upload_aos(tcp_ctx* c, const Aos3* t, int k, int n)
 {
     if (k == 0 || n == 0) return 0;
-    const size_t m = 3 * (size_t)n;
-    TRY(ensure_stage(c, m * k));
-    for (int j = 0; j < k; ++j) std::memcpy(c->h_stage + m * j, t[j].host, m * sizeof(double));
-    HIPCK(c, hipMemcpyAsync(c->d_stage, c->h_stage, m * k * sizeof(double), hipMemcpyHostToDevice,
-                            c->stream));
+    const size_t m = 3 * (size_t)n, total = m * k, chunk = stage_chunk(total);
+    TRY(ensure_stage(c, total));
+    for (size_t off = 0; off < total; off += chunk) {
+        const size_t len = total - off < chunk ? total - off : chunk;
+        host_piece(t, m, off, len, c->h_stage + off, true);
+        HIPCK(c, hipMemcpyAsync(c->d_stage + off, c->h_stage + off, len * sizeof(double),
+                                hipMemcpyHostToDevice, c->stream));
+    }
     for (int j = 0; j < k; ++j) launch_aos_to_soa(c->d_stage + m * j, t[j].x, t[j].y, t[j].z, n, c->stream);
     LAUNCHCK(c);
     return sync(c);
@@ -330,14 +355,23 @@ int upload_aos(tcp_ctx* c, const Aos3* t, int k, int n)
 int download_aos(tcp_ctx* c, const Aos3* t, int k, int n)
 {
     if (k == 0 || n == 0) return 0;
-    const size_t m = 3 * (size_t)n;
-    TRY(ensure_stage(c, m * k));
+    const size_t m = 3 * (size_t)n, total = m * k, chunk = stage_chunk(total);
+    TRY(ensure_stage(c, total));
     for (int j = 0; j < k; ++j) launch_soa_to_aos(t[j].x, t[j].y, t[j].z, c->d_stage + m * j, n, c->stream);
     LAUNCHCK(c);
-    HIPCK(c, hipMemcpyAsync(c->h_stage, c->d_stage, m * k * sizeof(double), hipMemcpyDeviceToHost,
-                            c->stream));
-    TRY(sync(c));
-    for (int j = 0; j < k; ++j) std::memcpy(t[j].host, c->h_stage + m * j, m * sizeof(double));
+    int e = 0;
+    for (size_t off = 0; off < total; off += chunk, ++e) {
+        const size_t len = total - off < chunk ? total - off : chunk;
+        HIPCK(c, hipMemcpyAsync(c->h_stage + off, c->d_stage + off, len * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipEventRecord(c->ev_stage[e], c->stream));
+    }
+    e = 0;
+    for (size_t off = 0; off < total; off += chunk, ++e) {
+        const size_t len = total - off < chunk ? total - off : chunk;
+        HIPCK(c, hipEventSynchronize(c->ev_stage[e]));
+        host_piece(t, m, off, len, c->h_stage + off, false);
+    }
     return 0;
 }
 
@@ -525,6 +559,9 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
                 return bail(fail(c, TCP_ERR_HIP, "hipEventCreate failed"));
     if (hipEventCreate(&c->ev_all[0]) != hipSuccess || hipEventCreate(&c->ev_all[1]) != hipSuccess)
         return bail(fail(c, TCP_ERR_HIP, "hipEventCreate failed"));
+    for (auto& e : c->ev_stage)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(c, TCP_ERR_HIP, "hipEventCreate failed"));
     c->events = true;
     const size_t n = c->N;
     int rc = 0;
@@ -597,6 +634,7 @@ int tcp_destroy(tcp_ctx* c)
             for (int k = 0; k < 2; ++k) (void)hipEventDestroy(c->ev_pair[k][e]);
         (void)hipEventDestroy(c->ev_all[0]);
         (void)hipEventDestroy(c->ev_all[1]);
+        for (auto e : c->ev_stage) (void)hipEventDestroy(e);
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
