@@ -172,7 +172,10 @@ int set_shape(tcp_batch* b, int shape)
     BHIP(b, hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(PairItem),
                            hipMemcpyHostToDevice, b->stream));
     b->nitems = (int)items.size();
-    for (int r = 0; r < b->nrep; ++r) b->h_reps[r].slots = slots_of(b, b->reps[r], shape);
+    for (int r = 0; r < b->nrep; ++r) {
+        b->h_reps[r].slots = slots_of(b, b->reps[r], shape);
+        b->h_reps[r].pu_row = n3_shape(mixed(b) ? shape : 0).tiles * 64;
+    }
     BHIP(b, hipMemcpyAsync(b->d_reps, b->h_reps.data(), b->nrep * sizeof(BatchRep),
                            hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));

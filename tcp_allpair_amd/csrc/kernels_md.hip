@@ -112,9 +112,12 @@ __global__ __launch_bounds__(kRedBlock) void upos_kernel(MdArgs m)
 }
 
 // f = sum over chunks of the pair partials (+ reciprocal forces), block
-// partials of sum f (3) and of sum u (1).
+// partials of sum f (3) and of sum u (1).  pu_row > 0 (the Newton kernels'
+// block-row width): the energy slots of particle i below its own block row
+// i / pu_row hold only zeros and are not written, so they are skipped (the
+// sum is the same, bit for bit).
 __device__ inline void combine_body(const MdArgs& m, int blk, const double* pf, const double* pu,
-                                    int nchunk, const double* frx, const double* fry,
+                                    int nchunk, int pu_row, const double* frx, const double* fry,
                                     const double* frz, double* part, int accumulate)
 {
     const int i = blk * kRedBlock + threadIdx.x;
@@ -122,12 +125,13 @@ __device__ inline void combine_body(const MdArgs& m, int blk, const double* pf, 
     if (i < m.n) {
         const size_t n = m.n;
         double f3[3] = {0.0, 0.0, 0.0}, u = 0.0;
+        const int c0 = pu_row > 0 ? i / pu_row : 0;  // uniform across a wavefront
         for (int c = 0; c < nchunk; ++c) {
             const double* p = pf + (size_t)c * 3 * n;
             f3[0] += p[i];
             f3[1] += p[n + i];
             f3[2] += p[2 * n + i];
-            u += pu[(size_t)c * n + i];
+            if (c >= c0) u += pu[(size_t)c * n + i];
         }
         if (accumulate) {
             f3[0] = m.fx[i] + f3[0];
@@ -149,11 +153,11 @@ __device__ inline void combine_body(const MdArgs& m, int blk, const double* pf, 
 
 __global__ __launch_bounds__(kRedBlock) void combine_kernel(MdArgs m, const double* pf,
                                                             const double* pu, int nchunk,
-                                                            const double* frx, const double* fry,
-                                                            const double* frz, double* part,
-                                                            int accumulate)
+                                                            int pu_row, const double* frx,
+                                                            const double* fry, const double* frz,
+                                                            double* part, int accumulate)
 {
-    combine_body(m, blockIdx.x, pf, pu, nchunk, frx, fry, frz, part, accumulate);
+    combine_body(m, blockIdx.x, pf, pu, nchunk, pu_row, frx, fry, frz, part, accumulate);
 }
 
 // sums[c] = sum over blocks of part[b * width + c], in block order (one block).
@@ -413,7 +417,7 @@ __global__ __launch_bounds__(kRedBlock) void combine_batch_kernel(const BatchRep
     const BatchRep& r = reps[blockIdx.y];
     if (!batch_block_live(r)) return;
     const bool rec = r.recip != TCP_RECIP_NONE;
-    combine_body(r.m, blockIdx.x, r.pa.pf, r.pa.pu, r.slots, rec ? r.ra.frx : nullptr,
+    combine_body(r.m, blockIdx.x, r.pa.pf, r.pa.pu, r.slots, r.pu_row, rec ? r.ra.frx : nullptr,
                  rec ? r.ra.fry : nullptr, rec ? r.ra.frz : nullptr, r.part, 0);
 }
 
@@ -528,12 +532,12 @@ void launch_make_upos(const MdArgs& m, hipStream_t s)
     hipLaunchKernelGGL(upos_kernel, dim3(md_blocks(m.n)), dim3(kRedBlock), 0, s, m);
 }
 
-void launch_combine(const MdArgs& m, const double* pf, const double* pu, int nchunk,
+void launch_combine(const MdArgs& m, const double* pf, const double* pu, int nchunk, int pu_row,
                     const double* frx, const double* fry, const double* frz, double* part,
                     bool accumulate, hipStream_t s)
 {
     hipLaunchKernelGGL(combine_kernel, dim3(md_blocks(m.n)), dim3(kRedBlock), 0, s, m, pf, pu,
-                       nchunk, frx, fry, frz, part, (int)accumulate);
+                       nchunk, pu_row, frx, fry, frz, part, (int)accumulate);
 }
 
 void launch_reduce_parts(const double* part, int nblk, int width, double* sums, hipStream_t s)

@@ -1018,7 +1018,7 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
         pf[jp] = racc[0][p];
         pf[n + jp] = racc[1][p];
         pf[2 * n + jp] = racc[2][p];
-        a.pu[(size_t)br * n + jp] = racc[3][p];
+        if (dblk) a.pu[(size_t)br * n + jp] = racc[3][p];  // zero off the diagonal: not stored
     }
     if (kRdf) {
         __syncthreads();
@@ -1330,7 +1330,7 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L
         pf[jp] = L.racc[0][p];
         pf[n + jp] = L.racc[1][p];
         pf[2 * n + jp] = L.racc[2][p];
-        a.pu[(size_t)br * n + jp] = L.racc[3][p];
+        if (dblk) a.pu[(size_t)br * n + jp] = L.racc[3][p];  // zero off the diagonal: not stored
     }
     if (kRdf) {
         __syncthreads();

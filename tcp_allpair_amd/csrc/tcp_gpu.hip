@@ -205,7 +205,8 @@ int direct_all(tcp_ctx* c, int rdf_weight, const double* frx, const double* fry,
     if (ev_slot >= 0) HIPCK(c, hipEventRecord(c->ev_pair[1][ev_slot], c->stream));
     // the Newton kernels give each unordered pair's energy once: Ur = sum u
     c->ur_scale = 1.0;
-    launch_combine(md_args(c), c->pf, c->pu, c->n3_rows, frx, fry, frz, c->part, false, c->stream);
+    launch_combine(md_args(c), c->pf, c->pu, c->n3_rows, n3_shape(c->n3_shape).tiles * 64, frx, fry,
+                   frz, c->part, false, c->stream);
     LAUNCHCK(c);
     launch_reduce_parts(c->part, md_blocks(c->N), 4, c->sc + SC_FSUM, c->stream);
     LAUNCHCK(c);
@@ -751,7 +752,7 @@ int tcp_force_direct_ext(tcp_ctx* c, const double* xg, int ne_g, int ni_g, doubl
     a.pu = c->pu;
     launch_pair(c, a, nc);
     LAUNCHCK(c);
-    launch_combine(md_args(c), c->pf, c->pu, nc, nullptr, nullptr, nullptr, c->part, true, c->stream);
+    launch_combine(md_args(c), c->pf, c->pu, nc, 0, nullptr, nullptr, nullptr, c->part, true, c->stream);
     launch_reduce_parts(c->part, md_blocks(c->N), 4, c->sc + SC_FSUM, c->stream);
     LAUNCHCK(c);
     double u;
@@ -778,7 +779,7 @@ int tcp_force_direct_ext(tcp_ctx* c, const double* xg, int ne_g, int ni_g, doubl
     LAUNCHCK(c);
     // the block partials of the external block's combine: 4 per block of ng
     TRY(dgrow(c, &c->part, &c->part_cap, (size_t)4 * md_blocks(ng)));
-    launch_combine(mg, c->gpf, c->gpu, nc2, nullptr, nullptr, nullptr, c->part, false, c->stream);
+    launch_combine(mg, c->gpf, c->gpu, nc2, 0, nullptr, nullptr, nullptr, c->part, false, c->stream);
     LAUNCHCK(c);
     if (gemit) TRY(download3(c, gemit, c->gfx, c->gfy, c->gfz, ng));
     TRY(sync(c));

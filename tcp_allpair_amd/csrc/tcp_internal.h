@@ -160,8 +160,10 @@ __host__ __device__ inline int md_blocks_dev(int n) { return (n + kRedBlock - 1)
 void launch_vv1(const MdArgs& m, int thermo, double dt, double fric, hipStream_t s);
 void launch_make_upos(const MdArgs& m, hipStream_t s);
 // f (=|+=) sum over chunks of the pair partials (+ fr when frx != null);
-// block partials (sum f, sum u) into part[4 * md_blocks(n)].
-void launch_combine(const MdArgs& m, const double* pf, const double* pu, int nchunk,
+// block partials (sum f, sum u) into part[4 * md_blocks(n)].  pu_row: block-row
+// width of the Newton kernels, whose energy slots below a particle's own row
+// are not written (0: every slot is read).
+void launch_combine(const MdArgs& m, const double* pf, const double* pu, int nchunk, int pu_row,
                     const double* frx, const double* fry, const double* frz, double* part,
                     bool accumulate, hipStream_t s);
 // sums[c] = sum_b part[b * width + c] in block order (single block)
@@ -201,6 +203,7 @@ struct BatchRep {
     double* part;              // block partials
     double* sc;                // device scalars SC_*
     int slots;                 // pair partial slots summed by combine
+    int pu_row;                // particles per block row of the pair kernel's shape
     int recip;                 // TCP_RECIP_NONE or TCP_RECIP_EWALD
     int kmax;
     double alpha;
