@@ -2,7 +2,10 @@
 // builders the batch code (batch.hip) shares with it.
 #pragma once
 
+#include <condition_variable>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "tcp_internal.h"
@@ -10,7 +13,32 @@
 namespace tcp {
 constexpr int kEvPool = 64;  // timed pair launches kept per call
 constexpr int kStageChunks = 16;  // pipelined pieces of one host transfer
-}
+
+// Host memcpy split over the calling thread and a few resident workers (the
+// copies between the caller's arrays and the pinned staging buffer).  One pool
+// serves the process, created on first use and never torn down, so no exit
+// path waits on its threads; copy() calls from several host threads take
+// turns.  The workers only copy memory: no HIP or MPI call runs on them.
+class CopyPool {
+public:
+    static CopyPool& shared();
+    void copy(void* dst, const void* src, size_t bytes);
+
+private:
+    explicit CopyPool(int workers);
+    void work(int id);
+    const int parts_;
+    std::vector<std::thread> th_;
+    std::mutex job_;  // one copy() at a time
+    std::mutex mu_;
+    std::condition_variable go_, done_;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t bytes_ = 0;
+    unsigned gen_ = 0;
+    int left_ = 0;
+};
+}  // namespace tcp
 
 struct tcp_ctx {
     tcp_params p{};
@@ -59,8 +87,8 @@ struct tcp_ctx {
     tcp::FixPos* gupos = nullptr;
     double *gpf = nullptr, *gpu = nullptr;
     size_t gpf_cap = 0;
-    // host-state transfers: AoS arrays staged in pinned host memory, moved by one
-    // DMA per call and transposed to/from the SoA state on the device
+    // host arrays of tcp_set_state/get_state staged through pinned host memory,
+    // moved in pipelined pieces and transposed to/from the SoA state on the device
     double* h_stage = nullptr;  // hipHostMalloc
     double* d_stage = nullptr;  // hipMalloc
     size_t stage_cap = 0;       // doubles in each

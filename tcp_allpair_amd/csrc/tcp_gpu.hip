@@ -16,6 +16,61 @@
 
 using namespace tcp;
 
+namespace tcp {
+
+CopyPool::CopyPool(int workers) : parts_(workers + 1)
+{
+    for (int w = 1; w <= workers; ++w) th_.emplace_back(&CopyPool::work, this, w);
+}
+
+CopyPool& CopyPool::shared()
+{
+    static CopyPool* pool = new CopyPool(3);
+    return *pool;
+}
+
+// part id of every job: bytes [bytes id / parts, bytes (id + 1) / parts)
+void CopyPool::work(int id)
+{
+    unsigned seen = 0;
+    for (;;) {
+        char* d;
+        const char* src;
+        size_t n;
+        {
+            std::unique_lock<std::mutex> l(mu_);
+            go_.wait(l, [&] { return gen_ != seen; });
+            seen = gen_;
+            d = dst_;
+            src = src_;
+            n = bytes_;
+        }
+        const size_t lo = n * id / parts_, hi = n * (id + 1) / parts_;
+        std::memcpy(d + lo, src + lo, hi - lo);
+        std::lock_guard<std::mutex> l(mu_);
+        if (--left_ == 0) done_.notify_one();
+    }
+}
+
+void CopyPool::copy(void* dst, const void* src, size_t bytes)
+{
+    std::lock_guard<std::mutex> job(job_);
+    {
+        std::lock_guard<std::mutex> l(mu_);
+        dst_ = (char*)dst;
+        src_ = (const char*)src;
+        bytes_ = bytes;
+        left_ = parts_ - 1;
+        ++gen_;
+    }
+    go_.notify_all();
+    std::memcpy(dst, src, bytes / parts_);
+    std::unique_lock<std::mutex> l(mu_);
+    done_.wait(l, [&] { return left_ == 0; });
+}
+
+}  // namespace tcp
+
 namespace {
 
 constexpr double kPi = 3.1415926535897931;      // data.f90:23
@@ -324,13 +379,16 @@ size_t stage_chunk(size_t total)
     return by_count > ((size_t)1 << 17) ? by_count : ((size_t)1 << 17);
 }
 
-// doubles [off, off + len) of the concatenated host arrays, to or from dst/src
+// doubles [off, off + len) of the concatenated host arrays, into (in) or out
+// of the staging buffer; runs of 256 Ki bytes or more go through the pool
 void host_piece(const Aos3* t, size_t m, size_t off, size_t len, double* stage, bool in)
 {
     while (len > 0) {
         const size_t j = off / m, o = off % m, l = len < m - o ? len : m - o;
-        if (in) std::memcpy(stage, t[j].host + o, l * sizeof(double));
-        else std::memcpy(t[j].host + o, stage, l * sizeof(double));
+        void* d = in ? (void*)stage : (void*)(t[j].host + o);
+        const void* src = in ? (const void*)(t[j].host + o) : (const void*)stage;
+        if (l * sizeof(double) >= ((size_t)1 << 18)) CopyPool::shared().copy(d, src, l * sizeof(double));
+        else std::memcpy(d, src, l * sizeof(double));
         stage += l;
         off += l;
         len -= l;
