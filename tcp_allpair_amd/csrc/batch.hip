@@ -48,6 +48,7 @@ struct tcp_batch {
     std::vector<double*> pf, pu;   // per replica pair partials, slots for every shape
     PairItem* d_items = nullptr;
     int item_cap = 0, nitems = 0;
+    int nfine = 0;            // FP64: the last nfine items belong to 4 x 4 tile replicas
     int* d_queue = nullptr;
     uint64_t* d_counter = nullptr;  // [0] noise step counter, [1] trace row
     unsigned long long* d_rdf_scratch = nullptr;
@@ -56,6 +57,7 @@ struct tcp_batch {
     // persistent launch shape
     int shape = 0;
     int grid = 1;
+    int grid_fine = 1;        // FP64: resident blocks of the 4 x 4 tile launch
     // captured step
     hipGraphExec_t exec = nullptr;
     struct Key {
@@ -63,12 +65,12 @@ struct tcp_batch {
         double dt = 0, tau = 0, fric = 0;
         uint64_t seed = 0;
         double* trace = nullptr;
-        int shape = -1, grid = -1, nitems = -1;
+        int shape = -1, grid = -1, nitems = -1, nfine = -1, grid_fine = -1;
         bool operator==(const Key& o) const
         {
             return thermo == o.thermo && dt == o.dt && tau == o.tau && fric == o.fric &&
                    seed == o.seed && trace == o.trace && shape == o.shape && grid == o.grid &&
-                   nitems == o.nitems;
+                   nitems == o.nitems && nfine == o.nfine && grid_fine == o.grid_fine;
         }
     } key;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -132,39 +134,54 @@ int bgrow(tcp_batch* b, T** p, size_t count)
 
 bool mixed(const tcp_batch* b) { return b->precision == TCP_MIXED; }
 
-int slots_of(const tcp_batch* b, const tcp_ctx* c, int shape)
+// The block shape of a replica's pair work: the batch's in mixed precision;
+// in FP64 its own context's (8 x 8 or, for small systems, 4 x 4 tiles), so that
+// every replica keeps its standalone summation order.
+int rep_shape(const tcp_batch* b, const tcp_ctx* c, int shape)
 {
-    return n3_block_rows_shape(c->N, mixed(b) ? shape : 0);
+    return mixed(b) ? shape : c->n3_shape;
 }
 
-// Work list of the persistent pair launch, largest items first.
-std::vector<PairItem> make_items(const tcp_batch* b, int shape)
+int slots_of(const tcp_batch* b, const tcp_ctx* c, int shape)
 {
-    std::vector<PairItem> items;
+    return n3_block_rows_shape(c->N, rep_shape(b, c, shape));
+}
+
+// Work list of the persistent pair launch, largest items first.  FP64: the
+// items of 8 x 8 tile replicas, then (*nfine of them) those of 4 x 4 tile
+// replicas, each group sorted on its own (one launch per group).
+std::vector<PairItem> make_items(const tcp_batch* b, int shape, int* nfine)
+{
+    std::vector<PairItem> items[2];
     for (int r = 0; r < b->nrep; ++r) {
         const tcp_ctx* c = b->reps[r];
-        // FP64 runs the 8 x 8 tile blocks of shape 0
-        const int bt = n3_shape(mixed(b) ? shape : 0).tiles;
+        const int rs = rep_shape(b, c, shape);
+        const int bt = n3_shape(rs).tiles;
         const int ntiles = (c->N + 63) / 64;
-        const int rows = n3_block_rows_shape(c->N, mixed(b) ? shape : 0);
+        const int rows = n3_block_rows_shape(c->N, rs);
+        auto& out = items[!mixed(b) && rs == kFineN3Shape ? 1 : 0];
         for (int br = 0; br < rows; ++br) {
             const int ti = std::min(bt, ntiles - br * bt);
             for (int bc = br; bc < rows; ++bc) {
                 const int tj = std::min(bt, ntiles - bc * bt);
                 const int cost = bc == br ? ti * (ti + 1) / 2 : ti * tj;
-                items.push_back(PairItem{r, br, bc, cost});
+                out.push_back(PairItem{r, br, bc, cost});
             }
         }
     }
-    std::stable_sort(items.begin(), items.end(),
-                     [](const PairItem& x, const PairItem& y) { return x.cost > y.cost; });
-    return items;
+    for (auto& v : items)
+        std::stable_sort(v.begin(), v.end(),
+                         [](const PairItem& x, const PairItem& y) { return x.cost > y.cost; });
+    *nfine = (int)items[1].size();
+    items[0].insert(items[0].end(), items[1].begin(), items[1].end());
+    return items[0];
 }
 
 // Uploads the work list and the replica table for a block shape.
 int set_shape(tcp_batch* b, int shape)
 {
-    const std::vector<PairItem> items = make_items(b, shape);
+    int nfine = 0;
+    const std::vector<PairItem> items = make_items(b, shape, &nfine);
     if ((int)items.size() > b->item_cap) {
         BTRY(bgrow(b, &b->d_items, items.size()));
         b->item_cap = (int)items.size();
@@ -172,15 +189,17 @@ int set_shape(tcp_batch* b, int shape)
     BHIP(b, hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(PairItem),
                            hipMemcpyHostToDevice, b->stream));
     b->nitems = (int)items.size();
+    b->nfine = nfine;
     for (int r = 0; r < b->nrep; ++r) {
         b->h_reps[r].slots = slots_of(b, b->reps[r], shape);
-        b->h_reps[r].pu_row = n3_shape(mixed(b) ? shape : 0).tiles * 64;
+        b->h_reps[r].pu_row = n3_shape(rep_shape(b, b->reps[r], shape)).tiles * 64;
     }
     BHIP(b, hipMemcpyAsync(b->d_reps, b->h_reps.data(), b->nrep * sizeof(BatchRep),
                            hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
     b->shape = shape;
-    b->grid = b->cus * pair_batch_occupancy(mixed(b), shape, true, b->kelbg);
+    b->grid = b->cus * pair_batch_occupancy(mixed(b), mixed(b) ? shape : 0, true, b->kelbg);
+    b->grid_fine = b->cus * pair_batch_occupancy(false, kFineN3Shape, true, b->kelbg);
     return 0;
 }
 
@@ -189,9 +208,16 @@ void launch_pair(tcp_batch* b)
     if (mixed(b))
         launch_pair_mixed_n3_batch(b->d_reps, b->d_items, b->nitems, b->d_queue, b->shape, true,
                                    b->kelbg, b->grid, b->stream);
-    else
-        launch_pair_f64_batch(b->d_reps, b->d_items, b->nitems, b->d_queue, true, b->kelbg,
-                              b->grid, b->stream);
+    else {
+        // two persistent launches, each with its own queue (d_queue[0..1], [2..3])
+        const int nfull = b->nitems - b->nfine;
+        if (nfull > 0)
+            launch_pair_f64_batch(b->d_reps, b->d_items, nfull, b->d_queue, true, b->kelbg, b->grid,
+                                  0, b->stream);
+        if (b->nfine > 0)
+            launch_pair_f64_batch(b->d_reps, b->d_items + nfull, b->nfine, b->d_queue + 2, true,
+                                  b->kelbg, b->grid_fine, kFineN3Shape, b->stream);
+    }
 }
 
 // One MD step of every replica, in the order of tcp_md_steps (main.f90:87-153).
@@ -304,10 +330,10 @@ int tcp_batch_create(tcp_batch** out, int nrep, const tcp_params* p, int precisi
         if (hipEventCreate(&e) != hipSuccess) return bail(bfail(b, TCP_ERR_HIP, "hipEventCreate failed"));
     int rc = 0;
     if ((rc = balloc(b, &b->d_reps, (size_t)nrep))) return bail(rc);
-    if ((rc = balloc(b, &b->d_queue, (size_t)2))) return bail(rc);
+    if ((rc = balloc(b, &b->d_queue, (size_t)4))) return bail(rc);  // two persistent launches
     if ((rc = balloc(b, &b->d_counter, (size_t)2))) return bail(rc);
     if ((rc = balloc(b, &b->d_rdf_scratch, (size_t)3 * kHistStride))) return bail(rc);
-    if (hipMemset(b->d_queue, 0, 2 * sizeof(int)) != hipSuccess ||
+    if (hipMemset(b->d_queue, 0, 4 * sizeof(int)) != hipSuccess ||
         hipMemset(b->d_counter, 0, 2 * sizeof(uint64_t)) != hipSuccess)
         return bail(bfail(b, TCP_ERR_HIP, "hipMemset failed"));
     b->h_reps.resize(nrep);
@@ -447,6 +473,8 @@ int tcp_batch_md_steps(tcp_batch* b, int nsteps, int thermo, double dt, const do
     k.shape = b->shape;
     k.grid = b->grid;
     k.nitems = b->nitems;
+    k.nfine = b->nfine;
+    k.grid_fine = b->grid_fine;
     BTRY(capture_step(b, k));
     BHIP(b, hipEventRecord(b->ev[2], b->stream));
     for (int s = 0; s < nsteps; ++s) BHIP(b, hipGraphLaunch(b->exec, b->stream));

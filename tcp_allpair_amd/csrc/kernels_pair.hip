@@ -1059,6 +1059,7 @@ __device__ inline void queue_done(int* queue)
 // j tile is done, so the result does not depend on scheduling.
 
 constexpr int kF64Waves = 8;
+constexpr int kFineTiles = 4;  // tiles per block side of the small-system shape
 
 __device__ inline double rot_d(double v)
 {
@@ -1238,26 +1239,29 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
 
 // LDS of one FP64 block: j positions and flags, the block's reaction sums
 // (slot br), the per-wave reaction buffer and the RDF histogram.
+template <int kBT>
 struct F64Lds {
-    F64J jp[kBlkTiles * kTile];
-    double racc[4][kBlkTiles * kTile];
-    double wbuf[kF64Waves][3][kTile];
+    F64J jp[kBT * kTile];
+    double racc[4][kBT * kTile];
+    double wbuf[kBT][3][kTile];
     unsigned hist[3 * kHistStride];
     double etab[kErfcxN * (kErfcxDeg + 1)] __attribute__((aligned(16)));
     double e2tab[kExp2N];
 };
 
-__device__ inline void load_tables(F64Lds& L)
+template <int kBT>
+__device__ inline void load_tables(F64Lds<kBT>& L)
 {
     const double* src = &kErfcxTab[0][0];
     for (int i = threadIdx.x; i < kErfcxN * (kErfcxDeg + 1); i += blockDim.x) L.etab[i] = src[i];
     for (int i = threadIdx.x; i < kExp2N; i += blockDim.x) L.e2tab[i] = kExp2Tab[i];
 }
 
-template <bool kRdf, bool kKelbg>
-__device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L)
+// kBT x kBT tiles, one i tile per wave (kBT waves)
+template <bool kRdf, bool kKelbg, int kBT>
+__device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds<kBT>& L)
 {
-    constexpr int kBP = kBlkTiles * kTile;
+    constexpr int kBP = kBT * kTile;
     const bool dblk = br == bc;
     const int n = a.n_i, ne = a.ne_i;
     for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
@@ -1270,7 +1274,7 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L
     __syncthreads();
     const int wave = threadIdx.x / kTile, lane = threadIdx.x % kTile;
     const int ntiles = (n + kTile - 1) / kTile;
-    const int I = br * kBlkTiles + wave;
+    const int I = br * kBT + wave;
     const bool exists = I < ntiles;
     const int ip = I * kTile + lane;
     const bool ilive = exists && ip < n;
@@ -1278,8 +1282,8 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L
     const double xi = a.pi.x[ic], yi = a.pi.y[ic], zi = a.pi.z[ic];
     const int si = ip >= ne;
     double Fx = 0.0, Fy = 0.0, Fz = 0.0, U = 0.0;
-    for (int tj = 0; tj < kBlkTiles; ++tj) {
-        const int J = bc * kBlkTiles + tj;
+    for (int tj = 0; tj < kBT; ++tj) {
+        const int J = bc * kBT + tj;
         double rx = 0.0, ry = 0.0, rz = 0.0;
         if (exists && J < ntiles && (!dblk || tj >= wave)) {
             const F64J* jt = L.jp + tj * kTile;
@@ -1302,7 +1306,7 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds& L
         for (int t = threadIdx.x; t < 3 * kTile; t += blockDim.x) {
             const int c = t / kTile, p = t % kTile;
             double s = L.racc[c][tj * kTile + p];
-            for (int w = 0; w < kF64Waves; ++w) s += L.wbuf[w][c][p];
+            for (int w = 0; w < kBT; ++w) s += L.wbuf[w][c][p];
             L.racc[c][tj * kTile + p] = s;
         }
         __syncthreads();
@@ -1344,31 +1348,50 @@ __global__ __launch_bounds__(kF64Waves * kTile, 4) void pair_f64_n3_kernel(PairA
                                                                         const int2* blkmap,
                                                                         int nblk, int* queue)
 {
-    __shared__ F64Lds lds;
+    __shared__ F64Lds<kF64Waves> lds;
     __shared__ int slot;
     load_tables(lds);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nblk) break;
         const int2 rc = blkmap[it];
-        n3_block_f64<kRdf, kKelbg>(a, rc.x, rc.y, lds);
+        n3_block_f64<kRdf, kKelbg, kF64Waves>(a, rc.x, rc.y, lds);
     }
     queue_done(queue);
 }
 
+// The same over blocks of 4 x 4 tiles by four waves, for small systems (47 KB
+// of LDS: three blocks, three waves per SIMD).
 template <bool kRdf, bool kKelbg>
-__global__ __launch_bounds__(kF64Waves * kTile, 4) void pair_f64_n3_batch_kernel(const BatchRep* reps,
-                                                                              const PairItem* items,
-                                                                              int nitems, int* queue)
+__global__ __launch_bounds__(kFineTiles * kTile, 3) void pair_f64_n3_fine_kernel(PairArgs a,
+                                                                              const int2* blkmap,
+                                                                              int nblk, int* queue)
 {
-    __shared__ F64Lds lds;
+    __shared__ F64Lds<kFineTiles> lds;
+    __shared__ int slot;
+    load_tables(lds);  // the first queue_next orders it before use
+    for (;;) {
+        const int it = queue_next(queue, &slot);
+        if (it >= nblk) break;
+        const int2 rc = blkmap[it];
+        n3_block_f64<kRdf, kKelbg, kFineTiles>(a, rc.x, rc.y, lds);
+    }
+    queue_done(queue);
+}
+
+template <bool kRdf, bool kKelbg, int kBT>
+__global__ __launch_bounds__(kBT * kTile, kBT == kFineTiles ? 3 : 4) void pair_f64_n3_batch_kernel(const BatchRep* reps,
+                                                                        const PairItem* items,
+                                                                        int nitems, int* queue)
+{
+    __shared__ F64Lds<kBT> lds;
     __shared__ int slot;
     load_tables(lds);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nitems) break;
         const PairItem w = items[it];
-        n3_block_f64<kRdf, kKelbg>(reps[w.rep].pa, w.a, w.b, lds);
+        n3_block_f64<kRdf, kKelbg, kBT>(reps[w.rep].pa, w.a, w.b, lds);
     }
     queue_done(queue);
 }
@@ -1487,22 +1510,43 @@ int pair_n3_occupancy(bool rdf, bool kelbg, int shape)
     return nb < 1 ? 1 : nb;
 }
 
-int pair_f64_n3_occupancy(bool rdf, bool kelbg)
+namespace {
+
+template <bool kRdf, bool kKelbg>
+const void* f64_n3_fn(int shape)
 {
-    const void* f = rdf ? (kelbg ? (const void*)pair_f64_n3_kernel<true, true>
-                                 : (const void*)pair_f64_n3_kernel<true, false>)
-                        : (kelbg ? (const void*)pair_f64_n3_kernel<false, true>
-                                 : (const void*)pair_f64_n3_kernel<false, false>);
+    return shape == kFineN3Shape ? (const void*)pair_f64_n3_fine_kernel<kRdf, kKelbg>
+                                 : (const void*)pair_f64_n3_kernel<kRdf, kKelbg>;
+}
+
+int f64_threads(int shape) { return (shape == kFineN3Shape ? kFineTiles : kF64Waves) * kTile; }
+
+}  // namespace
+
+int pair_f64_n3_occupancy(bool rdf, bool kelbg, int shape)
+{
+    const void* f = rdf ? (kelbg ? f64_n3_fn<true, true>(shape) : f64_n3_fn<true, false>(shape))
+                        : (kelbg ? f64_n3_fn<false, true>(shape) : f64_n3_fn<false, false>(shape));
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kF64Waves * kTile, 0) != hipSuccess) nb = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, f64_threads(shape), 0) != hipSuccess) nb = 1;
     return nb < 1 ? 1 : nb;
 }
 
 void launch_pair_f64_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
-                        hipStream_t s)
+                        int shape, hipStream_t s)
 {
-    const dim3 g(grid), b(kF64Waves * kTile);
+    const dim3 g(grid), b(f64_threads(shape));
     const bool kel = (a.flags & TCP_KELBG) != 0;
+    if (shape == kFineN3Shape) {
+        if (a.rdf_weight) {
+            if (kel) hipLaunchKernelGGL((pair_f64_n3_fine_kernel<true, true>), g, b, 0, s, a, blkmap, nblk, queue);
+            else hipLaunchKernelGGL((pair_f64_n3_fine_kernel<true, false>), g, b, 0, s, a, blkmap, nblk, queue);
+        } else {
+            if (kel) hipLaunchKernelGGL((pair_f64_n3_fine_kernel<false, true>), g, b, 0, s, a, blkmap, nblk, queue);
+            else hipLaunchKernelGGL((pair_f64_n3_fine_kernel<false, false>), g, b, 0, s, a, blkmap, nblk, queue);
+        }
+        return;
+    }
     if (a.rdf_weight) {
         if (kel) hipLaunchKernelGGL((pair_f64_n3_kernel<true, true>), g, b, 0, s, a, blkmap, nblk, queue);
         else hipLaunchKernelGGL((pair_f64_n3_kernel<true, false>), g, b, 0, s, a, blkmap, nblk, queue);
@@ -1595,17 +1639,38 @@ void launch_pair_mixed_n3_batch(const BatchRep* reps, const PairItem* items, int
     (void)hipLaunchKernel(n3_batch_kernel(shape, rdf, kelbg), g, b, args, 0, s);
 }
 
-void launch_pair_f64_batch(const BatchRep* reps, const PairItem* items, int nitems, int* queue,
-                           bool rdf, bool kelbg, int grid, hipStream_t s)
+namespace {
+
+template <int kBT>
+void launch_f64_batch_bt(const BatchRep* reps, const PairItem* items, int nitems, int* queue,
+                         bool rdf, bool kelbg, int grid, hipStream_t s)
 {
-    const dim3 g(grid), b(kF64Waves * kTile);
+    const dim3 g(grid), b(kBT * kTile);
     if (rdf) {
-        if (kelbg) hipLaunchKernelGGL((pair_f64_n3_batch_kernel<true, true>), g, b, 0, s, reps, items, nitems, queue);
-        else hipLaunchKernelGGL((pair_f64_n3_batch_kernel<true, false>), g, b, 0, s, reps, items, nitems, queue);
+        if (kelbg) hipLaunchKernelGGL((pair_f64_n3_batch_kernel<true, true, kBT>), g, b, 0, s, reps, items, nitems, queue);
+        else hipLaunchKernelGGL((pair_f64_n3_batch_kernel<true, false, kBT>), g, b, 0, s, reps, items, nitems, queue);
     } else {
-        if (kelbg) hipLaunchKernelGGL((pair_f64_n3_batch_kernel<false, true>), g, b, 0, s, reps, items, nitems, queue);
-        else hipLaunchKernelGGL((pair_f64_n3_batch_kernel<false, false>), g, b, 0, s, reps, items, nitems, queue);
+        if (kelbg) hipLaunchKernelGGL((pair_f64_n3_batch_kernel<false, true, kBT>), g, b, 0, s, reps, items, nitems, queue);
+        else hipLaunchKernelGGL((pair_f64_n3_batch_kernel<false, false, kBT>), g, b, 0, s, reps, items, nitems, queue);
     }
+}
+
+template <int kBT>
+const void* f64_batch_fn(bool rdf, bool kelbg)
+{
+    return rdf ? (kelbg ? (const void*)pair_f64_n3_batch_kernel<true, true, kBT>
+                        : (const void*)pair_f64_n3_batch_kernel<true, false, kBT>)
+               : (kelbg ? (const void*)pair_f64_n3_batch_kernel<false, true, kBT>
+                        : (const void*)pair_f64_n3_batch_kernel<false, false, kBT>);
+}
+
+}  // namespace
+
+void launch_pair_f64_batch(const BatchRep* reps, const PairItem* items, int nitems, int* queue,
+                           bool rdf, bool kelbg, int grid, int shape, hipStream_t s)
+{
+    if (shape == kFineN3Shape) launch_f64_batch_bt<kFineTiles>(reps, items, nitems, queue, rdf, kelbg, grid, s);
+    else launch_f64_batch_bt<kF64Waves>(reps, items, nitems, queue, rdf, kelbg, grid, s);
 }
 
 int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg)
@@ -1617,11 +1682,9 @@ int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg)
             hipSuccess)
             nb = 1;
     } else {
-        const void* f = rdf ? (kelbg ? (const void*)pair_f64_n3_batch_kernel<true, true>
-                                     : (const void*)pair_f64_n3_batch_kernel<true, false>)
-                            : (kelbg ? (const void*)pair_f64_n3_batch_kernel<false, true>
-                                     : (const void*)pair_f64_n3_batch_kernel<false, false>);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kF64Waves * kTile, 0) != hipSuccess)
+        const void* f = shape == kFineN3Shape ? f64_batch_fn<kFineTiles>(rdf, kelbg)
+                                              : f64_batch_fn<kF64Waves>(rdf, kelbg);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, f64_threads(shape), 0) != hipSuccess)
             nb = 1;
     }
     return nb < 1 ? 1 : nb;

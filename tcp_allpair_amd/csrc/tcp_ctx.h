@@ -58,7 +58,7 @@ struct tcp_ctx {
     int2* blkmap = nullptr;
     int* n3_queue = nullptr;  // work queue of the persistent Newton kernel
     int n3_grid = 1;          // its resident blocks
-    int n3_shape = 0;         // its block shape (0, or kFineN3Shape for small mixed systems)
+    int n3_shape = 0;         // its block shape (0, or kFineN3Shape for small systems)
     size_t pf_cap = 0;    // doubles
     size_t part_cap = 0;  // doubles of part
     double *pf = nullptr, *pu = nullptr;

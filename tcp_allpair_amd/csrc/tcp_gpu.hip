@@ -255,7 +255,8 @@ int direct_all(tcp_ctx* c, int rdf_weight, const double* frx, const double* fry,
         launch_pair_mixed_n3(a, c->blkmap, c->n3_blocks, c->n3_queue, c->n3_grid, c->n3_shape,
                              c->stream);
     else
-        launch_pair_f64_n3(a, c->blkmap, c->n3_blocks, c->n3_queue, c->n3_grid, c->stream);
+        launch_pair_f64_n3(a, c->blkmap, c->n3_blocks, c->n3_queue, c->n3_grid, c->n3_shape,
+                           c->stream);
     LAUNCHCK(c);
     if (ev_slot >= 0) HIPCK(c, hipEventRecord(c->ev_pair[1][ev_slot], c->stream));
     // the Newton kernels give each unordered pair's energy once: Ur = sum u
@@ -635,16 +636,13 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
             return bail(fail(c, TCP_ERR_HIP, "no multiprocessor count"));
         const bool kel = (p->flags & TCP_KELBG) != 0;
-        // mixed: 4 x 4 tile blocks when the 8 x 8 ones are fewer than twice the
+        // 4 x 4 tile blocks when the 8 x 8 ones are fewer than twice the
         // resident blocks (a function of N and the CU count only)
-        c->n3_shape = 0;
-        if (precision == TCP_MIXED) {
-            const int64_t rows0 = n3_block_rows_shape(c->N, 0);
-            if (rows0 * (rows0 + 1) / 2 < 2 * (int64_t)cus * pair_n3_occupancy(true, kel, 0))
-                c->n3_shape = kFineN3Shape;
-        }
-        c->n3_grid = cus * (precision == TCP_MIXED ? pair_n3_occupancy(true, kel, c->n3_shape)
-                                                   : pair_f64_n3_occupancy(true, kel));
+        const bool mixed = precision == TCP_MIXED;
+        c->n3_shape = n3_self_shape(c->N, cus * (mixed ? pair_n3_occupancy(true, kel, 0)
+                                                       : pair_f64_n3_occupancy(true, kel, 0)));
+        c->n3_grid = cus * (mixed ? pair_n3_occupancy(true, kel, c->n3_shape)
+                                  : pair_f64_n3_occupancy(true, kel, c->n3_shape));
         // upper triangle of block pairs, off-diagonal (full) blocks first
         c->n3_rows = n3_block_rows_shape(c->N, c->n3_shape);
         std::vector<int2> map;

@@ -66,9 +66,9 @@ void launch_pair_mixed(const PairArgs& a, int nchunk, hipStream_t s);
 int pair_n3_occupancy(bool rdf, bool kelbg, int shape);
 void launch_pair_mixed_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
                           int shape, hipStream_t s);
-int pair_f64_n3_occupancy(bool rdf, bool kelbg);
+int pair_f64_n3_occupancy(bool rdf, bool kelbg, int shape);
 void launch_pair_f64_n3(const PairArgs& a, const int2* blkmap, int nblk, int* queue, int grid,
-                        hipStream_t s);
+                        int shape, hipStream_t s);
 
 // SPME / Ewald reciprocal part
 struct RecipArgs {
@@ -240,16 +240,23 @@ struct N3Shape {
 };
 N3Shape n3_shape(int shape);
 int n3_block_rows_shape(int n, int shape);
-// the 4 x 4 tile, four-wave shape that a single mixed context takes when the
-// 8 x 8 tile blocks are fewer than twice its resident blocks
+// the 4 x 4 tile, four-wave shape that a single context takes when the 8 x 8
+// tile blocks are fewer than twice its resident blocks
 constexpr int kFineN3Shape = 3;
+// that rule, from the particle count and the resident blocks of shape 0
+inline int n3_self_shape(int n, int grid0)
+{
+    const int64_t rows0 = n3_block_rows_shape(n, 0);
+    return rows0 * (rows0 + 1) / 2 < 2 * (int64_t)grid0 ? kFineN3Shape : 0;
+}
 // Persistent launches over a work list; queue[2] must be zero before the first
 // launch and is left zero by every launch.  grid: resident blocks to start.
 void launch_pair_mixed_n3_batch(const BatchRep* reps, const PairItem* items, int nitems,
                                 int* queue, int shape, bool rdf, bool kelbg, int grid,
                                 hipStream_t s);
+// FP64: shape 0 or kFineN3Shape, the block shape of the items' replicas
 void launch_pair_f64_batch(const BatchRep* reps, const PairItem* items, int nitems, int* queue,
-                           bool rdf, bool kelbg, int grid, hipStream_t s);
+                           bool rdf, bool kelbg, int grid, int shape, hipStream_t s);
 // resident blocks per CU of the persistent kernels
 int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg);
 

@@ -100,6 +100,15 @@ def test_fp64_batch_no_recip_regularised_bitwise():
     assert_bitwise(replicas(RECIP_NONE, KELBG | REG_EE), FP64, 5, NOTEMP, reduced_dt(0.005))
 
 
+def test_fp64_batch_two_shapes_bitwise():
+    """FP64 replicas keep their own contexts' block shapes: the small ones 4 x 4
+    tiles and a 40000-particle one 8 x 8, in two persistent launches per step;
+    every replica matches its standalone run bit for bit."""
+    say("case batch fp64 two shapes")
+    reps = replicas(RECIP_NONE)[:3] + [make_plasma(20000, 20000, seed=121, recip=RECIP_NONE)]
+    assert_bitwise(reps, FP64, 2, NOTEMP, reduced_dt(0.005))
+
+
 @pytest.mark.parametrize("shape,large", [(3, False), (0, True)])
 def test_mixed_batch_matches_standalone(shape, large):
     """A single context runs the same kernel body on 4 x 4 tile blocks for
