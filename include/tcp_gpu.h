@@ -62,7 +62,8 @@ typedef struct tcp_params {
     double xm[2];            /* param%xm (amu) */
     double rc;               /* param%rc (A) */
     double rs;               /* param%rs: lambda of the electron-ion term (A) */
-    double alpha;            /* Ewald splitting; <= 0 selects 3.1/rc as force.f90:23 */
+    double alpha;            /* Ewald splitting; <= 0 selects 3.1/rc as force.f90:23.  TCP_MIXED
+                              * contexts need alpha*rc <= 4.5 (TCP_ERR_UNSUPPORTED beyond) */
     double eps;              /* e^2/(4 pi eps0), 14.399644154020082 eV*A (data.f90:21) */
     double dx_ee, dx_ei, dx_ii; /* NS%dx_*: RDF bin widths */
     double dv_e, dv_i;       /* NS%dv_*: VDF bin widths (main.f90:286-287) */
@@ -187,6 +188,10 @@ int tcp_batch_md_steps(tcp_batch* b, int nsteps, int thermo, double dt, const do
  * 16x16 by 4, 8x8 by 2, a wave taking tiles/waves i tiles at a time;
  * TCP_FP64: 0 only). */
 int tcp_batch_set_shape(tcp_batch* b, int shape);
+/* Block shape (an index as above) of replica rep's pair work: the batch's shape in
+ * TCP_MIXED; in TCP_FP64 the replica's own context's, 0 or, for small systems, 3
+ * (4x4 tiles by 4 waves), so that it keeps its standalone summation order. */
+int tcp_batch_replica_shape(tcp_batch* b, int rep, int* shape);
 /* Times `reps` pair launches per shape on the current positions, keeps the
  * fastest; *best = its index, ms[6] (may be NULL) = time per launch and shape. */
 int tcp_batch_autotune(tcp_batch* b, int reps, int* best, double* ms);

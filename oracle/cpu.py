@@ -96,6 +96,21 @@ def force_allpair(p: TcpParams, x, nthreads=0):
     return f, ur.value, rdf.reshape(3, NBIN)
 
 
+def force_rows(p: TcpParams, x, rows, nthreads=0):
+    """Real-space force and summed pair energy (no eps) of particles `rows` against
+    all others: (f[len(rows), 3], u[len(rows)]); the rows of force_allpair."""
+    a, rc2, _ = ewald_self(p)
+    x = _c(x)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    f = np.zeros((rows.size, 3))
+    u = np.zeros(rows.size)
+    _check(lib().tcpo_force_rows(ctypes.byref(CParams.from_params(p)), ctypes.c_double(a),
+                                 ctypes.c_double(rc2), _p(x), int(rows.size),
+                                 rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(f), _p(u),
+                                 int(nthreads)), "force_rows")
+    return f, u
+
+
 def spme(p: TcpParams, x):
     a = ewald_self(p)[0]
     x = _c(x)

@@ -18,7 +18,8 @@
  *   tcpo_ewald_kspace); FFTE is absent so Force_SPME4 cannot run here.
  *   tcpo_ewald_kspace: pinned by the Madelung constant of rock salt and by
  *   finite differences of its own energy.
- *   tcpo_force_allpair: pinned against tcpo_force_direct_self.
+ *   tcpo_force_allpair, tcpo_force_rows: pinned against tcpo_force_direct_self
+ *   (same rows, same pair maths, a different summation order).
  */
 #include "tcp_oracle.h"
 
@@ -224,10 +225,55 @@ static inline void nsum(double* s, double* c, double v)
     *s = t;
 }
 
+/* Row i of the all-pair sum: f3 = sum_j F_ij, *u = sum_j U_ij over every j != i
+ * inside the cut-off (compensated sums in j order), +1 per ordered pair into h.
+ * The pair maths of pair_term, with the energy terms kept apart per pair. */
+static void pair_row(const tcpo_params* p, double alpha, double rc2, const double* x, int i,
+                     double* f3, double* u, double* h)
+{
+    const int ne = p->ne, n = p->ne + p->ni;
+    int si = i >= ne;
+    double fs[3] = {0, 0, 0}, fc[3] = {0, 0, 0}, us = 0.0, uc = 0.0, xr[3];
+    for (int j = 0; j < n; ++j) {
+        if (j == i) continue;
+        double r2 = min_image(p->box, x + 3 * (size_t)i, x + 3 * (size_t)j, xr);
+        if (!(r2 < rc2)) continue;
+        int sj = j >= ne;
+        int hsel = si + sj; /* 0 ee, 1 ei, 2 ii */
+        double dx = hsel == 0 ? p->dx_ee : (hsel == 1 ? p->dx_ei : p->dx_ii);
+        double U = 0.0;
+        /* +1 per ordered pair == +2 per unordered pair */
+        double r = sqrt(r2);
+        int idx = (int)(r / dx) + 1;
+        if (h && idx <= NBIN) h[hsel * NBIN + idx - 1] += 1.0;
+        double xx = 1.0 - erf(alpha * r);
+        double g = 2.0 * exp(-(alpha * alpha * r2)) * alpha * r / SQRTPI;
+        double q1 = p->q[si], q2 = p->q[sj], df;
+        const int reg = si != sj || (si == 0 && sj == 0 && (p->flags & TCPO_REG_EE));
+        if (!reg) {
+            U = q1 * q2 * xx / r;
+            df = p->eps * q1 * q2 * (g + xx) / r2 / r;
+        } else if (p->flags & TCPO_KELBG) {
+            double s = r / p->rs;
+            double y2 = exp(-(s * s));
+            U = q1 * q2 * (xx - y2 + SQRTPI * s * erfc(s)) / r;
+            df = p->eps * q1 * q2 * (g + xx - y2) / r2 / r;
+        } else {
+            double y = exp(-r / p->rs);
+            U = q1 * q2 * (xx - y) / r;
+            df = p->eps * q1 * q2 * ((g + xx - y) / r2 / r - y / r2 / p->rs);
+        }
+        nsum(&us, &uc, U);
+        for (int k = 0; k < 3; ++k) nsum(&fs[k], &fc[k], df * xr[k]);
+    }
+    for (int k = 0; k < 3; ++k) f3[k] = fs[k] + fc[k];
+    *u = us + uc;
+}
+
 int tcpo_force_allpair(const tcpo_params* p, double alpha, double rc2,
                        const double* x, double* f, double* Ur, double* rdf, int nthreads)
 {
-    const int ne = p->ne, n = p->ne + p->ni;
+    const int n = p->ne + p->ni;
     double* usum = (double*)calloc((size_t)n, sizeof(double));
     if (!usum) return -2;
 #ifdef _OPENMP
@@ -246,44 +292,7 @@ int tcpo_force_allpair(const tcpo_params* p, double alpha, double rc2,
 #endif
         double* h = hist + (size_t)tid * 3 * NBIN;
 #pragma omp for schedule(dynamic, 16)
-        for (int i = 0; i < n; ++i) {
-            int si = i >= ne;
-            double fs[3] = {0, 0, 0}, fc[3] = {0, 0, 0}, us = 0.0, uc = 0.0, xr[3];
-            for (int j = 0; j < n; ++j) {
-                if (j == i) continue;
-                double r2 = min_image(p->box, x + 3 * (size_t)i, x + 3 * (size_t)j, xr);
-                if (!(r2 < rc2)) continue;
-                int sj = j >= ne;
-                int hsel = si + sj; /* 0 ee, 1 ei, 2 ii */
-                double dx = hsel == 0 ? p->dx_ee : (hsel == 1 ? p->dx_ei : p->dx_ii);
-                double U = 0.0;
-                /* +1 per ordered pair == +2 per unordered pair */
-                double r = sqrt(r2);
-                int idx = (int)(r / dx) + 1;
-                if (idx <= NBIN) h[hsel * NBIN + idx - 1] += 1.0;
-                double xx = 1.0 - erf(alpha * r);
-                double g = 2.0 * exp(-(alpha * alpha * r2)) * alpha * r / SQRTPI;
-                double q1 = p->q[si], q2 = p->q[sj], df;
-                const int reg = si != sj || (si == 0 && sj == 0 && (p->flags & TCPO_REG_EE));
-                if (!reg) {
-                    U = q1 * q2 * xx / r;
-                    df = p->eps * q1 * q2 * (g + xx) / r2 / r;
-                } else if (p->flags & TCPO_KELBG) {
-                    double s = r / p->rs;
-                    double y2 = exp(-(s * s));
-                    U = q1 * q2 * (xx - y2 + SQRTPI * s * erfc(s)) / r;
-                    df = p->eps * q1 * q2 * (g + xx - y2) / r2 / r;
-                } else {
-                    double y = exp(-r / p->rs);
-                    U = q1 * q2 * (xx - y) / r;
-                    df = p->eps * q1 * q2 * ((g + xx - y) / r2 / r - y / r2 / p->rs);
-                }
-                nsum(&us, &uc, U);
-                for (int k = 0; k < 3; ++k) nsum(&fs[k], &fc[k], df * xr[k]);
-            }
-            for (int k = 0; k < 3; ++k) f[3 * (size_t)i + k] = fs[k] + fc[k];
-            usum[i] = us + uc;
-        }
+        for (int i = 0; i < n; ++i) pair_row(p, alpha, rc2, x, i, f + 3 * (size_t)i, usum + i, h);
     }
     double s = 0.0, c = 0.0;
     for (int i = 0; i < n; ++i) nsum(&s, &c, usum[i]);
@@ -293,6 +302,22 @@ int tcpo_force_allpair(const tcpo_params* p, double alpha, double rc2,
             for (int b = 0; b < 3 * NBIN; ++b) rdf[b] += hist[(size_t)t * 3 * NBIN + b];
     free(hist);
     free(usum);
+    return 0;
+}
+
+int tcpo_force_rows(const tcpo_params* p, double alpha, double rc2, const double* x, int nrows,
+                    const int* rows, double* f, double* u, int nthreads)
+{
+    const int n = p->ne + p->ni;
+    for (int k = 0; k < nrows; ++k)
+        if (rows[k] < 0 || rows[k] >= n) return -1;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+    for (int k = 0; k < nrows; ++k) pair_row(p, alpha, rc2, x, rows[k], f + 3 * (size_t)k, u + k, NULL);
     return 0;
 }
 

@@ -65,6 +65,12 @@ int tcpo_force_direct_ext(const tcpo_params* p, double alpha, double rc2,
 int tcpo_force_allpair(const tcpo_params* p, double alpha, double rc2,
                        const double* x, double* f, double* Ur, double* rdf, int nthreads);
 
+/* Rows of tcpo_force_allpair: for particle rows[k] (k < nrows) against every other
+ * particle, f[3k..3k+2] = its real-space force and u[k] = the sum of its pair
+ * energies (no eps; twice its share of Ur).  For checking a sample of a large system. */
+int tcpo_force_rows(const tcpo_params* p, double alpha, double rc2, const double* x, int nrows,
+                    const int* rows, double* f, double* u, int nthreads);
+
 /* Force_SPME4 + SPME_SUM (force.f90:652-969) generalised to even order p:
  * writes the reciprocal forces into frec (overwrites) and Um (eps included). */
 int tcpo_spme(const tcpo_params* p, double alpha, const double* x, double* frec, double* Um);

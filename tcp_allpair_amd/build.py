@@ -30,8 +30,6 @@ FLAGS = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
 SOURCES = ["kernels_pair.hip", "kernels_spme.hip", "kernels_md.hip", "tcp_gpu.hip", "batch.hip"]
-PROBE = os.path.join(HERE, "copy_probe")   # HBM-counter calibration program
-VALU_PROBE = os.path.join(HERE, "valu_probe")  # VALU issue-rate calibration program
 HOST = os.path.join(HERE, "host")
 IO_LIB = os.path.join(HERE, "libtcp_io.so")
 RUN = os.path.join(HERE, "tcp_run")
@@ -58,17 +56,12 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
-    for src, exe in (("copy_probe.hip", PROBE), ("valu_probe.hip", VALU_PROBE)):
-        r = subprocess.run([HIPCC, *FLAGS, os.path.join(CSRC, src), "-o", exe], capture_output=True,
-                           text=True)
-        if r.returncode != 0:
-            raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
     _run([GXX, *HOST_FLAGS, "-fPIC", "-shared", os.path.join(HOST, "tcp_io.cpp"), "-o", IO_LIB],
          "libtcp_io.so")
     _run([GXX, *HOST_FLAGS, os.path.join(HOST, "tcp_run.cpp"), os.path.join(HOST, "tcp_io.cpp"),
           "-L", HERE, "-ltcp_gpu", "-Wl,-rpath,$ORIGIN", "-o", RUN], "tcp_run")
     if verbose:
-        print("built", LIB, PROBE, VALU_PROBE, IO_LIB, "and", RUN)
+        print("built", LIB, IO_LIB, "and", RUN)
     return LIB
 
 

@@ -411,6 +411,14 @@ int tcp_batch_set_shape(tcp_batch* b, int shape)
     return set_shape(b, shape);
 }
 
+int tcp_batch_replica_shape(tcp_batch* b, int rep, int* shape)
+{
+    if (!b || !shape) return TCP_ERR_ARG;
+    BTRY(check_rep(b, rep));
+    *shape = rep_shape(b, b->reps[rep], b->shape);
+    return 0;
+}
+
 int tcp_batch_autotune(tcp_batch* b, int reps, int* best, double* ms)
 {
     if (!b || reps < 1) return TCP_ERR_ARG;

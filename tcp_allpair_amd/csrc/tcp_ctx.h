@@ -18,7 +18,9 @@ constexpr int kStageChunks = 16;  // pipelined pieces of one host transfer
 // copies between the caller's arrays and the pinned staging buffer).  One pool
 // serves the process, created on first use and never torn down, so no exit
 // path waits on its threads; copy() calls from several host threads take
-// turns.  The workers only copy memory: no HIP or MPI call runs on them.
+// turns.  The workers only copy memory: no HIP or MPI call runs on them.  In a
+// child process made by fork() (which has no copy of the workers) copy() is a
+// plain memcpy on the calling thread.
 class CopyPool {
 public:
     static CopyPool& shared();
@@ -28,6 +30,7 @@ private:
     explicit CopyPool(int workers);
     void work(int id);
     const int parts_;
+    const int pid_;  // process that started the workers
     std::vector<std::thread> th_;
     std::mutex job_;  // one copy() at a time
     std::mutex mu_;

@@ -12,13 +12,15 @@
 #include <string>
 #include <vector>
 
+#include <unistd.h>
+
 #include "tcp_ctx.h"
 
 using namespace tcp;
 
 namespace tcp {
 
-CopyPool::CopyPool(int workers) : parts_(workers + 1)
+CopyPool::CopyPool(int workers) : parts_(workers + 1), pid_(getpid())
 {
     for (int w = 1; w <= workers; ++w) th_.emplace_back(&CopyPool::work, this, w);
 }
@@ -54,6 +56,12 @@ void CopyPool::work(int id)
 
 void CopyPool::copy(void* dst, const void* src, size_t bytes)
 {
+    // a forked child inherits the pool object but not its threads (and maybe a
+    // held lock): it copies on the calling thread alone
+    if (getpid() != pid_) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
     std::lock_guard<std::mutex> job(job_);
     {
         std::lock_guard<std::mutex> l(mu_);
@@ -563,6 +571,16 @@ int validate_params(const tcp_params* p, int precision, std::string* msg)
         return vfail(msg, TCP_ERR_ARG, "rc, rs and the RDF bin widths must be positive");
     const double rc_cut = std::fmin(std::fmin(p->box[0], p->box[1]), p->box[2]) / 2.0;
     if (p->rc > rc_cut) return vfail(msg, TCP_ERR_CUTOFF, " *** error at cut-off radius *** ");
+    if (std::isnan(p->alpha) || std::isinf(p->alpha))
+        return vfail(msg, TCP_ERR_ARG, "alpha must be finite (<= 0 selects 3.1/rc)");
+    // the mixed pair kernel's erfc(x) = exp(-x^2) P7(w) is fitted on x = alpha r in
+    // [0, kMixedErfcMax]; every pair it evaluates has r < rc
+    if (precision == TCP_MIXED) {
+        const double a = p->alpha > 0 ? p->alpha : 3.1 / p->rc;
+        if (a * p->rc > kMixedErfcMax)
+            return vfail(msg, TCP_ERR_UNSUPPORTED,
+                         "mixed precision needs alpha*rc <= 4.5, the range of its erfc fit");
+    }
     if (p->recip == TCP_RECIP_SPME) {
         for (int d = 0; d < 3; ++d)
             if (!pow2(p->K[d]) || p->K[d] > 1024 || p->K[d] < p->order)

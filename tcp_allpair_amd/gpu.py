@@ -50,6 +50,7 @@ SIGNATURES = {
     "tcp_batch_md_steps": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp,
                            ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _dp],
     "tcp_batch_set_shape": [_vp, ctypes.c_int],
+    "tcp_batch_replica_shape": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
     "tcp_batch_autotune": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp],
     "tcp_batch_last_timings": [_vp, _dp, _dp],
 }
@@ -320,6 +321,13 @@ class Batch:
 
     def set_shape(self, shape):
         self._ck(self.lib.tcp_batch_set_shape(self.h, int(shape)), "tcp_batch_set_shape")
+
+    def replica_shape(self, rep):
+        """Block shape of replica rep's pair work (tcp_batch_replica_shape)."""
+        sh = ctypes.c_int()
+        self._ck(self.lib.tcp_batch_replica_shape(self.h, int(rep), ctypes.byref(sh)),
+                 "tcp_batch_replica_shape")
+        return sh.value
 
     def autotune(self, reps=3):
         best = ctypes.c_int()

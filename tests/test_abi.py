@@ -9,7 +9,7 @@ import pytest
 
 from tcp_allpair_amd import gpu
 from tcp_allpair_amd.inputs import make_plasma
-from tcp_allpair_amd.params import CParams, FP64
+from tcp_allpair_amd.params import CParams, FP64, MIXED
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "tcp_gpu.h")
@@ -82,6 +82,22 @@ def test_cutoff_error_before_device_use():
 def test_parameter_errors(change, code):
     p, _, _ = make_plasma(10, 10)
     rc, msg, h = _create(p.replace(**change))
+    assert rc == code and msg and not h.value
+
+
+@pytest.mark.parametrize("arc,precision,code", [
+    (4.5, MIXED, 0), (4.6, MIXED, -4), (6.0, FP64, 0), (float("inf"), FP64, -1),
+    (float("nan"), MIXED, -1)])
+def test_free_alpha_range(arc, precision, code):
+    """alpha > 0 overrides 3.1/rc (force.f90:23); a mixed context's alpha*rc must stay in
+    [0, 4.5], the range of its FP32 erfc fit (FP64 has no such bound)."""
+    p, _, _ = make_plasma(10, 10)
+    rc, msg, h = _create(p.replace(alpha=arc / p.rc), precision)
+    if code == 0:  # past validation: created, or stopped at the first device call
+        assert rc in (0, -3), (rc, msg)
+        if h.value:
+            gpu.load().tcp_destroy(h)
+        return
     assert rc == code and msg and not h.value
 
 

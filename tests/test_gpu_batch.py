@@ -106,6 +106,10 @@ def test_fp64_batch_two_shapes_bitwise():
     every replica matches its standalone run bit for bit."""
     say("case batch fp64 two shapes")
     reps = replicas(RECIP_NONE)[:3] + [make_plasma(20000, 20000, seed=121, recip=RECIP_NONE)]
+    with Batch([p for p, _, _ in reps], FP64) as b:
+        shapes = [b.replica_shape(r) for r in range(len(reps))]
+    say("replica shapes", shapes)
+    assert set(shapes) == {0, 3}  # both persistent launches take part
     assert_bitwise(reps, FP64, 2, NOTEMP, reduced_dt(0.005))
 
 
@@ -132,6 +136,56 @@ def test_mixed_batch_shapes_vs_fp64(shape):
         f, f64 = bst[r][2], fst[r][2]
         assert np.sqrt(((f - f64) ** 2).sum() / (f64 ** 2).sum()) <= 1e-6
         assert abs(bh[r][0].sum() - fh[r][0].sum()) <= 1e-4 * fh[r][0].sum()
+
+
+def test_batch_workload_vs_oracle():
+    """The batching workload itself (tcp_allpair_amd.batchbench.batch_workload): 64
+    replicas, N_e log-uniform in 256..8192 with as many ions, densities U[0.5, 2], T*
+    U[0.5, 4], lambda = 0.5, rc = L/2, direct k-sum kmax = 6.  One step with dt = 0
+    leaves the positions where they are and the forces of main.f90:100-141 (real space
+    + k-sum, mean force removed).  FP64: six replicas spanning the sizes against the
+    oracle (force_allpair + ewald_kspace + remove_rigidmtn): per-particle force
+    <= 1e-10, Ur and Um <= 1e-12, RDF identical.  Mixed (autotuned shape): every
+    replica's forces within 1e-6 RMS of the FP64 batch's."""
+    from oracle import cpu
+    from tcp_allpair_amd.batchbench import batch_workload
+    say("case batch workload 64 replicas vs oracle")
+    reps = batch_workload()
+    n = np.array([p.n for p, _, _ in reps])
+    order = np.argsort(n)
+    picks = [int(order[k]) for k in np.linspace(0, len(reps) - 1, 6).round().astype(int)]
+    res = {}
+    for prec in (FP64, MIXED):
+        with Batch([p for p, _, _ in reps], prec) as b:
+            for r, (p, x, v) in enumerate(reps):
+                b.set_state(r, x=x, v=np.zeros_like(v), f=np.zeros_like(x))
+            if prec == MIXED:
+                best, _ = b.autotune(1)
+                say("mixed shape", best)
+            tr = b.md_steps(1, NOTEMP, 0.0)
+            st = [b.get_state(r) for r in range(len(reps))]
+            h = [b.histograms(r)[0] for r in range(len(reps))]
+        res[prec] = (tr, st, h)
+    tr, st, h = res[FP64]
+    for r in picks:
+        p, x, _ = reps[r]
+        say("replica", r, "n", p.n, "L", p.box[0])
+        np.testing.assert_array_equal(st[r][0], x)
+        fo, uro, rdfo = cpu.force_allpair(p, x)
+        fko, umo = cpu.ewald_kspace(p, x)
+        fo = cpu.remove_rigidmtn(p, fo + fko)
+        f = st[r][2]
+        nref = np.linalg.norm(fo, axis=1)
+        err = np.linalg.norm(f - fo, axis=1) / np.maximum(nref, 1e-3 * np.sqrt(np.mean(nref ** 2)))
+        assert err.max() <= 1e-10, (r, err.max())
+        assert abs(tr[0, r, 0] - uro) <= 1e-12 * abs(uro), r
+        assert abs(tr[0, r, 2] - umo) <= 1e-12 * abs(umo), r
+        np.testing.assert_array_equal(h[r], rdfo)
+    _, stm, _ = res[MIXED]
+    for r in range(len(reps)):
+        f, f64 = stm[r][2], st[r][2]
+        e = np.sqrt(((f - f64) ** 2).sum() / (f64 ** 2).sum())
+        assert e <= 1e-6, (r, e)
 
 
 def test_autotune_and_graph_reuse():

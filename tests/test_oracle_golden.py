@@ -53,6 +53,20 @@ def test_force_allpair_matches_reference(name):
     np.testing.assert_array_equal(rdf, d["out_rdf"])
 
 
+def test_force_rows_matches_reference():
+    """A sample of rows (the check used on systems too large for the full sum) and all
+    rows: forces per particle and 0.5 * sum of the row energies = Ur."""
+    p, d = load("force_c0")
+    rows = np.random.default_rng(3).choice(p.n, 300, replace=False)
+    f, u = cpu.force_rows(p, d["in_x"], rows, nthreads=4)
+    assert rel_force_err(f, d["out_f"][rows]).max() < 1e-11
+    fa, ua = cpu.force_rows(p, d["in_x"], np.arange(p.n), nthreads=4)
+    assert rel_force_err(fa, d["out_f"]).max() < 1e-11
+    assert abs(0.5 * ua.sum() - d["out_Ur"]) <= 1e-12 * abs(d["out_Ur"])
+    with pytest.raises(RuntimeError):
+        cpu.force_rows(p, d["in_x"], [p.n])
+
+
 def test_psi():
     p, d = load("psi")
     np.testing.assert_array_equal(cpu.psi(p).ravel(), d["out_psi"].ravel())
