@@ -162,6 +162,31 @@ int tcp_energy_adjoint(tcp_ctx* ctx, const double* q, double* U, double* dUdx, d
  * last tcp_md_steps call (HIP events on the context's stream). */
 int tcp_last_timings(tcp_ctx* ctx, double* pair_ms, double* steps_ms);
 
+/* ------------------------------------------------------ no allocation ---
+ * Force_direct_self (force.f90:52-180) as tcp_force_direct computes it (same
+ * kernels, block shape and summation order), with NO device allocation: no
+ * context, and every buffer beyond the caller's inputs and outputs in one
+ * static device array of the library (about 8.6 MB per GPU, held from the
+ * library's first use on that GPU to the end of the process).  No counterpart
+ * in the reference: this entry point is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops).
+ *   p, precision: as tcp_create; at most TCP_NOALLOC_MAX_N particles
+ *   x, y, z [N]:  device FP64 positions, electrons first (inputs)
+ *   fx, fy, fz [N]: device FP64 real-space forces (outputs, overwritten)
+ *   Ur:   device double, sys%Ur (no eps) (output)
+ *   rdf:  device unsigned long long[3*512] or NULL: RDF counts (+2 per pair
+ *         within rc) added to bins 0..500 of each 512-entry row (entries
+ *         501..511 receive the counts past Ndist+1 that the reference drops)
+ *   stream: a hipStream_t (NULL: the null stream)
+ * Asynchronous: the work is queued on stream and the call returns without
+ * waiting.  All calls on one GPU share that GPU's array, so calls that overlap
+ * on one GPU (two streams, or two host threads) race; calls queued on one
+ * stream are ordered.  Errors: tcp_last_error(NULL). */
+#define TCP_NOALLOC_MAX_N 8192
+int tcp_force_direct_noalloc(const tcp_params* p, int precision, const double* x, const double* y,
+                             const double* z, double* fx, double* fy, double* fz, double* Ur,
+                             unsigned long long* rdf, void* stream);
+
 /* ---------------------------------------------------------------- batches ---
  * Many independent plasmas of different sizes stepped together, for ensembles
  * and parameter scans.  No counterpart in the reference, which runs one system

@@ -29,7 +29,8 @@ FLAGS = [
     "-Wall", "-Wno-unused-function",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
-SOURCES = ["kernels_pair.hip", "kernels_spme.hip", "kernels_md.hip", "tcp_gpu.hip", "batch.hip"]
+SOURCES = ["kernels_pair.hip", "kernels_spme.hip", "kernels_md.hip", "tcp_gpu.hip", "batch.hip",
+           "noalloc.hip"]
 HOST = os.path.join(HERE, "host")
 IO_LIB = os.path.join(HERE, "libtcp_io.so")
 RUN = os.path.join(HERE, "tcp_run")

@@ -119,5 +119,7 @@ PairArgs pair_args(tcp_ctx* c);
 void choose_chunks(int n_i, int n_j, int* nchunk, int* jchunk);
 // Parameter checks of tcp_create (no device use): 0 or a TCP_ERR_* code and *msg.
 int validate_params(const tcp_params* p, int precision, std::string* msg);
+// Sets the text tcp_last_error(NULL) returns on this thread; returns code.
+int set_create_error(int code, const char* msg);
 
 }  // namespace tcp

@@ -546,6 +546,12 @@ int setup_recip(tcp_ctx* c)
 
 namespace tcp {
 
+int set_create_error(int code, const char* msg)
+{
+    g_create_error = msg;
+    return code;
+}
+
 int validate_params(const tcp_params* p, int precision, std::string* msg)
 {
     auto vfail = [](std::string* m, int code, const char* text) {

@@ -41,6 +41,8 @@ SIGNATURES = {
     "tcp_vverlet2_scale": [_vp, ctypes.c_double, ctypes.c_double, _dp, _dp],
     "tcp_set_lambda": [_vp, ctypes.c_double],
     "tcp_energy_adjoint": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
+    "tcp_force_direct_noalloc": [ctypes.POINTER(CParams), ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                 _vp, _vp, _vp],
     "tcp_batch_create": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(CParams), ctypes.c_int],
     "tcp_batch_destroy": [_vp],
     "tcp_batch_last_error": [_vp],
