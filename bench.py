@@ -24,8 +24,12 @@ a bounded row slice of the same workload.  batch: the 64-replica batching
 workload of tcp_allpair_amd/batchbench.py (one persistent pair launch and a
 captured step for all replicas) against the same replicas stepped one
 context after another.  dropin_host_buffers: the step driven as the Fortran
-drop-in drives it, each call copying the host arrays it reads and writes (the
-PCIe-inclusive rate; never value).
+drop-in drives it, the host arrays coming back every step (the PCIe-inclusive
+rate; never value).  nve_drift: the slope of the total energy per step over
+1000 NVE steps of the mixed path divided by |E_tot(0)| (nve_energy_slope_timed_steps
+is the same slope over the timed steps only, a fluctuation measure).
+roofline_fp64: the FP64 pair kernel in the same flop count against the
+78.6 TFLOP/s FP64 vector peak.
 """
 from __future__ import annotations
 
@@ -83,18 +87,21 @@ def run(p, x, v, f0, precision, steps, warmup, dt):
 
 def dropin_run(p, x, v, f0, precision, steps, dt):
     """The step as the Fortran drop-in drives it with one direct rank
-    (tcp_allpair_amd/fortran/tcp_force_gpu.f90): every call uploads the host
-    arrays it reads and downloads what it writes, the host state (AoS x, v, f)
-    staying authoritative.  Per step: vverlet1 (x, v, f in; x, v out),
-    force_direct (x in; f out), force_recip (x, f in; f out), remove_rigidmtn
-    (f in and out), vverlet2 (v, f in; v out): 9 uploads and 6 downloads of
-    3N doubles.  The shim's own copies between the PT records and these arrays
-    are not included."""
+    (tcp_allpair_amd/fortran/tcp_force_gpu.f90), the host state (AoS x, v, f)
+    and the histograms coming back every step: an array is uploaded only when
+    the host copy changed (here it never does between steps) and the step's
+    results stay on the device from VVerlet*1 to VVerlet*2.  Per step:
+    vverlet1, force_direct, force_recip, remove_rigidmtn, vverlet2, then one
+    download of 9N doubles (x, v, f) and one of the RDF and VDF counts.  The
+    shim's own loops between the PT records and these arrays are not
+    included."""
     import ctypes
     with Context(p, precision) as c:
         lib, h = c.lib, c.h
         hx, hv, hf = (np.ascontiguousarray(a, dtype=np.float64).copy() for a in (x, v, f0))
         px, pv, pf = (a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in (hx, hv, hf))
+        rdf, vdf = np.zeros(3 * 501), np.zeros(2 * 501)
+        prdf, pvdf = (a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in (rdf, vdf))
         ur, um, a, b = (ctypes.c_double() for _ in range(4))
 
         def ck(rc, what):
@@ -102,23 +109,16 @@ def dropin_run(p, x, v, f0, precision, steps, dt):
                 raise RuntimeError(f"{what}: {lib.tcp_last_error(h).decode()}")
 
         def step():
-            ck(lib.tcp_set_state(h, px, pv, pf), "set_state")
             ck(lib.tcp_vverlet1(h, NOTEMP, dt, 0.0), "vverlet1")
-            ck(lib.tcp_get_state(h, px, pv, None), "get_state")
-            ck(lib.tcp_set_state(h, px, None, None), "set_state")
             ck(lib.tcp_force_direct(h, ctypes.byref(ur)), "force_direct")
-            ck(lib.tcp_get_state(h, None, None, pf), "get_state")
-            ck(lib.tcp_set_state(h, px, None, pf), "set_state")
             ck(lib.tcp_force_recip(h, ctypes.byref(um)), "force_recip")
-            ck(lib.tcp_get_state(h, None, None, pf), "get_state")
-            ck(lib.tcp_set_state(h, None, None, pf), "set_state")
             ck(lib.tcp_remove_rigidmtn(h), "remove_rigidmtn")
-            ck(lib.tcp_get_state(h, None, None, pf), "get_state")
-            ck(lib.tcp_set_state(h, None, pv, pf), "set_state")
             ck(lib.tcp_vverlet2(h, NOTEMP, dt, p.Te, p.Ti, p.tau, 0.0, 1, ctypes.byref(a),
                                 ctypes.byref(b)), "vverlet2")
-            ck(lib.tcp_get_state(h, None, pv, None), "get_state")
+            ck(lib.tcp_get_state(h, px, pv, pf), "get_state")
+            ck(lib.tcp_get_histograms(h, prdf, pvdf, 1), "get_histograms")
 
+        ck(lib.tcp_set_state(h, px, pv, pf), "set_state")
         step()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -127,14 +127,16 @@ def dropin_run(p, x, v, f0, precision, steps, dt):
 
 
 def cpu_baseline(p, x):
-    """The reference's Force_direct_ext on host cores: each worker times its own
-    512-particle row slice against all N particles (about 10 s per worker)."""
+    """The reference's Force_direct_ext on host cores: each worker process times
+    one call on its own 512-particle row slice against all N particles (after one
+    warm call; a few seconds each).  At most 16 processes: the share of host
+    cores that goes with one GPU of the box."""
     from oracle import refharness as rh
-    cores = max(1, min(16, os.cpu_count() or 1))
+    host_cores = os.cpu_count() or 1
+    cores = max(1, min(16, host_cores))
     n_own = 512
     if not rh.available():
         return None
-    import concurrent.futures as cf
     import tempfile
     procs = []
     with tempfile.TemporaryDirectory() as td:
@@ -152,16 +154,44 @@ def cpu_baseline(p, x):
                             nown=(n_own // 2, n_own // 2))
             procs.append((subprocess.Popen([rh.HARNESS, fin, fout], stdout=subprocess.DEVNULL,
                                            stderr=subprocess.DEVNULL), fout))
-        rates = []
+        rates, secs = [], []
         for pr, fout in procs:
             if pr.wait(timeout=300) != 0:
                 return None
-            pairs, secs, _ = np.fromfile(fout, dtype="<f8", count=3)
-            rates.append(pairs / secs)
+            pairs, sec, _ = np.fromfile(fout, dtype="<f8", count=3)
+            rates.append(pairs / sec)
+            secs.append(sec)
     return dict(value=float(sum(rates)), unit="pair-interactions/s", cores=cores, kind="reference",
+                host_cores=host_cores, per_process_rate=float(np.mean(rates)),
+                seconds_per_process=float(np.mean(secs)),
                 sample=f"Force_direct_ext (force.f90:183) of a {n_own}-particle row slice against "
-                       f"all {p.n} particles per core, {cores} processes in parallel, one timed "
-                       "call each after one warm call; rate summed over processes")
+                       f"all {p.n} particles per process ({n_own * p.n} ordered pairs), {cores} "
+                       "single-threaded processes in parallel (at most 16: the host-core share of "
+                       "one GPU), one timed call each after one warm call; rate summed over "
+                       "processes")
+
+
+def nve_drift(p, x, v, f0, steps, dt):
+    """Slope of the total energy per step over `steps` NVE steps of the mixed
+    path, divided by |E_tot(0)| (least squares over every step)."""
+    with Context(p, MIXED) as c:
+        c.set_state(x=x, v=v, f=f0)
+        tr = c.md_steps(steps, NOTEMP, dt, trace=True)
+    etot = 0.5 * (tr[:, 4] + tr[:, 5]) + tr[:, 3] + tr[:, 2]
+    slope = np.polyfit(np.arange(len(etot)), etot, 1)[0]
+    return float(slope / abs(etot[0])), float(np.ptp(etot) / abs(etot[0]))
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 --pmc summary (scaled
+    FETCH_SIZE + WRITE_SIZE), or None."""
+    if not os.path.exists(PROFILE_TRAFFIC):
+        return None
+    for line in open(PROFILE_TRAFFIC):
+        cols = [c.strip() for c in line.split("|")]
+        if len(cols) > 4 and cols[1] == kernel:
+            return float(cols[4])
+    return None
 
 
 def main():
@@ -175,6 +205,8 @@ def main():
                     help="steps of the 64-replica batching workload (0 skips it)")
     ap.add_argument("--dropin-steps", type=int, default=5,
                     help="steps driven as the Fortran drop-in drives them (0 skips it)")
+    ap.add_argument("--drift-steps", type=int, default=1000,
+                    help="NVE steps of the mixed path for the energy drift (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -200,18 +232,21 @@ def main():
     etot = 0.5 * (tr[:, 4] + tr[:, 5]) + tr[:, 3] + tr[:, 2]
     flops = N * (N - 1) / 2 * FLOP_PER_UNORDERED_PAIR_NEWTON
     achieved = flops / (mx["pair_ms"] * 1e-3) / 1e12
-    traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):   # measured by rocprofv3 --pmc, see that file
-        for line in open(PROFILE_TRAFFIC):
-            if line.startswith("hbm_bytes_per_launch:"):
-                traffic = float(line.split(":")[1])
+    achieved64 = flops / (fp["pair_ms"] * 1e-3) / 1e12
+    drift = None
+    if args.drift_steps > 1:
+        print(f"[bench] nve drift over {args.drift_steps} mixed steps", file=sys.stderr, flush=True)
+        d, excursion = nve_drift(p, x, v, f64_0, args.drift_steps, dt)
+        drift = {"steps": args.drift_steps, "drift_per_step": d, "max_excursion": excursion}
     dropin = None
     if args.dropin_steps > 0:
         print("[bench] host-buffer steps (the drop-in's call pattern)", file=sys.stderr, flush=True)
         dropin = {"steps": args.dropin_steps,
                   "mixed_ms_per_step": dropin_run(p, x, v, f64_0, MIXED, args.dropin_steps, dt),
                   "fp64_ms_per_step": dropin_run(p, x, v, f64_0, FP64, args.dropin_steps, dt),
-                  "transfers_per_step": "9 uploads + 6 downloads of 3N doubles (AoS host arrays)"}
+                  "transfers_per_step": "one download of 9N doubles (x, v, f) and one of the "
+                                        "histograms; no uploads while the host arrays are "
+                                        "unchanged"}
     batch = None
     if args.batch_steps > 0:
         print("[bench] batching workload (64 replicas)", file=sys.stderr, flush=True)
@@ -249,12 +284,20 @@ def main():
                  / FP64_VECTOR_PEAK_TFLOPS},
         "mixed_over_fp64_steps": sps / sps64,
         "force_err_rms_vs_fp64": err,
-        "nve_energy_drift_per_step": float(np.polyfit(np.arange(len(etot)), etot, 1)[0] / abs(etot[0]))
-        if len(etot) > 1 else None,
+        "nve_energy_drift_per_step": None if drift is None else drift["drift_per_step"],
+        "nve_drift": drift,
+        "nve_energy_slope_timed_steps": float(np.polyfit(np.arange(len(etot)), etot, 1)[0]
+                                              / abs(etot[0])) if len(etot) > 1 else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
-                     "traffic": traffic, "kernel": "pair_mixed_n3_kernel<true, false>",
+                     "traffic": profiled_traffic("pair_mixed_n3_kernel<true, false>"),
+                     "kernel": "pair_mixed_n3_kernel<true, false>",
                      "kernel_ms": mx["pair_ms"], "flop_per_launch": flops},
+        "roofline_fp64": {"bound": "valu", "achieved": achieved64, "peak": FP64_VECTOR_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": achieved64 / FP64_VECTOR_PEAK_TFLOPS,
+                          "traffic": profiled_traffic("pair_f64_n3_kernel<true, false>"),
+                          "kernel": "pair_f64_n3_kernel<true, false>",
+                          "kernel_ms": fp["pair_ms"], "flop_per_launch": flops},
         "cpu_baseline": cpu,
         "dropin_host_buffers": dropin,
         "batch": batch,

@@ -6,10 +6,14 @@
  * init.f90 call (file:line of the reference given per function), so an
  * ISO_C_BINDING shim with the same subroutine names can bind them.  Plain
  * pointers and sizes only.  All functions return 0 on success and a negative
- * code on error; tcp_last_error() returns the message.  Calls are synchronous
- * with respect to the host (results are in host memory on return).  A context
- * owns all device memory for its lifetime, is used by one host thread at a
- * time and runs on its own HIP stream.
+ * code on error; tcp_last_error() returns the message.  A call that takes or
+ * returns host data is synchronous (results are in host memory, host inputs
+ * may be reused, on return); tcp_vverlet1 and tcp_remove_rigidmtn, which only
+ * change device state, queue their kernels on the context's stream and
+ * return, and the next call that returns host data waits for them (a fault of
+ * theirs is reported there).  A context owns all device memory for its
+ * lifetime, is used by one host thread at a time and runs on its own HIP
+ * stream.
  *
  * Host arrays are AoS double[3*N] (x,y,z per particle) with the electrons first
  * and the ions after them, the order of qel(:) followed by qion(:)

@@ -736,10 +736,13 @@ int tcpo_vverlet1(const tcpo_params* p, int thermo, double dt, double fric,
     return 0;
 }
 
+/* id = INT(mv2/3/dv) + 1, counted when id <= Ndist+1 (vverlet.f90:201-204); the
+ * quotient is range-checked before the conversion, so a vanishing bin width
+ * (T -> 0) counts nothing instead of converting an out-of-range value */
 static void vdf_add(double* vdf, double mv2, double dv)
 {
-    int id = (int)(mv2 / 3.0 / dv) + 1;
-    if (id <= NBIN) vdf[id - 1] += 1.0;
+    double t = mv2 / 3.0 / dv;
+    if (t >= 0.0 && t < (double)NBIN) vdf[(int)t] += 1.0;
 }
 
 /* VVerletNotemp2 (:190-219), VVerletBerend2 (:245-294), VVerletIsokin2 (:319-368),

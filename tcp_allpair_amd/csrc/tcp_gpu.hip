@@ -891,7 +891,7 @@ int tcp_remove_rigidmtn(tcp_ctx* c)
     launch_reduce_parts(c->part, md_blocks(c->N), 3, c->sc + SC_FSUM, c->stream);
     launch_sub_mean(m, c->sc + SC_FSUM, c->stream);
     LAUNCHCK(c);
-    return sync(c);
+    return 0;  // device state only: the next call that returns host data waits
 }
 
 int tcp_vverlet1(tcp_ctx* c, int thermo, double dt, double fric)
@@ -899,7 +899,7 @@ int tcp_vverlet1(tcp_ctx* c, int thermo, double dt, double fric)
     if (!c || thermo < TCP_NOTEMP || thermo > TCP_STOCH) return TCP_ERR_ARG;
     launch_vv1(md_args(c), thermo, dt, fric, c->stream);
     LAUNCHCK(c);
-    return sync(c);
+    return 0;  // device state only: the next call that returns host data waits
 }
 
 int tcp_vverlet2(tcp_ctx* c, int thermo, double dt, double Te, double Ti, double tau, double fric,
@@ -956,16 +956,15 @@ int tcp_get_histograms(tcp_ctx* c, double* rdf, double* vdf, int reset)
     HIPCK(c, hipMemcpyAsync(h.data() + 3 * kHistStride, c->vdf,
                             2 * kHistStride * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                             c->stream));
+    if (reset) {  // queued behind the copies on the same stream: one wait for both
+        HIPCK(c, hipMemsetAsync(c->rdf, 0, 3 * kHistStride * sizeof(unsigned long long), c->stream));
+        HIPCK(c, hipMemsetAsync(c->vdf, 0, 2 * kHistStride * sizeof(unsigned long long), c->stream));
+    }
     TRY(sync(c));
     for (int t = 0; t < 3 && rdf; ++t)
         for (int b = 0; b < kNbin; ++b) rdf[t * kNbin + b] += (double)h[t * kHistStride + b];
     for (int t = 0; t < 2 && vdf; ++t)
         for (int b = 0; b < kNbin; ++b) vdf[t * kNbin + b] += (double)h[(3 + t) * kHistStride + b];
-    if (reset) {
-        HIPCK(c, hipMemsetAsync(c->rdf, 0, 3 * kHistStride * sizeof(unsigned long long), c->stream));
-        HIPCK(c, hipMemsetAsync(c->vdf, 0, 2 * kHistStride * sizeof(unsigned long long), c->stream));
-        TRY(sync(c));
-    }
     return 0;
 }
 

@@ -3,9 +3,20 @@
 ! through ISO_C_BINDING.  Link it with the unchanged data.f90, main.f90,
 ! init.f90 and result.f90.
 !
-! The host arrays qel/qion stay authoritative: each subroutine uploads what it
-! reads and downloads what it writes, so Restart/DUMP/SAMPLING (result.f90) see
-! exactly the state the reference would hold.  With one direct rank
+! Transfers: the shim keeps a shadow of the device's x, v and f (hx, hv, hf).
+! A subroutine uploads an array it reads only when the host copy differs from
+! the shadow (a caller or another subroutine changed it), and downloads what it
+! writes.  With one direct rank the step's results stay on the device from
+! VVerlet*1 to VVerlet*2 (x and v of the kick-drift, the forces of
+! Force_direct_self, SPME_SUM and REMOVE_RIGIDMTN, the RDF counts), and
+! VVerlet*2 brings x, v, f and both histograms back in one call: the step of
+! main.f90:87-153 reads none of them on the host in between (gsend feeds only
+! the LOCAL gather of several direct ranks), and Restart/DUMP/SAMPLING
+! (result.f90) run after VVerlet*2, so they see exactly the state the
+! reference would hold.  Forces computed outside a step (Finit) stay on the
+! device for the next VVerlet*1; their RDF counts reach NS%RDF_* at once, as
+! the reference counts them before main.f90:63 clears them.
+! TCP_DROPIN_SYNC=1 downloads after every subroutine instead.  With one direct rank
 ! (`mpirun -np 2`) rank 0, the PME rank of data.f90:151, does nothing (the
 ! LOCAL gather/scatter are dropped on both ranks alike) and rank 1 owns the GPU
 ! context with every particle, reciprocal part included.  With several direct
@@ -26,7 +37,30 @@ IMPLICIT NONE
 INTEGER(C_INT64_T), SAVE :: stoch_seed = 20240607_C_INT64_T
 ! rank of this direct rank in COMM1D: separates the ranks' noise streams
 INTEGER(C_INT64_T), SAVE :: noise_rank = 0_C_INT64_T
+! the device's x, v, f are newer than qel/qion and the shadows (download deferred)
+LOGICAL, SAVE :: x_on_device = .FALSE., v_on_device = .FALSE., f_on_device = .FALSE.
+! between VVerlet*1 and VVerlet*2 of a step; RDF counts of the step not yet added
+LOGICAL, SAVE :: in_step = .FALSE., rdf_pending = .FALSE.
+! 0: forces may stay on the device between subroutines of one step (one direct
+! rank); 1: every subroutine downloads what it writes; -1: not yet read
+INTEGER, SAVE :: sync_every = -1
 CONTAINS
+
+! Results stay on the device between the subroutines of a step: one direct rank
+! and TCP_DROPIN_SYNC not set to 1.
+LOGICAL FUNCTION defer_forces(NS)
+   TYPE(NM) :: NS
+   CHARACTER(LEN=8) :: val
+   INTEGER :: ln, st
+   IF (sync_every < 0) THEN
+      sync_every = 0
+      CALL GET_ENVIRONMENT_VARIABLE('TCP_DROPIN_SYNC', val, ln, st)
+      IF (st == 0 .AND. ln >= 1) THEN
+         IF (val(1:1) == '1') sync_every = 1
+      END IF
+   END IF
+   defer_forces = sync_every == 0 .AND. direct_ranks(NS) == 1
+END FUNCTION defer_forces
 
 ! direct ranks of the (single) PME group
 INTEGER FUNCTION direct_ranks(NS)
@@ -69,30 +103,76 @@ SUBROUTINE ensure_ctx(NS, param, ne, ni, with_recip)
    tcp_n = ne + ni
    tcp_ne = ne
    ALLOCATE(hx(3, tcp_n), hv(3, tcp_n), hf(3, tcp_n))
+   ! a new context holds zeros: so does its shadow
+   hx = 0.0D0
+   hv = 0.0D0
+   hf = 0.0D0
 END SUBROUTINE ensure_ctx
 
+! Field k (1 = xx, 2 = xv, 3 = ff) of qel/qion into the shadow sh; true when
+! any value differed in any bit (the device copy is then out of date).
+LOGICAL FUNCTION to_shadow(NS, qel, qion, k, sh)
+   TYPE(NM) :: NS
+   TYPE(PT) :: qel(NS%Nel), qion(NS%Nion)
+   INTEGER, INTENT(IN) :: k
+   REAL(C_DOUBLE), INTENT(INOUT) :: sh(:,:)
+   INTEGER :: i
+   REAL(KIND=DP) :: t(3)
+   to_shadow = .FALSE.
+   DO i = 1, NS%Nel + NS%Nion
+      IF (i <= NS%Nel) THEN
+         SELECT CASE (k)
+         CASE (1); t = qel(i)%xx
+         CASE (2); t = qel(i)%xv
+         CASE DEFAULT; t = qel(i)%ff
+         END SELECT
+      ELSE
+         SELECT CASE (k)
+         CASE (1); t = qion(i - NS%Nel)%xx
+         CASE (2); t = qion(i - NS%Nel)%xv
+         CASE DEFAULT; t = qion(i - NS%Nel)%ff
+         END SELECT
+      END IF
+      ! bit patterns, so that -0 against +0 and NaNs count as changes too
+      IF (ANY(TRANSFER(t, 0_C_INT64_T, 3) /= TRANSFER(sh(:, i), 0_C_INT64_T, 3))) THEN
+         sh(:, i) = t
+         to_shadow = .TRUE.
+      END IF
+   END DO
+END FUNCTION to_shadow
+
+! Uploads the requested arrays whose host copy differs from the device's (an
+! array whose download is deferred is newer on the device and is not uploaded).
 SUBROUTINE put(NS, qel, qion, wx, wv, wf)
    TYPE(NM) :: NS
    TYPE(PT) :: qel(NS%Nel), qion(NS%Nion)
    LOGICAL, INTENT(IN) :: wx, wv, wf
-   INTEGER :: i
    TYPE(C_PTR) :: px, pv, pf
    px = C_NULL_PTR; pv = C_NULL_PTR; pf = C_NULL_PTR
-   DO i = 1, NS%Nel
-      IF (wx) hx(:, i) = qel(i)%xx
-      IF (wv) hv(:, i) = qel(i)%xv
-      IF (wf) hf(:, i) = qel(i)%ff
-   END DO
-   DO i = 1, NS%Nion
-      IF (wx) hx(:, NS%Nel + i) = qion(i)%xx
-      IF (wv) hv(:, NS%Nel + i) = qion(i)%xv
-      IF (wf) hf(:, NS%Nel + i) = qion(i)%ff
-   END DO
-   IF (wx) px = C_LOC(hx)
-   IF (wv) pv = C_LOC(hv)
-   IF (wf) pf = C_LOC(hf)
-   CALL tcp_check(tcp_set_state(tcp_ctx_h, px, pv, pf), 'tcp_set_state')
+   IF (wx .AND. .NOT. x_on_device) THEN
+      IF (to_shadow(NS, qel, qion, 1, hx)) px = C_LOC(hx)
+   END IF
+   IF (wv .AND. .NOT. v_on_device) THEN
+      IF (to_shadow(NS, qel, qion, 2, hv)) pv = C_LOC(hv)
+   END IF
+   IF (wf .AND. .NOT. f_on_device) THEN
+      IF (to_shadow(NS, qel, qion, 3, hf)) pf = C_LOC(hf)
+   END IF
+   IF (C_ASSOCIATED(px) .OR. C_ASSOCIATED(pv) .OR. C_ASSOCIATED(pf)) &
+        CALL tcp_check(tcp_set_state(tcp_ctx_h, px, pv, pf), 'tcp_set_state')
 END SUBROUTINE put
+
+! After a kernel changed f on the device: keep it there (one direct rank) or
+! bring it back now.
+SUBROUTINE forces_written(NS, qel, qion)
+   TYPE(NM) :: NS
+   TYPE(PT) :: qel(NS%Nel), qion(NS%Nion)
+   IF (defer_forces(NS)) THEN
+      f_on_device = .TRUE.
+   ELSE
+      CALL get(NS, qel, qion, .FALSE., .FALSE., .TRUE.)
+   END IF
+END SUBROUTINE forces_written
 
 SUBROUTINE get(NS, qel, qion, wx, wv, wf)
    TYPE(NM) :: NS
@@ -115,6 +195,9 @@ SUBROUTINE get(NS, qel, qion, wx, wv, wf)
       IF (wv) qion(i)%xv = hv(:, NS%Nel + i)
       IF (wf) qion(i)%ff = hf(:, NS%Nel + i)
    END DO
+   IF (wx) x_on_device = .FALSE.
+   IF (wv) v_on_device = .FALSE.
+   IF (wf) f_on_device = .FALSE.
 END SUBROUTINE get
 
 SUBROUTINE add_rdf(NS)
@@ -125,6 +208,19 @@ SUBROUTINE add_rdf(NS)
    NS%RDF_ei = NS%RDF_ei + hrdf(:, 2)
    NS%RDF_ii = NS%RDF_ii + hrdf(:, 3)
 END SUBROUTINE add_rdf
+
+SUBROUTINE add_histograms(NS)
+   TYPE(NM) :: NS
+   hrdf = 0.0D0
+   hvdf = 0.0D0
+   CALL tcp_check(tcp_get_histograms(tcp_ctx_h, C_LOC(hrdf), C_LOC(hvdf), 1_C_INT), &
+        'tcp_get_histograms')
+   NS%RDF_ee = NS%RDF_ee + hrdf(:, 1)
+   NS%RDF_ei = NS%RDF_ei + hrdf(:, 2)
+   NS%RDF_ii = NS%RDF_ii + hrdf(:, 3)
+   NS%VDF_e = NS%VDF_e + hvdf(:, 1)
+   NS%VDF_i = NS%VDF_i + hvdf(:, 2)
+END SUBROUTINE add_histograms
 
 SUBROUTINE add_vdf(NS)
    TYPE(NM) :: NS
@@ -142,14 +238,24 @@ SUBROUTINE vv2_common(NS, qel, qion, param, sys, dt, code)
    REAL(KIND=DP) :: dt
    INTEGER(C_INT), INTENT(IN) :: code
    REAL(C_DOUBLE) :: a, b
+   LOGICAL :: bx, bf
    CALL put(NS, qel, qion, .FALSE., .TRUE., .TRUE.)
    stoch_seed = stoch_seed + 1
    CALL tcp_check(tcp_vverlet2(tcp_ctx_h, code, dt, param%Te, param%Ti, param%tau, param%alpha, &
         stoch_seed + noise_rank*1099511627776_C_INT64_T, a, b), 'tcp_vverlet2')
    sys%mv2el = a
    sys%mv2ion = b
-   CALL get(NS, qel, qion, .FALSE., .TRUE., code == TCP_STOCH)
-   CALL add_vdf(NS)
+   ! the end of the step: the host holds x, v, f and the histograms again
+   bx = x_on_device
+   bf = code == TCP_STOCH .OR. f_on_device
+   CALL get(NS, qel, qion, bx, .TRUE., bf)
+   IF (rdf_pending) THEN
+      CALL add_histograms(NS)
+   ELSE
+      CALL add_vdf(NS)
+   END IF
+   rdf_pending = .FALSE.
+   in_step = .FALSE.
 END SUBROUTINE vv2_common
 
 ! VVerletBerend2 / VVerletIsokin2 with several direct ranks: kick, the
@@ -238,8 +344,12 @@ REAL(C_DOUBLE) :: ur
 CALL put(NS, qel, qion, .TRUE., .FALSE., .FALSE.)
 CALL tcp_check(tcp_force_direct(tcp_ctx_h, ur), 'tcp_force_direct')
 sys%Ur = ur
-CALL get(NS, qel, qion, .FALSE., .FALSE., .TRUE.)
-CALL add_rdf(NS)
+CALL forces_written(NS, qel, qion)
+IF (in_step .AND. defer_forces(NS)) THEN
+   rdf_pending = .TRUE.
+ELSE
+   CALL add_rdf(NS)
+END IF
 END SUBROUTINE Force_direct_self
 
 SUBROUTINE Force_direct_ext(NS, qel, qion, param, grec, gemit, Ur, Ne, Ni)
@@ -367,7 +477,7 @@ IF (direct_ranks(NS) == 1) THEN
    CALL put(NS, qel, qion, .TRUE., .FALSE., .TRUE.)
    CALL tcp_check(tcp_force_recip(tcp_ctx_h, um), 'tcp_force_recip')
    sys%Um = um
-   CALL get(NS, qel, qion, .FALSE., .FALSE., .TRUE.)
+   CALL forces_written(NS, qel, qion)
    RETURN
 END IF
 ! several direct ranks: the reference's broadcast and scatter (force.f90:950-965)
@@ -401,7 +511,7 @@ INTEGER :: i, ierr
 IF (COMM%NUMPROC == 1) THEN
    CALL put(NS, qel, qion, .FALSE., .FALSE., .TRUE.)
    CALL tcp_check(tcp_remove_rigidmtn(tcp_ctx_h), 'tcp_remove_rigidmtn')
-   CALL get(NS, qel, qion, .FALSE., .FALSE., .TRUE.)
+   CALL forces_written(NS, qel, qion)
 ELSE
    ! several direct ranks: the mean over all of them needs the COMM1D sum
    fl = 0.0D0
@@ -436,6 +546,13 @@ REAL(KIND=DP)::dt
 INTEGER :: i
 CALL put(NS, qel, qion, .TRUE., .TRUE., .TRUE.)
 CALL tcp_check(tcp_vverlet1(tcp_ctx_h, TCP_NOTEMP, dt, 0.0D0), 'tcp_vverlet1')
+IF (defer_forces(NS)) THEN
+   ! one direct rank: x and v come back with VVerlet*2; gsend has no reader
+   x_on_device = .TRUE.
+   v_on_device = .TRUE.
+   in_step = .TRUE.
+   RETURN
+END IF
 CALL get(NS, qel, qion, .TRUE., .TRUE., .FALSE.)
 DO i = 1, NS%Nel
    gsend(i)%xx = qel(i)%xx
@@ -457,6 +574,13 @@ REAL(KIND=DP)::dt
 INTEGER :: i
 CALL put(NS, qel, qion, .TRUE., .TRUE., .TRUE.)
 CALL tcp_check(tcp_vverlet1(tcp_ctx_h, TCP_STOCH, dt, param%alpha), 'tcp_vverlet1')
+IF (defer_forces(NS)) THEN
+   ! one direct rank: x and v come back with VVerlet*2; gsend has no reader
+   x_on_device = .TRUE.
+   v_on_device = .TRUE.
+   in_step = .TRUE.
+   RETURN
+END IF
 CALL get(NS, qel, qion, .TRUE., .TRUE., .FALSE.)
 DO i = 1, NS%Nel
    gsend(i)%xx = qel(i)%xx

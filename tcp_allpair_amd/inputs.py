@@ -4,7 +4,9 @@
 then ions, positions uniform in [-L/2, L/2)^3 with L = (N/density)^(1/3),
 velocities Maxwell-Boltzmann per species.  `write_tcp_prm` / `write_restart_xyz`
 emit the positional input formats read by main.f90:221-294 and :383-440 so the
-unchanged Fortran driver can start from exactly the same state (rest='ON').
+unchanged Fortran driver can start from exactly the same state (rest='ON');
+`write_input_xyz` writes the positions-only file of a fresh start (rest='OFF',
+main.f90:347-379).
 """
 from __future__ import annotations
 
@@ -69,6 +71,17 @@ def write_restart_xyz(path, p: TcpParams, x, v, f):
             idn = i + 1 if i < p.ne else i - p.ne + 1
             vals = " ".join("%.17g" % t for t in (*x[i], *v[i], *f[i]))
             fh.write(f"{name} {vals} {idn}\n")
+
+
+def write_input_xyz(path, p: TcpParams, x):
+    """input.xyz as read by the fresh start (rest = 'OFF', main.f90:347-379):
+    two header lines, then name x y z id per particle, electrons first."""
+    with open(path, "w") as fh:
+        fh.write(f"{p.n}\nframe = 0\n")
+        for i in range(p.n):
+            name = "Cu" if i < p.ne else "C"
+            idn = i + 1 if i < p.ne else i - p.ne + 1
+            fh.write(f"{name} {' '.join('%.17g' % t for t in x[i])} {idn}\n")
 
 
 def read_restart_xyz(path):

@@ -67,6 +67,16 @@ def test_force_rows_matches_reference():
         cpu.force_rows(p, d["in_x"], [p.n])
 
 
+def test_vverlet2_vanishing_vdf_bin_counts_nothing():
+    """Te = Ti = 1e-30 eV gives VDF bins of 8e-33: every kinetic energy lies past the
+    last bin, so nothing is counted (and nothing is written out of range)."""
+    p, d = load("vv2_notemp")
+    p = p.replace(Te=1e-30, Ti=1e-30)
+    v, _, _, _, vdf = cpu.vverlet2(p, 0, float(d["dt"]), d["in_v"], d["in_f"])
+    assert not vdf.any()
+    np.testing.assert_array_equal(v, d["out_v"])
+
+
 def test_psi():
     p, d = load("psi")
     np.testing.assert_array_equal(cpu.psi(p).ravel(), d["out_psi"].ravel())
