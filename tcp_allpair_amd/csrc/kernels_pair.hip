@@ -390,6 +390,8 @@ struct MixConst {
     float sx, sy, sz;          // L_d 2^-32
     float rc2;
     float half_alpha;          // alpha/2
+    float inv_half_alpha;      // 2/alpha: w = x/(2 + x) = 1/(1 + (2/alpha)/r) for x = alpha r
+    float two_rs;              // 2 lambda: the same for Kelbg's erfc(r/lambda)
     float na2l;                // -alpha^2 log2(e)
     float two_a_sqrtpi;        // 2 alpha/sqrt(pi)
     float inv_rs;              // 1/lambda
@@ -409,6 +411,8 @@ __device__ inline MixConst mix_const(const PairArgs& a)
     k.sz = (float)(a.box[2] * 0x1p-32);
     k.rc2 = (float)a.rc2;
     k.half_alpha = (float)(0.5 * a.alpha);
+    k.inv_half_alpha = (float)(2.0 / a.alpha);
+    k.two_rs = (float)(2.0 * a.rs);
     k.na2l = (float)(-a.alpha * a.alpha * 1.4426950408889634);
     k.two_a_sqrtpi = (float)(2.0 * a.alpha / kSqrtPi);
     k.inv_rs = (float)(1.0 / a.rs);
@@ -462,27 +466,26 @@ __device__ inline MixRaw mix_raw(const MixConst& k, const FixPos& pi, const FixP
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
     const float e2 = __builtin_amdgcn_exp2f(k.na2l * r2);
-    const float px = k.half_alpha * r;
-    const float t = __builtin_amdgcn_rcpf(px + 1.0f);
-    const float ec = e2 * erfcx_poly7(px * t);
+    // w = x/(2 + x), x = alpha r, from 1/r: one rcp and no multiply by r
+    const float ec = e2 * erfcx_poly7(__builtin_amdgcn_rcpf(fmaf(rinv, k.inv_half_alpha, 1.0f)));
     float num = fmaf(k.two_a_sqrtpi * r, e2, ec);
     float en = ec;
     if (reg) {
         const float sr = r * k.inv_rs;
         if (kKelbg) {
             const float y2 = __builtin_amdgcn_exp2f(-1.4426950408889634f * sr * sr);
-            const float hs = 0.5f * sr;
-            const float ecs = y2 * erfcx_poly(hs * __builtin_amdgcn_rcpf(hs + 1.0f));  // erfc(s)
+            // erfc(s) = exp(-s^2) P(s/(2 + s)), s/(2 + s) = 1/(1 + 2 lambda/r)
+            const float ecs = y2 * erfcx_poly(__builtin_amdgcn_rcpf(fmaf(rinv, k.two_rs, 1.0f)));
             num -= y2;
             en = fmaf(kSqrtPiF * sr, ecs, en - y2);
             if (o.in && r < k.rclose)
-                close_kelbg(sr, 2.0f * px, ecs, k.two_a_sqrtpi * r * e2, ec, &num, &en);
+                close_kelbg(sr, 2.0f * k.half_alpha * r, ecs, k.two_a_sqrtpi * r * e2, ec, &num, &en);
         } else {
             const float y = __builtin_amdgcn_exp2f(k.nl2rs * r);
             num = fmaf(-y, sr + 1.0f, num);
             en -= y;
             if (o.in && r < k.rclose)
-                close_ei(r, sr, 2.0f * px, y, ec, k.two_a_sqrtpi * r * e2, &num, &en);
+                close_ei(r, sr, 2.0f * k.half_alpha * r, y, ec, k.two_a_sqrtpi * r * e2, &num, &en);
         }
     }
     o.r = r;
@@ -697,6 +700,7 @@ struct MixUnits {
     float inv_u;             // FP32 residuals (physical) to u
     float rc2, rfix2;        // u^2
     float half_alpha, two_a_sqrtpi, inv_rs, nl2rs;  // x u
+    float inv_half_alpha, two_rs;  // 2/alpha, 2 lambda: / u
     float na2l;              // x u^2
     float rclose;            // / u
     float idx_ee, idx_ei, idx_ii;  // RDF bin widths (reciprocal) x u
@@ -715,6 +719,8 @@ __device__ inline MixUnits mix_units(const PairArgs& a, bool cubic)
     const double lmax = fmax(fmax(a.box[0], a.box[1]), a.box[2]);
     w.rfix2 = (float)(lmax * lmax * 0x1p-16 / (u * u));
     w.half_alpha = (float)(0.5 * a.alpha * u);
+    w.inv_half_alpha = (float)(2.0 / (a.alpha * u));
+    w.two_rs = (float)(2.0 * a.rs / u);
     w.two_a_sqrtpi = (float)(2.0 * a.alpha / kSqrtPi * u);
     w.inv_rs = (float)(u / a.rs);
     w.nl2rs = (float)(-1.4426950408889634 * u / a.rs);
@@ -729,11 +735,20 @@ __device__ inline MixUnits mix_units(const PairArgs& a, bool cubic)
     return w;
 }
 
+// a 32-bit word of LDS addressed by its byte offset
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
+
+// The three fixed-point coordinates of a j (12 bytes of its FixPos): one
+// ds_read_b96, no register for the unused fourth word.
+struct __align__(16) FixU3 {
+    uint32_t x, y, z;
+};
+
 // mix_raw for the uniform tile path, in the unit u of MixUnits: j's
 // fixed-point coordinates in qu, its FP32 residuals (read only for a pair
 // closer than L 2^-8) in q.
 template <bool kKelbg, bool kCubic>
-__device__ inline MixRaw mix_raw_u(const MixUnits& w, const FixPos& pi, uint4 qu, const FixPos& q,
+__device__ inline MixRaw mix_raw_u(const MixUnits& w, const FixPos& pi, FixU3 qu, const FixPos& q,
                                    bool reg)
 {
     MixRaw o;
@@ -753,9 +768,8 @@ __device__ inline MixRaw mix_raw_u(const MixUnits& w, const FixPos& pi, uint4 qu
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
     const float e2 = __builtin_amdgcn_exp2f(w.na2l * r2);
-    const float px = w.half_alpha * r;
-    const float t = __builtin_amdgcn_rcpf(px + 1.0f);
-    const float ec = e2 * erfcx_poly7(px * t);
+    // w = x/(2 + x), x = alpha r, from 1/r: one rcp and no multiply by r
+    const float ec = e2 * erfcx_poly7(__builtin_amdgcn_rcpf(fmaf(rinv, w.inv_half_alpha, 1.0f)));
     const float g = w.two_a_sqrtpi * r;  // 2 x / sqrt(pi)
     float num = fmaf(g, e2, ec);
     float en = ec;
@@ -763,16 +777,18 @@ __device__ inline MixRaw mix_raw_u(const MixUnits& w, const FixPos& pi, uint4 qu
         const float sr = r * w.inv_rs;
         if (kKelbg) {
             const float y2 = __builtin_amdgcn_exp2f(-1.4426950408889634f * sr * sr);
-            const float hs = 0.5f * sr;
-            const float ecs = y2 * erfcx_poly(hs * __builtin_amdgcn_rcpf(hs + 1.0f));  // erfc(s)
+            // erfc(s) = exp(-s^2) P(s/(2 + s)), s/(2 + s) = 1/(1 + 2 lambda/r)
+            const float ecs = y2 * erfcx_poly(__builtin_amdgcn_rcpf(fmaf(rinv, w.two_rs, 1.0f)));
             num -= y2;
             en = fmaf(kSqrtPiF * sr, ecs, en - y2);
-            if (o.in && r < w.rclose) close_kelbg(sr, 2.0f * px, ecs, g * e2, ec, &num, &en);
+            if (o.in && r < w.rclose)
+                close_kelbg(sr, 2.0f * w.half_alpha * r, ecs, g * e2, ec, &num, &en);
         } else {
             const float y = __builtin_amdgcn_exp2f(w.nl2rs * r);
             num = fmaf(-y, sr + 1.0f, num);
             en -= y;
-            if (o.in && r < w.rclose) close_ei(r, sr, 2.0f * px, y, ec, g * e2, &num, &en);
+            if (o.in && r < w.rclose)
+                close_ei(r, sr, 2.0f * w.half_alpha * r, y, ec, g * e2, &num, &en);
         }
     }
     o.r = r;
@@ -798,11 +814,13 @@ __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsig
     // the next step's coordinates are read while this step's pairs compute.
     const char* jb = (const char*)jpos;
     uint32_t off = (uint32_t)(tj * kTile + (int)(threadIdx.x % kTile)) * (uint32_t)sizeof(FixPos);
-    uint4 qu = *(const uint4*)(jb + off);
+    FixU3 qu = *(const FixU3*)(jb + off);
     const bool reg = si != sj || ((a.flags & TCP_REG_EE) != 0 && si == 0 && sj == 0);
     const int ht = si + sj;
     const float idx = pick3(w.idx_ee, w.idx_ei, w.idx_ii, ht);
     unsigned* hrow = hist + (si + sj) * kHistStride;
+    // LDS byte address of bin b of this row is hbias + 4 (2^23 + b) (mod 2^32)
+    const uint32_t hbias = (uint32_t)(uintptr_t)(LdsU32*)hrow - 4u * 0x4B000000u;
     float fx[kNI], fy[kNI], fz[kNI], u[kNI];
 #pragma unroll
     for (int n = 0; n < kNI; ++n) fx[n] = fy[n] = fz[n] = u[n] = 0.f;
@@ -810,7 +828,7 @@ __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsig
 #pragma unroll 2
     for (int s = 0; s < kTile; ++s) {  // by two: the carried offset and coordinates alternate registers
         const uint32_t noff = rot_u(off);
-        const uint4 nqu = *(const uint4*)(jb + noff);
+        const FixU3 nqu = *(const FixU3*)(jb + noff);
         const FixPos& q = *(const FixPos*)(jb + off);
 #pragma unroll
         for (int n = 0; n < kNI; ++n) {
@@ -825,10 +843,15 @@ __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsig
             ry = fmaf(-g, m.dy, ry);
             rz = fmaf(-g, m.dz, rz);
             // bins past Ndist+1 land in the padding bins 501..511, never read;
-            // without kClamp every pair inside rc has its bin below 511
+            // without kClamp every pair inside rc has its bin below 511.  The
+            // bin floor(r idx) comes out of the mantissa: r idx + (2^23 - 1/2),
+            // rounded once by the fma, is 2^23 + floor(r idx) unless r idx is an
+            // integer (then possibly one below)
             if (kRdf && m.in) {
-                const int b = (int)(m.r * idx);
-                atomicAdd(&hrow[kClamp ? min(b, kHistStride - 1) : b], 2u);
+                const uint32_t t = __float_as_uint(fmaf(m.r, idx, 8388607.5f));
+                const uint32_t b = kClamp ? min(t, 0x4B000000u + (kHistStride - 1)) : t;
+                __hip_atomic_fetch_add((LdsU32*)(uintptr_t)(hbias + 4u * b), 2u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
         off = noff;
