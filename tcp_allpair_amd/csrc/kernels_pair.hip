@@ -744,14 +744,24 @@ struct __align__(16) FixU3 {
     uint32_t x, y, z;
 };
 
-// mix_raw for the uniform tile path, in the unit u of MixUnits: j's
+// One pair of the uniform tile path in the unit u of MixUnits: j's
 // fixed-point coordinates in qu, its FP32 residuals (read only for a pair
-// closer than L 2^-8) in q.
-template <bool kKelbg, bool kCubic>
-__device__ inline MixRaw mix_raw_u(const MixUnits& w, const FixPos& pi, FixU3 qu, const FixPos& q,
-                                   bool reg)
+// closer than L 2^-8) in q.  Returns the separation, r, the cut-off flag, and
+// the masked charge-free sums' terms er = en/r and fs = num/r^3 (zero beyond
+// rc).  With num = en + (num - en), num/r = en/r + k e2 + (regularisation
+// term)/r, where k e2 = (2 alpha r/sqrt(pi)) e^{-alpha^2 r^2}/r needs no r:
+// the Deutsch term -e^{-r/lambda} r/lambda and the Kelbg term
+// -sqrt(pi) (r/lambda) erfc(r/lambda) give -y/lambda and -sqrt(pi) erfc/lambda.
+// Close electron-ion pairs take num and en from the cancellation-free series.
+struct MixUni {
+    float dx, dy, dz, r, er, fs;
+    bool in;
+};
+
+template <bool kKelbg, bool kCubic, bool kReg>
+__device__ inline MixUni mix_uni(const MixUnits& w, const FixPos& pi, FixU3 qu, const FixPos& q)
 {
-    MixRaw o;
+    MixUni o;
     const float ix = (float)(int)(pi.ux - qu.x), iy = (float)(int)(pi.uy - qu.y),
                 iz = (float)(int)(pi.uz - qu.z);
     o.dx = kCubic ? ix : ix * w.sx;
@@ -767,41 +777,51 @@ __device__ inline MixRaw mix_raw_u(const MixUnits& w, const FixPos& pi, FixU3 qu
     o.in = r2 < w.rc2;
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
+    const float rm = o.in ? rinv : 0.f;  // 1/r inside rc, else 0
+    const float t = rm * rinv;           // 1/r^2 inside rc
     const float e2 = __builtin_amdgcn_exp2f(w.na2l * r2);
     // w = x/(2 + x), x = alpha r, from 1/r: one rcp and no multiply by r
     const float ec = e2 * erfcx_poly7(__builtin_amdgcn_rcpf(fmaf(rinv, w.inv_half_alpha, 1.0f)));
-    const float g = w.two_a_sqrtpi * r;  // 2 x / sqrt(pi)
-    float num = fmaf(g, e2, ec);
-    float en = ec;
-    if (reg) {
+    float er, inner;  // en/r and num/r (inside rc)
+    if (kReg) {
         const float sr = r * w.inv_rs;
         if (kKelbg) {
             const float y2 = __builtin_amdgcn_exp2f(-1.4426950408889634f * sr * sr);
             // erfc(s) = exp(-s^2) P(s/(2 + s)), s/(2 + s) = 1/(1 + 2 lambda/r)
             const float ecs = y2 * erfcx_poly(__builtin_amdgcn_rcpf(fmaf(rinv, w.two_rs, 1.0f)));
-            num -= y2;
-            en = fmaf(kSqrtPiF * sr, ecs, en - y2);
-            if (o.in && r < w.rclose)
-                close_kelbg(sr, 2.0f * w.half_alpha * r, ecs, g * e2, ec, &num, &en);
+            er = fmaf(kSqrtPiF * sr, ecs, ec - y2) * rm;
+            inner = fmaf(-kSqrtPiF * w.inv_rs, ecs, fmaf(w.two_a_sqrtpi, e2, er));
+            if (o.in && r < w.rclose) {
+                float num, en;
+                close_kelbg(sr, 2.0f * w.half_alpha * r, ecs, w.two_a_sqrtpi * r * e2, ec, &num, &en);
+                er = en * rinv;
+                inner = num * rinv;
+            }
         } else {
             const float y = __builtin_amdgcn_exp2f(w.nl2rs * r);
-            num = fmaf(-y, sr + 1.0f, num);
-            en -= y;
-            if (o.in && r < w.rclose)
-                close_ei(r, sr, 2.0f * w.half_alpha * r, y, ec, g * e2, &num, &en);
+            er = (ec - y) * rm;
+            inner = fmaf(-w.inv_rs, y, fmaf(w.two_a_sqrtpi, e2, er));
+            if (o.in && r < w.rclose) {
+                float num, en;
+                close_ei(r, sr, 2.0f * w.half_alpha * r, y, ec, w.two_a_sqrtpi * r * e2, &num, &en);
+                er = en * rinv;
+                inner = num * rinv;
+            }
         }
+    } else {
+        er = ec * rm;
+        inner = fmaf(w.two_a_sqrtpi, e2, er);
     }
     o.r = r;
-    o.rinv = rinv;
-    o.num = num;
-    o.en = en;
+    o.er = er;
+    o.fs = t * inner;
     return o;
 }
 
 // kNI i tiles against one j tile, all live and of one species each: the
 // charge-free sums of num d / r^3 (i side, and the reactions rx, ry, rz) and
 // en / r in the unit of w; the caller applies the charges and powers of u.
-template <bool kRdf, bool kKelbg, int kNI, bool kClamp, bool kCubic>
+template <bool kRdf, bool kKelbg, int kNI, bool kClamp, bool kCubic, bool kReg>
 __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsigned* hist,
                                      const FixPos (&pi)[kNI], int si, int sj, const FixPos* jpos,
                                      int tj, float (&fxo)[kNI], float (&fyo)[kNI],
@@ -815,7 +835,6 @@ __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsig
     const char* jb = (const char*)jpos;
     uint32_t off = (uint32_t)(tj * kTile + (int)(threadIdx.x % kTile)) * (uint32_t)sizeof(FixPos);
     FixU3 qu = *(const FixU3*)(jb + off);
-    const bool reg = si != sj || ((a.flags & TCP_REG_EE) != 0 && si == 0 && sj == 0);
     const int ht = si + sj;
     const float idx = pick3(w.idx_ee, w.idx_ei, w.idx_ii, ht);
     unsigned* hrow = hist + (si + sj) * kHistStride;
@@ -832,13 +851,12 @@ __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsig
         const FixPos& q = *(const FixPos*)(jb + off);
 #pragma unroll
         for (int n = 0; n < kNI; ++n) {
-            const MixRaw m = mix_raw_u<kKelbg, kCubic>(w, pi[n], qu, q, reg);
-            const float rm = m.in ? m.rinv : 0.f;
-            const float g = m.num * (m.rinv * m.rinv) * rm;
+            const MixUni m = mix_uni<kKelbg, kCubic, kReg>(w, pi[n], qu, q);
+            const float g = m.fs;
             fx[n] = fmaf(g, m.dx, fx[n]);
             fy[n] = fmaf(g, m.dy, fy[n]);
             fz[n] = fmaf(g, m.dz, fz[n]);
-            u[n] = fmaf(m.en, rm, u[n]);
+            u[n] += m.er;
             rx = fmaf(-g, m.dx, rx);
             ry = fmaf(-g, m.dy, ry);
             rz = fmaf(-g, m.dz, rz);
@@ -924,17 +942,29 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
             const int q = tj * kTile + lane;
             if (all_uni && juni) {
                 float fx[kNI], fy[kNI], fz[kNI], u[kNI], rx, ry, rz;
+                // the tiles are single-species: regularised or not for the whole tile pair
+                const bool reg = s0 != sj || ((a.flags & TCP_REG_EE) != 0 && s0 == 0 && sj == 0);
+#define TCP_N3_MULTI(CLAMP, CUBIC, REG)                                                          \
+    n3_tile_multi<kRdf, kKelbg, kNI, CLAMP, CUBIC, REG>(w, a, hist, pi, s0, sj, jpos, tj, fx, fy, \
+                                                        fz, u, rx, ry, rz)
                 if (cubic) {
-                    if (k.rdf_clamp)
-                        n3_tile_multi<kRdf, kKelbg, kNI, true, true>(w, a, hist, pi, s0, sj, jpos, tj, fx, fy, fz, u, rx, ry, rz);
-                    else
-                        n3_tile_multi<kRdf, kKelbg, kNI, false, true>(w, a, hist, pi, s0, sj, jpos, tj, fx, fy, fz, u, rx, ry, rz);
+                    if (k.rdf_clamp) {
+                        if (reg) TCP_N3_MULTI(true, true, true);
+                        else TCP_N3_MULTI(true, true, false);
+                    } else {
+                        if (reg) TCP_N3_MULTI(false, true, true);
+                        else TCP_N3_MULTI(false, true, false);
+                    }
                 } else {
-                    if (k.rdf_clamp)
-                        n3_tile_multi<kRdf, kKelbg, kNI, true, false>(w, a, hist, pi, s0, sj, jpos, tj, fx, fy, fz, u, rx, ry, rz);
-                    else
-                        n3_tile_multi<kRdf, kKelbg, kNI, false, false>(w, a, hist, pi, s0, sj, jpos, tj, fx, fy, fz, u, rx, ry, rz);
+                    if (k.rdf_clamp) {
+                        if (reg) TCP_N3_MULTI(true, false, true);
+                        else TCP_N3_MULTI(true, false, false);
+                    } else {
+                        if (reg) TCP_N3_MULTI(false, false, true);
+                        else TCP_N3_MULTI(false, false, false);
+                    }
                 }
+#undef TCP_N3_MULTI
                 // the charge-free FP32 sums take eps qi qj u^-2 and qi qj u^-1 in FP64
                 const double qq = a.q[s0] * a.q[sj];
                 const double sf = a.eps * qq * w.f_scale, su = qq * w.e_scale;
