@@ -20,12 +20,13 @@
 ! (`mpirun -np 2`) rank 0, the PME rank of data.f90:151, does nothing (the
 ! LOCAL gather/scatter are dropped on both ranks alike) and rank 1 owns the GPU
 ! context with every particle, reciprocal part included.  With several direct
-! ranks (one PME group, at most 9 ranks) every rank keeps the reference's
-! communication: the direct ranks run the real-space ring (Force_direct_ext,
-! main.f90:107-130) on their own contexts, the PME rank receives every position
-! through the LOCAL gather (force.f90:679) and computes the reciprocal part on
-! its own context, and the kinetic sums of BEREND/ISOKIN are reduced over
-! COMM1D between the kick and the rescale (vverlet.f90:254, 278).
+! ranks every rank keeps the reference's communication: the direct ranks run
+! the real-space ring (Force_direct_ext, main.f90:107-130) on their own
+! contexts, each PME rank receives its group's positions through the LOCAL
+! gather (force.f90:679), the PME ranks of several groups exchange them, and
+! each computes the reciprocal part on its own context; the kinetic sums of
+! BEREND/ISOKIN are reduced over COMM1D between the kick and the rescale
+! (vverlet.f90:254, 278).
 !
 ! Environment: TCP_PRECISION=MIXED selects the mixed-precision pair kernel
 ! (FP64 otherwise).
@@ -62,7 +63,7 @@ LOGICAL FUNCTION defer_forces(NS)
    defer_forces = sync_every == 0 .AND. direct_ranks(NS) == 1
 END FUNCTION defer_forces
 
-! direct ranks of the (single) PME group
+! direct ranks of all PME groups
 INTEGER FUNCTION direct_ranks(NS)
    TYPE(NM) :: NS
    direct_ranks = NS%NCPUS - NS%Npe
@@ -319,7 +320,6 @@ END IF
 param%a = 3.1D0/param%rc
 param%rc2 = param%rc**2
 PSI = 0.0D0
-IF (NS%Npe > 1) STOP " *** GPU shim: one PME group only (at most 9 MPI ranks) *** "
 IF (COMM%TAG) THEN
    ! Uo from the rank's own particles, as force.f90:45-46
    CALL ensure_ctx(NS, param, NS%Nel, NS%Nion, direct_ranks(NS) == 1)
@@ -402,10 +402,14 @@ END SUBROUTINE Force_direct_sum
 
 ! One direct rank: positions reach the GPU in SPME_SUM, no LOCAL gather.
 ! Several: the reference's LOCAL gather (force.f90:679-680), then the PME rank
-! computes the reciprocal forces of every gathered particle on its context and
+! computes the reciprocal forces of the gathered particles on its context and
 ! leaves them in pme for SPME_SUM's scatter.  pme holds each direct rank's
 ! electrons then its ions (force.f90:694-697); the context holds all electrons
-! first.
+! first.  With several PME groups (data.f90:151) the mesh needs every group's
+! charges (the reference sums the groups' grids over the PME ranks,
+! force.f90:821-822): the PME ranks exchange their groups' positions over
+! their COMM1D, groups in order, and each computes the whole system's
+! reciprocal part and keeps its own group's forces.
 SUBROUTINE Force_SPME4(NS, param, sys, COMM, PSI, gsend, pme)
 USE DATASTR
 USE TCP_GPU_STATE
@@ -418,35 +422,80 @@ TYPE(MP):: COMM
 TYPE(GH):: gsend(NS%Ng), pme(NS%NLpt)
 REAL(KIND=DP) :: PSI(param%K(1), param%K(2), param%K(3))
 REAL(C_DOUBLE) :: um
-INTEGER :: l, x, m, ke, ki, IERR
+INTEGER :: l, x, m, ke, ki, IERR, ge, gi, e0, i0, g, mine
+INTEGER, ALLOCATABLE :: cnt(:,:), rcv(:), dsp(:)
+REAL(C_DOUBLE), ALLOCATABLE :: gxe(:,:), gxi(:,:)
 IF (direct_ranks(NS) == 1) RETURN
 CALL MPI_GATHERV(gsend, NS%CNT(COMM%LID+1), COMM%PTCL, pme, NS%CNT, &
      NS%DPL, COMM%PTCL, 0, COMM%LOCAL, IERR)
 IF (COMM%TAG) RETURN
-m = 0
-ke = 0
-ki = NS%NTel
-DO l = 1, COMM%Ncolmn - 1
-   DO x = 1, NS%NGHB(l,1)
-      m = m + 1
-      ke = ke + 1
-      hx(:, ke) = pme(m)%xx
+! the group's direct ranks: LOCAL ranks 1 .. Nlsize-1 (main.f90:322-333)
+ge = SUM(NS%NGHB(1:COMM%Nlsize-1, 1))
+gi = SUM(NS%NGHB(1:COMM%Nlsize-1, 2))
+e0 = 0
+i0 = 0
+IF (NS%Npe > 1) THEN
+   ALLOCATE(cnt(2, NS%Npe), rcv(NS%Npe), dsp(NS%Npe), gxe(3, MAX(ge, 1)), gxi(3, MAX(gi, 1)))
+   CALL MPI_ALLGATHER((/ge, gi/), 2, MPI_INTEGER, cnt, 2, MPI_INTEGER, COMM%COMM1D, IERR)
+   mine = COMM%MYID2 + 1
+   DO g = 1, mine - 1
+      e0 = e0 + cnt(1, g)
+      i0 = i0 + cnt(2, g)
    END DO
-   DO x = 1, NS%NGHB(l,2)
-      m = m + 1
-      ki = ki + 1
-      hx(:, ki) = pme(m)%xx
+   m = 0
+   ke = 0
+   ki = 0
+   DO l = 1, COMM%Nlsize - 1
+      DO x = 1, NS%NGHB(l,1)
+         m = m + 1
+         ke = ke + 1
+         gxe(:, ke) = pme(m)%xx
+      END DO
+      DO x = 1, NS%NGHB(l,2)
+         m = m + 1
+         ki = ki + 1
+         gxi(:, ki) = pme(m)%xx
+      END DO
    END DO
-END DO
+   dsp(1) = 0
+   DO g = 1, NS%Npe
+      rcv(g) = 3*cnt(1, g)
+      IF (g > 1) dsp(g) = dsp(g-1) + rcv(g-1)
+   END DO
+   CALL MPI_ALLGATHERV(gxe, 3*ge, MPI_REAL8, hx, rcv, dsp, MPI_REAL8, COMM%COMM1D, IERR)
+   DO g = 1, NS%Npe
+      rcv(g) = 3*cnt(2, g)
+      IF (g > 1) dsp(g) = dsp(g-1) + rcv(g-1)
+   END DO
+   CALL MPI_ALLGATHERV(gxi, 3*gi, MPI_REAL8, hx(1, NS%NTel + 1), rcv, dsp, MPI_REAL8, &
+        COMM%COMM1D, IERR)
+   DEALLOCATE(cnt, rcv, dsp, gxe, gxi)
+ELSE
+   m = 0
+   ke = 0
+   ki = NS%NTel
+   DO l = 1, COMM%Nlsize - 1
+      DO x = 1, NS%NGHB(l,1)
+         m = m + 1
+         ke = ke + 1
+         hx(:, ke) = pme(m)%xx
+      END DO
+      DO x = 1, NS%NGHB(l,2)
+         m = m + 1
+         ki = ki + 1
+         hx(:, ki) = pme(m)%xx
+      END DO
+   END DO
+END IF
 hf = 0.0D0
 CALL tcp_check(tcp_set_state(tcp_ctx_h, C_LOC(hx), C_NULL_PTR, C_LOC(hf)), 'tcp_set_state')
 CALL tcp_check(tcp_force_recip(tcp_ctx_h, um), 'tcp_force_recip')
 sys%Um = um
 CALL tcp_check(tcp_get_state(tcp_ctx_h, C_NULL_PTR, C_NULL_PTR, C_LOC(hf)), 'tcp_get_state')
 m = 0
-ke = 0
-ki = NS%NTel
-DO l = 1, COMM%Ncolmn - 1
+ke = e0
+ki = NS%NTel + i0
+DO l = 1, COMM%Nlsize - 1
    DO x = 1, NS%NGHB(l,1)
       m = m + 1
       ke = ke + 1

@@ -7,7 +7,9 @@ every particle on its context.  `-np 3` and `-np 4`: two and three direct
 ranks run the reference's real-space ring (the even case with its antipodal
 exchange), the PME rank computes the reciprocal part of every gathered
 particle on its own context, and BEREND/ISOKIN reduce their kinetic sums
-over the direct ranks.
+over the direct ranks.  `-np 10` and `-np 12`: two PME groups
+(data.f90:151) of four and five direct ranks; the PME ranks exchange their
+groups' positions and each computes the whole mesh.
 
 Tolerances: the reference's own writers print ES15.8 energies and ES14.6
 positions (result.f90:324, :131), so the files are compared at 2e-8 and 2e-6
@@ -38,7 +40,8 @@ THERMO_NAME = {NOTEMP: "NOTEMP", BEREND: "BEREND", ISOKIN: "ISOKIN"}
 @pytest.mark.parametrize("nranks,thermo,sync", [(2, NOTEMP, False), (2, NOTEMP, True),
                                                 (3, NOTEMP, False), (4, NOTEMP, False),
                                                 (4, BEREND, False), (3, ISOKIN, False),
-                                                (2, ISOKIN, False)])
+                                                (2, ISOKIN, False), (10, NOTEMP, False),
+                                                (12, BEREND, False)])
 def test_dropin_driver(tmp_path, nranks, thermo, sync):
     """sync: TCP_DROPIN_SYNC=1, every subroutine downloads what it writes (with one
     direct rank the step's results otherwise stay on the device until VVerlet*2)."""
@@ -88,7 +91,8 @@ def test_dropin_driver(tmp_path, nranks, thermo, sync):
     hv = vdf5.reshape(2, NBIN)
     np.testing.assert_allclose(vcols[:, 1], hv[0, :NBIN - 1] / ntau, rtol=2e-6)
     np.testing.assert_allclose(vcols[:, 3], hv[1, :NBIN - 1] / ntau, rtol=2e-6)
-    if (nranks - 1) % 2 == 0 and nranks > 2:
+    direct = nranks - ((nranks - 1) // 8 + 1)  # ranks minus PME ranks (data.f90:151)
+    if direct % 2 == 0:  # antipodal blocks counted by both partners (main.f90:121-130)
         return
     # RDF001.dat: counts of the first 5 steps, normalised as result.f90:244-252
     V = p.volume
