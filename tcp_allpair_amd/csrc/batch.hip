@@ -74,6 +74,11 @@ struct tcp_batch {
         }
     } key;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // FP64: the 4 x 4 tile launch runs on a second stream (same priority),
+    // forked from and joined back to `stream` by these events, so its blocks
+    // fill the CUs the tail of the 8 x 8 tile launch leaves idle
+    hipStream_t stream2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     double pair_ms = 0, steps_ms = 0;
     uint64_t counter = 0;
     std::vector<void*> allocs;
@@ -209,14 +214,24 @@ void launch_pair(tcp_batch* b)
         launch_pair_mixed_n3_batch(b->d_reps, b->d_items, b->nitems, b->d_queue, b->shape, true,
                                    b->kelbg, b->grid, b->stream);
     else {
-        // two persistent launches, each with its own queue (d_queue[0..1], [2..3])
+        // two persistent launches, each with its own queue (d_queue[0..1], [2..3]);
+        // they touch disjoint replicas, so the second may run beside the first
         const int nfull = b->nitems - b->nfine;
+        const bool both = nfull > 0 && b->nfine > 0;
+        if (both) {
+            (void)hipEventRecord(b->fork, b->stream);
+            (void)hipStreamWaitEvent(b->stream2, b->fork, 0);
+        }
         if (nfull > 0)
             launch_pair_f64_batch(b->d_reps, b->d_items, nfull, b->d_queue, true, b->kelbg, b->grid,
                                   0, b->stream);
         if (b->nfine > 0)
             launch_pair_f64_batch(b->d_reps, b->d_items + nfull, b->nfine, b->d_queue + 2, true,
-                                  b->kelbg, b->grid_fine, kFineN3Shape, b->stream);
+                                  b->kelbg, b->grid_fine, kFineN3Shape, both ? b->stream2 : b->stream);
+        if (both) {
+            (void)hipEventRecord(b->join, b->stream2);
+            (void)hipStreamWaitEvent(b->stream, b->join, 0);
+        }
     }
 }
 
@@ -278,12 +293,16 @@ int tcp_batch_destroy(tcp_batch* b)
 {
     if (!b) return 0;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->stream2) (void)hipStreamSynchronize(b->stream2);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
     for (auto e : b->ev)
         if (e) (void)hipEventDestroy(e);
     for (void* a : b->allocs) (void)hipFree(a);
     for (tcp_ctx* c : b->reps) tcp_destroy(c);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stream2) (void)hipStreamDestroy(b->stream2);
     delete b;
     return 0;
 }
@@ -324,8 +343,12 @@ int tcp_batch_create(tcp_batch** out, int nrep, const tcp_params* p, int precisi
     if (hipGetDevice(&b->device) != hipSuccess ||
         hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
         return bail(bfail(b, TCP_ERR_HIP, "no HIP device"));
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking) != hipSuccess)
         return bail(bfail(b, TCP_ERR_HIP, "hipStreamCreate failed"));
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)
+        return bail(bfail(b, TCP_ERR_HIP, "hipEventCreate failed"));
     for (auto& e : b->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(bfail(b, TCP_ERR_HIP, "hipEventCreate failed"));
     int rc = 0;
