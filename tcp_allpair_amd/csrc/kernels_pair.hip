@@ -887,13 +887,36 @@ __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsig
     }
 }
 
+// Reaction staging of the blocks with one i tile per wave on 4 x 4 tiles:
+// each wave's FP32 reaction sums for every j tile and the FP64 factor they
+// take (eps qi qj u^-2 for a uniform tile pair, 1 otherwise), 12 KB of LDS.
+template <int kBT, int kW>
+struct N3Stage {
+    static constexpr bool kOn = kBT == kW && kBT == 4;
+    float r[kOn ? kW * kBT * 3 * kTile : 1];  // [wave][j tile][axis][lane]
+    double scale[kOn ? kW * kBT : 1];         // [wave][j tile]
+};
+
 // One block pair (row br, column bc) of kBT x kBT tile pairs by kW wavefronts.
-// jpos, racc, hist: the block's LDS (kBT * kTile particles).
+// jpos, racc, hist, stg: the block's LDS (kBT * kTile particles).  The sums
+// of a block do not depend on the order in which its waves run, so the
+// kernel's result is the same from run to run:
+//  - with staging (N3Stage::kOn) each wave writes its reactions for every j
+//    tile to its own slots, and the slots are summed in wave order at the end;
+//  - otherwise the m j tiles of the block column are visited in kW turns of
+//    s = ceil(m / kW) steps, wave w taking the j tile (t + s w) mod s kW at
+//    step t, with a barrier between turns: within a turn two steps' tiles
+//    differ by less than s and two waves' tiles by a nonzero multiple of s,
+//    so each j tile's reactions (racc) take at most one wave's add per turn,
+//    in an order the turns fix.  Every wave still meets every j tile once.
+//  The diagonal block's i sums join racc after its last step.
 template <bool kRdf, bool kKelbg, int kBT, int kW>
 __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
-                                double (*racc)[kBT * kTile], unsigned* hist)
+                                double (*racc)[kBT * kTile], unsigned* hist,
+                                N3Stage<kBT, kW>& stg)
 {
     constexpr int kBP = kBT * kTile;  // particles per block side
+    static_assert((kBT & (kBT - 1)) == 0 && kBT % kW == 0, "kBT: a power of two, kW divides it");
     const bool dblk = br == bc;
     const int n = a.n_i, ne = a.ne_i;
     for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
@@ -909,13 +932,42 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
     const MixConst k = mix_const(a);
     const bool cubic = a.box[0] == a.box[1] && a.box[1] == a.box[2];
     const MixUnits w = mix_units(a, cubic);
-    const int wave = threadIdx.x / kTile, lane = threadIdx.x % kTile;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile), lane = threadIdx.x % kTile;
     const int ntiles = (n + kTile - 1) / kTile;
     // a tile holds one species and only live particles
     auto uniform_tile = [&](int T) {
         return (T * kTile >= ne || T * kTile + kTile - 1 < ne) && T * kTile + kTile - 1 < n;
     };
     constexpr int kNI = kBT / kW;  // i tiles per wave
+    constexpr bool kStage = N3Stage<kBT, kW>::kOn;
+    const int m = min(kBT, ntiles - bc * kBT);  // j tiles of this block column
+    const int nturn = kStage ? 1 : kW;
+    const int s = kStage ? kBT : (m + kW - 1) / kW;  // steps per turn
+    const int L = s * nturn;                          // steps: m <= L <= kBT
+    // j tile of this wave at step t (tiles past m are skipped)
+    auto jtile = [&](int t) {
+        const int x = kStage ? t : t + s * wave;
+        return x >= L ? x - L : x;
+    };
+    if (kStage) {  // slots of tiles this wave does not meet stay zero
+        for (int c = 0; c < kBT * 3; ++c) stg.r[(wave * kBT * 3 + c) * kTile + lane] = 0.f;
+        if (lane < kBT) stg.scale[wave * kBT + lane] = 0.0;
+    }
+    // reactions of this wave's tile pair with j tile tj, times sc
+    auto react = [&](int tj, float rx, float ry, float rz, double sc) {
+        if (kStage) {
+            float* r = stg.r + (wave * kBT + tj) * 3 * kTile + lane;
+            r[0] = rx;
+            r[kTile] = ry;
+            r[2 * kTile] = rz;
+            if (lane == 0) stg.scale[wave * kBT + tj] = sc;
+        } else {
+            const int q = tj * kTile + lane;
+            atomicAdd(&racc[0][q], sc * (double)rx);
+            atomicAdd(&racc[1][q], sc * (double)ry);
+            atomicAdd(&racc[2][q], sc * (double)rz);
+        }
+    };
     if (!dblk) {
         // off the diagonal: all kNI i tiles of the wave walk the j tiles together
         FixPos pi[kNI];
@@ -933,13 +985,14 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
             all_uni = all_uni && exists[h] && uniform_tile(I) && ((I * kTile >= ne) == (s0 != 0));
             Fx[h] = Fy[h] = Fz[h] = U[h] = 0.0;
         }
-        for (int tj = 0; tj < kBT; ++tj) {
+        for (int t = 0; t < L; ++t) {
+            if (t && t % s == 0) __syncthreads();  // the last turn's reaction adds are done
+            const int tj = jtile(t);
+            if (tj >= m) continue;
             const int J = bc * kBT + tj;
-            if (J >= ntiles) break;
             const FixPos pj = jpos[tj * kTile + lane];
             const int sj = J * kTile >= ne;
             const bool juni = uniform_tile(J);
-            const int q = tj * kTile + lane;
             if (all_uni && juni) {
                 float fx[kNI], fy[kNI], fz[kNI], u[kNI], rx, ry, rz;
                 // the tiles are single-species: regularised or not for the whole tile pair
@@ -975,9 +1028,7 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
                     Fz[h] += sf * (double)fz[h];
                     U[h] += su * (double)u[h];
                 }
-                atomicAdd(&racc[0][q], sf * (double)rx);
-                atomicAdd(&racc[1][q], sf * (double)ry);
-                atomicAdd(&racc[2][q], sf * (double)rz);
+                react(tj, rx, ry, rz, sf);
             } else {
 #pragma unroll
                 for (int h = 0; h < kNI; ++h) {
@@ -994,9 +1045,7 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
                     Fy[h] += (double)fy;
                     Fz[h] += (double)fz;
                     U[h] += (double)u;
-                    atomicAdd(&racc[0][q], (double)rx);
-                    atomicAdd(&racc[1][q], (double)ry);
-                    atomicAdd(&racc[2][q], (double)rz);
+                    react(tj, rx, ry, rz, 1.0);
                 }
             }
         }
@@ -1010,67 +1059,87 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
             a.pu[(size_t)bc * n + ip[h]] = U[h];
         }
     }
-    for (int h = 0; h < (dblk ? kBT / kW : 0); ++h) {
-        const int ti = wave + h * kW;
-        const int I = br * kBT + ti;
-        if (I >= ntiles) continue;
-        const int ip = I * kTile + lane;
-        const bool ilive = ip < n;
-        const FixPos pi = a.ui[min(ip, n - 1)];
-        const int si = ip >= ne;
-        // the i tile holds one species and only live particles
-        const bool iuni = (I * kTile >= ne || I * kTile + kTile - 1 < ne) && I * kTile + kTile - 1 < n;
-        double Fx = 0.0, Fy = 0.0, Fz = 0.0, U = 0.0;
-        for (int tj = dblk ? ti : 0; tj < kBT; ++tj) {
+    if (dblk) {
+        // the tile pairs (ti, tj >= ti) of the diagonal block, in the same
+        // j-tile turns; each wave adds its i sums to its own tiles at the end
+        FixPos pi[kNI];
+        bool exists[kNI], iuni[kNI];
+        double Fx[kNI], Fy[kNI], Fz[kNI], U[kNI];
+#pragma unroll
+        for (int h = 0; h < kNI; ++h) {
+            const int I = br * kBT + wave + h * kW;
+            exists[h] = I < ntiles;
+            pi[h] = a.ui[min(I * kTile + lane, n - 1)];
+            // the i tile holds one species and only live particles
+            iuni[h] = (I * kTile >= ne || I * kTile + kTile - 1 < ne) && I * kTile + kTile - 1 < n;
+            Fx[h] = Fy[h] = Fz[h] = U[h] = 0.0;
+        }
+        for (int t = 0; t < L; ++t) {
+            if (t && t % s == 0) __syncthreads();  // the last turn's reaction adds are done
+            const int tj = jtile(t);
+            if (tj >= m) continue;
             const int J = bc * kBT + tj;
-            if (J >= ntiles) break;
-            float fx, fy, fz, u, rx, ry, rz;
             const FixPos pj = jpos[tj * kTile + lane];
             const int sj_uni = J * kTile >= ne;
-            const bool uni = iuni && (J * kTile >= ne || J * kTile + kTile - 1 < ne) &&
-                             J * kTile + kTile - 1 < n;
-            if (dblk && tj == ti) {
-                if (uni)
-                    n3_tile<kRdf, kKelbg, true, true>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
-                else
-                    n3_tile<kRdf, kKelbg, true, false>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
-            } else {
-                if (uni)
-                    n3_tile<kRdf, kKelbg, false, true>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
-                else
-                    n3_tile<kRdf, kKelbg, false, false>(k, a, hist, pi, si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
-                const int q = tj * kTile + lane;
-                atomicAdd(&racc[0][q], (double)rx);
-                atomicAdd(&racc[1][q], (double)ry);
-                atomicAdd(&racc[2][q], (double)rz);
+            const bool juni = (J * kTile >= ne || J * kTile + kTile - 1 < ne) && J * kTile + kTile - 1 < n;
+#pragma unroll
+            for (int h = 0; h < kNI; ++h) {
+                const int ti = wave + h * kW;
+                if (!exists[h] || tj < ti) continue;
+                const int ip = (br * kBT + ti) * kTile + lane;
+                const bool ilive = ip < n;
+                const int si = ip >= ne;
+                const bool uni = iuni[h] && juni;
+                float fx, fy, fz, u, rx, ry, rz;
+                if (tj == ti) {
+                    if (uni)
+                        n3_tile<kRdf, kKelbg, true, true>(k, a, hist, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                    else
+                        n3_tile<kRdf, kKelbg, true, false>(k, a, hist, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                } else {
+                    if (uni)
+                        n3_tile<kRdf, kKelbg, false, true>(k, a, hist, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                    else
+                        n3_tile<kRdf, kKelbg, false, false>(k, a, hist, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                    react(tj, rx, ry, rz, 1.0);
+                }
+                Fx[h] += (double)fx;
+                Fy[h] += (double)fy;
+                Fz[h] += (double)fz;
+                U[h] += (double)u;
             }
-            Fx += (double)fx;
-            Fy += (double)fy;
-            Fz += (double)fz;
-            U += (double)u;
         }
-        if (dblk) {
-            const int q = ti * kTile + lane;
-            atomicAdd(&racc[0][q], Fx);
-            atomicAdd(&racc[1][q], Fy);
-            atomicAdd(&racc[2][q], Fz);
-            atomicAdd(&racc[3][q], U);
-        } else if (ilive) {
-            double* pf = a.pf + (size_t)bc * 3 * n;
-            pf[ip] = Fx;
-            pf[n + ip] = Fy;
-            pf[2 * n + ip] = Fz;
-            a.pu[(size_t)bc * n + ip] = U;
+        if (!kStage) __syncthreads();  // every reaction of the block is in racc
+#pragma unroll
+        for (int h = 0; h < kNI; ++h) {
+            if (!exists[h]) continue;
+            const int q = (wave + h * kW) * kTile + lane;
+            racc[0][q] += Fx[h];
+            racc[1][q] += Fy[h];
+            racc[2][q] += Fz[h];
+            racc[3][q] += U[h];
         }
     }
     __syncthreads();
     for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
         const int jp = bc * kBP + p;
         if (jp >= n) continue;
+        double rx = racc[0][p], ry = racc[1][p], rz = racc[2][p];
+        if (kStage) {  // the staged reactions, in wave order
+            const int tj = p / kTile, l = p % kTile;
+#pragma unroll
+            for (int v = 0; v < kW; ++v) {
+                const double sc = stg.scale[v * kBT + tj];
+                const float* r = stg.r + (v * kBT + tj) * 3 * kTile + l;
+                rx += sc * (double)r[0];
+                ry += sc * (double)r[kTile];
+                rz += sc * (double)r[2 * kTile];
+            }
+        }
         double* pf = a.pf + (size_t)br * 3 * n;
-        pf[jp] = racc[0][p];
-        pf[n + jp] = racc[1][p];
-        pf[2 * n + jp] = racc[2][p];
+        pf[jp] = rx;
+        pf[n + jp] = ry;
+        pf[2 * n + jp] = rz;
         if (dblk) a.pu[(size_t)br * n + jp] = racc[3][p];  // zero off the diagonal: not stored
     }
     if (kRdf) {
@@ -1459,12 +1528,13 @@ __global__ __launch_bounds__(kN3Waves * kTile, 4) void pair_mixed_n3_kernel(Pair
     __shared__ FixPos jpos[kBlkTiles * kTile];
     __shared__ double racc[4][kBlkTiles * kTile];
     __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
+    __shared__ N3Stage<kBlkTiles, kN3Waves> stg;
     __shared__ int slot;
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nblk) break;
         const int2 rc = blkmap[it];
-        n3_block<kRdf, kKelbg, kBlkTiles, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist);
+        n3_block<kRdf, kKelbg, kBlkTiles, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist, stg);
     }
     queue_done(queue);
 }
@@ -1480,12 +1550,13 @@ __global__ __launch_bounds__(kN3Waves * kTile, 4) void pair_mixed_n3_fine_kernel
     __shared__ FixPos jpos[kBT * kTile];
     __shared__ double racc[4][kBT * kTile];
     __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
+    __shared__ N3Stage<kBT, kN3Waves> stg;
     __shared__ int slot;
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nblk) break;
         const int2 rc = blkmap[it];
-        n3_block<kRdf, kKelbg, kBT, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist);
+        n3_block<kRdf, kKelbg, kBT, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist, stg);
     }
     queue_done(queue);
 }
@@ -1509,12 +1580,13 @@ __global__ __launch_bounds__(kW * kTile, (kBT / kW) > 2 ? 2 : 4) void pair_mixed
     __shared__ FixPos jpos[kBT * kTile];
     __shared__ double racc[4][kBT * kTile];
     __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
+    __shared__ N3Stage<kBT, kW> stg;
     __shared__ int slot;
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nitems) break;
         const PairItem w = items[it];
-        n3_block<kRdf, kKelbg, kBT, kW>(reps[w.rep].pa, w.a, w.b, jpos, racc, hist);
+        n3_block<kRdf, kKelbg, kBT, kW>(reps[w.rep].pa, w.a, w.b, jpos, racc, hist, stg);
     }
     queue_done(queue);
 }

@@ -2,7 +2,7 @@
 
 Tolerances:
   FP64 batch vs each replica's own context:   bit for bit (state, trace, histograms)
-  MIXED batch at its context's block shape:   1e-11 relative (LDS-atomic reaction order)
+  MIXED batch at its context's block shape:   bit for bit
   MIXED batch, every shape, vs FP64:          RMS relative force error <= 1e-6
 """
 import sys
@@ -117,13 +117,11 @@ def test_fp64_batch_two_shapes_bitwise():
 def test_mixed_batch_matches_standalone(shape, large):
     """A single context runs the same kernel body on 4 x 4 tile blocks for
     systems as small as the five replicas (shape 3) and on 8 x 8 tile blocks for
-    a 40000-particle one (shape 0); the batch at the context's shape matches it.
-    The reactions of different waves meet in LDS FP64 atomics whose order may
-    vary, so the comparison allows rounding (histograms are integers)."""
+    a 40000-particle one (shape 0); the batch at the context's shape matches it
+    bit for bit (the waves of a block add their reactions in fixed turns)."""
     say("case batch mixed vs standalone, shape", shape)
     reps = [make_plasma(20000, 20000, seed=120, recip=RECIP_NONE)] if large else replicas()
-    assert_bitwise(reps, MIXED, 5 if not large else 2, NOTEMP, reduced_dt(0.005), shape=shape,
-                   rtol=1e-11)
+    assert_bitwise(reps, MIXED, 5 if not large else 2, NOTEMP, reduced_dt(0.005), shape=shape)
 
 
 @pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
@@ -208,11 +206,10 @@ def test_autotune_and_graph_reuse():
         t2 = b.md_steps(3, NOTEMP, dt)  # replays the captured step
         s1 = [b.get_state(r) for r in range(len(reps))]
     tr, st, _ = run_batch(reps, MIXED, 6, NOTEMP, dt, shape=0)
-    # the mixed kernel adds the reactions of its waves with LDS FP64 atomics, in
-    # the order the waves run: replays agree with a fresh run to rounding
-    np.testing.assert_allclose(np.concatenate([t1, t2]), tr, rtol=1e-11, atol=0)
+    # replays of the captured step equal a fresh run bit for bit
+    np.testing.assert_array_equal(np.concatenate([t1, t2]), tr)
     for r in range(len(reps)):
-        np.testing.assert_allclose(s1[r][0], st[r][0], rtol=1e-11, atol=1e-12 * np.abs(st[r][0]).max())
+        np.testing.assert_array_equal(s1[r][0], st[r][0])
 
 
 def test_batch_errors():

@@ -373,21 +373,30 @@ def test_isokin_32k_spme64_vs_oracle():
     np.testing.assert_allclose(tr[:, 5], 3 * p.ni * p.Ti, rtol=1e-12)
 
 
-@pytest.mark.parametrize("recip,order", [(RECIP_NONE, 4), (RECIP_SPME, 4), (RECIP_SPME, 8),
-                                         (RECIP_EWALD, 4)])
-def test_fp64_deterministic(recip, order):
-    """Two runs of the same FP64 steps are identical bit for bit, the reciprocal part
-    included (fixed-point brick spreading, chunked structure factors)."""
-    say("case deterministic", recip, order)
-    p, x, v = make_plasma(3000, 3000, seed=5, K=(32, 32, 32), order=order, recip=recip, kmax=5)
+@pytest.mark.parametrize("precision,recip,order,ne", [(FP64, RECIP_NONE, 4, 3000),
+                                                     (FP64, RECIP_SPME, 4, 3000),
+                                                     (FP64, RECIP_SPME, 8, 3000),
+                                                     (FP64, RECIP_EWALD, 4, 3000),
+                                                     (MIXED, RECIP_SPME, 4, 3000),
+                                                     (MIXED, RECIP_NONE, 4, 20000)])
+def test_deterministic(precision, recip, order, ne):
+    """Two runs of the same steps are identical bit for bit, the reciprocal part
+    included (fixed-point brick spreading, chunked structure factors).  The
+    mixed kernel's waves add their reactions in turns fixed by the j-tile step,
+    so it repeats as well: 6000 particles run its 4 x 4 tile blocks, 40000 its
+    8 x 8 ones."""
+    say("case deterministic", precision, recip, order, 2 * ne)
+    p, x, v = make_plasma(ne, ne, seed=5, K=(32, 32, 32), order=order, recip=recip, kmax=5)
+    nsteps = 3 if ne <= 3000 else 2
     outs = []
     for _ in range(2):
-        with Context(p, FP64) as c:
+        with Context(p, precision) as c:
             c.set_state(x=x, v=v, f=np.zeros_like(x))
-            tr = c.md_steps(3, NOTEMP, reduced_dt(0.005))
-            outs.append((tr, c.get_state()[2]))
+            tr = c.md_steps(nsteps, NOTEMP, reduced_dt(0.005))
+            outs.append((tr, c.get_state()[2], c.histograms()[0]))
     np.testing.assert_array_equal(outs[0][0], outs[1][0])
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    np.testing.assert_array_equal(outs[0][2], outs[1][2])
 
 
 def test_mixed_vs_fp64_16k():
@@ -427,8 +436,8 @@ def test_mixed_vs_fp64_box_shapes(aniso, flags):
 def test_fullsize_mixed_step_properties():
     """131072 particles, SPME 128^3 order 6, mixed: after a step the mean force
     is removed exactly and the energies are finite; two calls of one step each
-    equal one call of two steps (the state carries over on the device) up to
-    the order of the mixed kernel's LDS FP64 atomic adds of reactions."""
+    equal one call of two steps bit for bit (the state carries over on the
+    device)."""
     say("case fullsize")
     p, x, v = make_plasma(65536, 65536, seed=7, K=(128, 128, 128), order=6)
     dt = reduced_dt(0.002)
@@ -441,9 +450,9 @@ def test_fullsize_mixed_step_properties():
         c.set_state(x=x, v=v, f=np.zeros_like(x))
         t12 = c.md_steps(2, NOTEMP, dt)
         s2 = c.get_state()
-    np.testing.assert_allclose(np.vstack([t1, t2]), t12, rtol=1e-12)
+    np.testing.assert_array_equal(np.vstack([t1, t2]), t12)
     for a, b in zip(s1, s2):
-        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+        np.testing.assert_array_equal(a, b)
     f = s1[2]
     assert np.all(np.isfinite(t12)) and np.all(np.isfinite(f))
     assert np.abs(f.sum(0)).max() <= 1e-10 * np.abs(f).sum()

@@ -1,9 +1,8 @@
 """tcp_force_direct_noalloc (no device allocation, scratch in a static device
 array) against tcp_force_direct on the same positions, through the C ABI.
 
-Tolerances: FP64 bit for bit (same kernels, block shape and summation order);
-MIXED forces and Ur to 1e-11 relative (the mixed kernel adds reactions with LDS
-FP64 atomics whose order follows the wave schedule); RDF counts identical.
+Tolerances: bit for bit in both precisions (same kernels, block shape and
+summation order); RDF counts identical.
 Two calls in a row share the scratch array, the second on a smaller system.
 """
 import ctypes
@@ -67,17 +66,13 @@ def test_noalloc_matches_force_direct(precision, flags):
         f, ur, rdf = noalloc(p, precision, x)
         fc, urc, rdfc = context(p, precision, x)
         assert np.all(np.isfinite(f))
-        if precision == FP64:
-            np.testing.assert_array_equal(f, fc)
-            assert ur == urc
-        else:
-            np.testing.assert_allclose(f, fc, rtol=1e-11, atol=1e-11 * np.abs(fc).max())
-            assert abs(ur - urc) <= 1e-11 * abs(urc)
+        np.testing.assert_array_equal(f, fc)
+        assert ur == urc
         np.testing.assert_array_equal(rdf, rdfc)
     # without an RDF buffer: the same forces
     f2, ur2, _ = noalloc(p, precision, x, with_rdf=False)
-    np.testing.assert_allclose(f2, fc, rtol=1e-11, atol=1e-11 * np.abs(fc).max())
-    assert abs(ur2 - urc) <= 1e-11 * abs(urc)
+    np.testing.assert_array_equal(f2, fc)
+    assert ur2 == urc
 
 
 def test_noalloc_rejects_large_systems_before_device_use():
