@@ -197,8 +197,10 @@ int tcp_force_direct_noalloc(const tcp_params* p, int precision, const double* x
  * per job.  Each replica behaves like its own context (same state layout, same
  * histograms); one call steps all of them with one persistent pair launch per
  * step for all replicas and the step captured in a hipGraph.  Replicas must
- * share recip (TCP_RECIP_NONE or TCP_RECIP_EWALD), kmax and flags.  In FP64
- * every replica reproduces tcp_md_steps on its own context bit for bit. */
+ * share recip (TCP_RECIP_NONE or TCP_RECIP_EWALD), kmax and flags.  Every
+ * replica reproduces tcp_md_steps on its own context bit for bit: in FP64
+ * always, in TCP_MIXED when the batch's shape (tcp_batch_set_shape) is the
+ * context's own, 0 above about 32000 particles and 3 below. */
 typedef struct tcp_batch tcp_batch;
 
 int tcp_batch_create(tcp_batch** out, int nrep, const tcp_params* p /* [nrep] */, int precision);

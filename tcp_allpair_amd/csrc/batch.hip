@@ -7,12 +7,10 @@
 // launch that takes (replica, block pair) items from a device queue, and the
 // whole step is captured once in a hipGraph and replayed.  The kernel bodies
 // are those of the single-context path with the same block decomposition, so
-// in FP64 every replica reproduces its standalone run bit for bit.  In mixed
-// precision the reactions of a block meet in LDS FP64 atomics whose order
-// follows the wave schedule, so a replica agrees with its standalone run to
-// rounding at the standalone run's tile shape (4 x 4 tiles below about 32000
-// particles, 8 x 8 above), and the autotuner may pick another shape for the
-// persistent kernel (a different, equally valid FP32 grouping).
+// every replica reproduces its standalone run bit for bit: always in FP64, and
+// in mixed precision at the standalone run's tile shape (4 x 4 tiles below
+// about 32000 particles, 8 x 8 above).  The autotuner may pick another shape
+// for the persistent mixed kernel (a different, equally valid FP32 grouping).
 #include <algorithm>
 #include <cmath>
 #include <string>
