@@ -699,6 +699,7 @@ struct MixUnits {
     float sx, sy, sz;        // fixed-point difference to u per axis (u = 1 only)
     float inv_u;             // FP32 residuals (physical) to u
     float rc2, rfix2;        // u^2
+    float rnear2;            // u^2: max(rfix2, rclose^2)
     float half_alpha, two_a_sqrtpi, inv_rs, nl2rs;  // x u
     float inv_half_alpha, two_rs;  // 2/alpha, 2 lambda: / u
     float na2l;              // x u^2
@@ -727,6 +728,7 @@ __device__ inline MixUnits mix_units(const PairArgs& a, bool cubic)
     w.na2l = (float)(-a.alpha * a.alpha * 1.4426950408889634 * u * u);
     const bool kelbg = (a.flags & TCP_KELBG) != 0;
     w.rclose = (float)((kelbg ? 1.0 : 2.0) * a.rs / u);
+    w.rnear2 = fmaxf(w.rfix2, w.rclose * w.rclose);
     w.idx_ee = (float)(a.inv_dx[0] * u);
     w.idx_ei = (float)(a.inv_dx[1] * u);
     w.idx_ii = (float)(a.inv_dx[2] * u);
@@ -758,22 +760,12 @@ struct MixUni {
     bool in;
 };
 
-template <bool kKelbg, bool kCubic, bool kReg>
-__device__ inline MixUni mix_uni(const MixUnits& w, const FixPos& pi, FixU3 qu, const FixPos& q)
+// The terms of one pair from its separation o.dx, o.dy, o.dz and r2 (unit u):
+// o.r, o.in, o.er and o.fs.  kClose: a close electron-ion pair (r < rclose)
+// takes num and en from the cancellation-free series.
+template <bool kKelbg, bool kReg, bool kClose>
+__device__ inline void mix_terms(const MixUnits& w, float r2, MixUni& o)
 {
-    MixUni o;
-    const float ix = (float)(int)(pi.ux - qu.x), iy = (float)(int)(pi.uy - qu.y),
-                iz = (float)(int)(pi.uz - qu.z);
-    o.dx = kCubic ? ix : ix * w.sx;
-    o.dy = kCubic ? iy : iy * w.sy;
-    o.dz = kCubic ? iz : iz * w.sz;
-    float r2 = fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz));
-    if (r2 < w.rfix2) {  // close pair (rare): add the FP32 residuals
-        o.dx = kCubic ? fmaf(pi.rx - q.rx, w.inv_u, ix) : fmaf(ix, w.sx, pi.rx - q.rx);
-        o.dy = kCubic ? fmaf(pi.ry - q.ry, w.inv_u, iy) : fmaf(iy, w.sy, pi.ry - q.ry);
-        o.dz = kCubic ? fmaf(pi.rz - q.rz, w.inv_u, iz) : fmaf(iz, w.sz, pi.rz - q.rz);
-        r2 = fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz));
-    }
     o.in = r2 < w.rc2;
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
@@ -791,7 +783,7 @@ __device__ inline MixUni mix_uni(const MixUnits& w, const FixPos& pi, FixU3 qu, 
             const float ecs = y2 * erfcx_poly(__builtin_amdgcn_rcpf(fmaf(rinv, w.two_rs, 1.0f)));
             er = fmaf(kSqrtPiF * sr, ecs, ec - y2) * rm;
             inner = fmaf(-kSqrtPiF * w.inv_rs, ecs, fmaf(w.two_a_sqrtpi, e2, er));
-            if (o.in && r < w.rclose) {
+            if (kClose && o.in && r < w.rclose) {
                 float num, en;
                 close_kelbg(sr, 2.0f * w.half_alpha * r, ecs, w.two_a_sqrtpi * r * e2, ec, &num, &en);
                 er = en * rinv;
@@ -801,7 +793,7 @@ __device__ inline MixUni mix_uni(const MixUnits& w, const FixPos& pi, FixU3 qu, 
             const float y = __builtin_amdgcn_exp2f(w.nl2rs * r);
             er = (ec - y) * rm;
             inner = fmaf(-w.inv_rs, y, fmaf(w.two_a_sqrtpi, e2, er));
-            if (o.in && r < w.rclose) {
+            if (kClose && o.in && r < w.rclose) {
                 float num, en;
                 close_ei(r, sr, 2.0f * w.half_alpha * r, y, ec, w.two_a_sqrtpi * r * e2, &num, &en);
                 er = en * rinv;
@@ -815,6 +807,29 @@ __device__ inline MixUni mix_uni(const MixUnits& w, const FixPos& pi, FixU3 qu, 
     o.r = r;
     o.er = er;
     o.fs = t * inner;
+}
+
+template <bool kKelbg, bool kCubic, bool kReg>
+__device__ inline MixUni mix_uni(const MixUnits& w, const FixPos& pi, FixU3 qu, const FixPos& q)
+{
+    MixUni o;
+    const float ix = (float)(int)(pi.ux - qu.x), iy = (float)(int)(pi.uy - qu.y),
+                iz = (float)(int)(pi.uz - qu.z);
+    o.dx = kCubic ? ix : ix * w.sx;
+    o.dy = kCubic ? iy : iy * w.sy;
+    o.dz = kCubic ? iz : iz * w.sz;
+    const float r2 = fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz));
+    mix_terms<kKelbg, kReg, false>(w, r2, o);
+    // A close pair (rare) is evaluated again: with the FP32 residuals below
+    // L 2^-8, where the 32-bit quantisation would exceed FP32 rounding, and,
+    // for a regularised pair below rclose, with the series.  One test covers
+    // both, so the common path carries a single compare for them.
+    if (r2 < (kReg ? w.rnear2 : w.rfix2)) {
+        o.dx = kCubic ? fmaf(pi.rx - q.rx, w.inv_u, ix) : fmaf(ix, w.sx, pi.rx - q.rx);
+        o.dy = kCubic ? fmaf(pi.ry - q.ry, w.inv_u, iy) : fmaf(iy, w.sy, pi.ry - q.ry);
+        o.dz = kCubic ? fmaf(pi.rz - q.rz, w.inv_u, iz) : fmaf(iz, w.sz, pi.rz - q.rz);
+        mix_terms<kKelbg, kReg, kReg>(w, fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz)), o);
+    }
     return o;
 }
 
