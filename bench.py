@@ -161,9 +161,12 @@ def cpu_baseline(p, x):
             pairs, sec, _ = np.fromfile(fout, dtype="<f8", count=3)
             rates.append(pairs / sec)
             secs.append(sec)
-    return dict(value=float(sum(rates)), unit="pair-interactions/s", cores=cores, kind="reference",
+    rate = float(sum(rates))
+    return dict(value=rate, unit="pair-interactions/s", cores=cores, kind="reference",
                 host_cores=host_cores, per_process_rate=float(np.mean(rates)),
                 seconds_per_process=float(np.mean(secs)),
+                # one step's N(N-1)/2 pair interactions at this rate
+                pair_seconds_per_step=p.n * (p.n - 1) / 2 / rate,
                 sample=f"Force_direct_ext (force.f90:183) of a {n_own}-particle row slice against "
                        f"all {p.n} particles per process ({n_own * p.n} ordered pairs), {cores} "
                        "single-threaded processes in parallel (at most 16: the host-core share of "

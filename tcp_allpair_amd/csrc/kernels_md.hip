@@ -15,7 +15,7 @@ namespace tcp {
 namespace {
 
 // Block sum of W values per thread into part[blockIdx.x * W + c], fixed order.
-template <int W>
+template <int W, int S = W>  // S: the record length of part (W <= S)
 __device__ inline void block_sum_store(double (&v)[W], double* part, int blk)
 {
     __shared__ double red[W][kRedBlock];
@@ -31,7 +31,7 @@ __device__ inline void block_sum_store(double (&v)[W], double* part, int blk)
     }
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int c = 0; c < W; ++c) part[blk * W + c] = red[c][0];
+        for (int c = 0; c < W; ++c) part[blk * S + c] = red[c][0];
     }
 }
 
@@ -158,6 +158,26 @@ __global__ __launch_bounds__(kRedBlock) void combine_kernel(MdArgs m, const doub
                                                             double* part, int accumulate)
 {
     combine_body(m, blockIdx.x, pf, pu, nchunk, pu_row, frx, fry, frz, part, accumulate);
+}
+
+// f += fr (SPME_SUM) after a combine_kernel call that left fr out, and the
+// force entries of its block partials (part[b * 4 + 0..2]; entry 3, sum u,
+// stays): the same values and partials as a combine_kernel call given fr.
+__global__ __launch_bounds__(kRedBlock) void add_recip_kernel(MdArgs m, const double* frx,
+                                                              const double* fry,
+                                                              const double* frz, double* part)
+{
+    const int i = blockIdx.x * kRedBlock + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (i < m.n) {
+        v[0] = m.fx[i] + frx[i];
+        v[1] = m.fy[i] + fry[i];
+        v[2] = m.fz[i] + frz[i];
+        m.fx[i] = v[0];
+        m.fy[i] = v[1];
+        m.fz[i] = v[2];
+    }
+    block_sum_store<3, 4>(v, part, blockIdx.x);
 }
 
 // sums[c] = sum over blocks of part[b * width + c], in block order (one block).
@@ -538,6 +558,13 @@ void launch_combine(const MdArgs& m, const double* pf, const double* pu, int nch
 {
     hipLaunchKernelGGL(combine_kernel, dim3(md_blocks(m.n)), dim3(kRedBlock), 0, s, m, pf, pu,
                        nchunk, pu_row, frx, fry, frz, part, (int)accumulate);
+}
+
+void launch_add_recip(const MdArgs& m, const double* frx, const double* fry, const double* frz,
+                      double* part, hipStream_t s)
+{
+    hipLaunchKernelGGL(add_recip_kernel, dim3(md_blocks(m.n)), dim3(kRedBlock), 0, s, m, frx, fry,
+                       frz, part);
 }
 
 void launch_reduce_parts(const double* part, int nblk, int width, double* sums, hipStream_t s)

@@ -50,6 +50,10 @@ struct tcp_ctx {
     int device = 0;
     double alpha = 0, rc2 = 0, Uo = 0;
     hipStream_t stream = nullptr;
+    // tcp_md_steps runs the reciprocal part on stream2 beside combine_kernel:
+    // forked from stream after the pair kernel, joined before the forces add
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // state
     double *x = nullptr, *y = nullptr, *z = nullptr;
     double *vx = nullptr, *vy = nullptr, *vz = nullptr;

@@ -244,8 +244,15 @@ void launch_pair(tcp_ctx* c, const PairArgs& a, int nchunk)
 }
 
 // Real-space all-pair forces of the whole system into f, block partials into part.
+// Reciprocal forces into frx/fry/frz and Um into sc[SC_UM] on stream s (null:
+// the context's stream); with qp the per-particle charges replace the species
+// charges, pot receives dUm/dq.
+int recip(tcp_ctx* c, const double* qp = nullptr, double* pot = nullptr, hipStream_t s = nullptr);
+
+// with_recip: the reciprocal part (recip) runs on stream2 beside the force sum
+// and its forces join after it (SPME_SUM), with the same result as frx etc.
 int direct_all(tcp_ctx* c, int rdf_weight, const double* frx, const double* fry, const double* frz,
-               int ev_slot)
+               int ev_slot, bool with_recip = false)
 {
     PairArgs a = pair_args(c);
     a.pi = PosSoA{c->x, c->y, c->z};
@@ -267,36 +274,47 @@ int direct_all(tcp_ctx* c, int rdf_weight, const double* frx, const double* fry,
                            c->stream);
     LAUNCHCK(c);
     if (ev_slot >= 0) HIPCK(c, hipEventRecord(c->ev_pair[1][ev_slot], c->stream));
+    if (with_recip) {  // positions are final: the reciprocal part runs beside the sum
+        HIPCK(c, hipEventRecord(c->ev_fork, c->stream));
+        HIPCK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+        TRY(recip(c, nullptr, nullptr, c->stream2));
+        HIPCK(c, hipEventRecord(c->ev_join, c->stream2));
+    }
     // the Newton kernels give each unordered pair's energy once: Ur = sum u
     c->ur_scale = 1.0;
-    launch_combine(md_args(c), c->pf, c->pu, c->n3_rows, n3_shape(c->n3_shape).tiles * 64, frx, fry,
-                   frz, c->part, false, c->stream);
+    launch_combine(md_args(c), c->pf, c->pu, c->n3_rows, n3_shape(c->n3_shape).tiles * 64,
+                   with_recip ? nullptr : frx, with_recip ? nullptr : fry,
+                   with_recip ? nullptr : frz, c->part, false, c->stream);
     LAUNCHCK(c);
+    if (with_recip) {
+        HIPCK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        launch_add_recip(md_args(c), c->frx, c->fry, c->frz, c->part, c->stream);
+        LAUNCHCK(c);
+    }
     launch_reduce_parts(c->part, md_blocks(c->N), 4, c->sc + SC_FSUM, c->stream);
     LAUNCHCK(c);
     return 0;
 }
 
-// Reciprocal forces into frx/fry/frz and Um into sc[SC_UM]; with qp the
-// per-particle charges replace the species charges, pot receives dUm/dq.
-int recip(tcp_ctx* c, const double* qp = nullptr, double* pot = nullptr)
+int recip(tcp_ctx* c, const double* qp, double* pot, hipStream_t s)
 {
+    if (!s) s = c->stream;
     RecipArgs a = recip_args(c);
     a.qp = qp;
     a.pot = pot;
     const double V = c->p.box[0] * c->p.box[1] * c->p.box[2];
     if (c->p.recip == TCP_RECIP_SPME) {
-        launch_spme_spread(a, c->stream);
+        launch_spme_spread(a, s);
         LAUNCHCK(c);
-        launch_spme_convolve(a, c->stream);
+        launch_spme_convolve(a, s);
         LAUNCHCK(c);
-        launch_spme_gather(a, c->stream);
+        launch_spme_gather(a, s);
         LAUNCHCK(c);
-        launch_um(c->epart, c->nepart, c->p.eps / (2.0 * kPi * V), c->sc + SC_UM, c->stream);
+        launch_um(c->epart, c->nepart, c->p.eps / (2.0 * kPi * V), c->sc + SC_UM, s);
     } else {
-        launch_ewald_kspace(a, c->p.kmax, c->alpha, c->sk, c->stream);
+        launch_ewald_kspace(a, c->p.kmax, c->alpha, c->sk, s);
         LAUNCHCK(c);
-        launch_um(c->epart, 1, c->p.eps / (2.0 * kPi * V), c->sc + SC_UM, c->stream);
+        launch_um(c->epart, 1, c->p.eps / (2.0 * kPi * V), c->sc + SC_UM, s);
     }
     LAUNCHCK(c);
     return 0;
@@ -635,8 +653,12 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
         return rc;
     };
     if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(c, TCP_ERR_HIP, "no HIP device"));
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(c, TCP_ERR_HIP, "hipStreamCreate failed"));
+    if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(c, TCP_ERR_HIP, "hipEventCreate failed"));
     for (int e = 0; e < kEvPool; ++e)
         for (int k = 0; k < 2; ++k)
             if (hipEventCreate(&c->ev_pair[k][e]) != hipSuccess)
@@ -717,7 +739,10 @@ int tcp_destroy(tcp_ctx* c)
         (void)hipEventDestroy(c->ev_all[1]);
         for (auto e : c->ev_stage) (void)hipEventDestroy(e);
     }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     delete c;
     return 0;
 }
@@ -985,10 +1010,9 @@ int tcp_md_steps(tcp_ctx* c, int nsteps, int thermo, double dt, double Te, doubl
     for (int s = 0; s < nsteps; ++s) {
         launch_vv1(m, thermo, dt, fric, c->stream);  // main.f90:87-95
         LAUNCHCK(c);
-        if (rec) TRY(recip(c));                      // main.f90:100
         const int slot = s >= first_timed ? s - first_timed : -1;
-        TRY(direct_all(c, 1, rec ? c->frx : nullptr, rec ? c->fry : nullptr,
-                       rec ? c->frz : nullptr, slot));  // main.f90:106, 136
+        // main.f90:100 (the reciprocal part, beside the force sum), 106, 136
+        TRY(direct_all(c, 1, nullptr, nullptr, nullptr, slot, rec));
         TRY(vv2(c, thermo, dt, Te, Ti, tau, fric, seed, c->sc + SC_FSUM));  // main.f90:141-153
         if (trace) {
             launch_trace(c->trace + 6 * (size_t)s, c->sc + SC_USUM, c->ur_scale, c->Uo,

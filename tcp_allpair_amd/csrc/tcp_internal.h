@@ -167,6 +167,10 @@ void launch_make_upos(const MdArgs& m, hipStream_t s);
 void launch_combine(const MdArgs& m, const double* pf, const double* pu, int nchunk, int pu_row,
                     const double* frx, const double* fry, const double* frz, double* part,
                     bool accumulate, hipStream_t s);
+// f += fr and the force entries of combine_kernel's block partials, for a
+// combine_kernel call that left fr out (fr computed beside it)
+void launch_add_recip(const MdArgs& m, const double* frx, const double* fry, const double* frz,
+                      double* part, hipStream_t s);
 // sums[c] = sum_b part[b * width + c] in block order (single block)
 void launch_reduce_parts(const double* part, int nblk, int width, double* sums, hipStream_t s);
 void launch_add_forces(const MdArgs& m, const double* frx, const double* fry, const double* frz,
