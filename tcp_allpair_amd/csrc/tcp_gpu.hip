@@ -993,6 +993,9 @@ int tcp_get_histograms(tcp_ctx* c, double* rdf, double* vdf, int reset)
     return 0;
 }
 
+// systems from this size on run the reciprocal part beside the force sum
+constexpr int kRecipBesideMinN = 65536;
+
 int tcp_md_steps(tcp_ctx* c, int nsteps, int thermo, double dt, double Te, double Ti, double tau,
                  double fric, uint64_t seed, double* trace)
 {
@@ -1011,8 +1014,14 @@ int tcp_md_steps(tcp_ctx* c, int nsteps, int thermo, double dt, double Te, doubl
         launch_vv1(m, thermo, dt, fric, c->stream);  // main.f90:87-95
         LAUNCHCK(c);
         const int slot = s >= first_timed ? s - first_timed : -1;
-        // main.f90:100 (the reciprocal part, beside the force sum), 106, 136
-        TRY(direct_all(c, 1, nullptr, nullptr, nullptr, slot, rec));
+        // main.f90:100 (the reciprocal part), 106, 136.  Large systems run the
+        // reciprocal part beside the force sum; below kRecipBesideMinN the
+        // sum is too short to hide it behind and the stream fork and join
+        // would cost more than they save.
+        const bool beside = rec && c->N >= kRecipBesideMinN;
+        if (rec && !beside) TRY(recip(c));
+        TRY(direct_all(c, 1, rec && !beside ? c->frx : nullptr, rec && !beside ? c->fry : nullptr,
+                       rec && !beside ? c->frz : nullptr, slot, beside));
         TRY(vv2(c, thermo, dt, Te, Ti, tau, fric, seed, c->sc + SC_FSUM));  // main.f90:141-153
         if (trace) {
             launch_trace(c->trace + 6 * (size_t)s, c->sc + SC_USUM, c->ur_scale, c->Uo,

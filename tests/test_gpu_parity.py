@@ -433,6 +433,37 @@ def test_mixed_vs_fp64_box_shapes(aniso, flags):
     assert abs(h32.sum() - h64.sum()) <= 1e-4 * h64.sum()
 
 
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_md_step_recip_beside_matches_subroutines(precision):
+    """65536 particles (32768 + 32768, SPME 64^3 order 4): from this size on
+    tcp_md_steps runs the reciprocal part on a second stream beside the force
+    sum.  One step equals the drop-in's subroutine sequence run one call after
+    another (VVerletNotemp1, Force_direct_self, SPME_SUM, REMOVE_RIGIDMTN,
+    VVerletNotemp2, main.f90:87-153): positions, Ur and Um bit for bit, forces
+    and velocities to the rounding of the mean-force sum, which the two forms
+    add in different orders (1e-13)."""
+    say("case recip beside", precision)
+    p, x, v = make_plasma(32768, 32768, seed=11, K=(64, 64, 64), order=4)
+    dt = reduced_dt(0.005)
+    f0 = np.zeros_like(x)
+    with Context(p, precision) as c:
+        c.set_state(x=x, v=v, f=f0)
+        tr = c.md_steps(1, NOTEMP, dt)
+        xa, va, fa = c.get_state()
+    with Context(p, precision) as c:
+        c.set_state(x=x, v=v, f=f0)
+        c.vverlet1(NOTEMP, dt)
+        ur = c.force_direct()
+        um = c.force_recip()
+        c.remove_rigidmtn()
+        c.vverlet2(NOTEMP, dt)
+        xb, vb, fb = c.get_state()
+    np.testing.assert_array_equal(xa, xb)
+    assert tr[0, 0] == ur and tr[0, 2] == um
+    np.testing.assert_allclose(fa, fb, rtol=0, atol=1e-13 * np.abs(fb).max())
+    np.testing.assert_allclose(va, vb, rtol=0, atol=1e-13 * np.abs(vb).max())
+
+
 def test_fullsize_mixed_step_properties():
     """131072 particles, SPME 128^3 order 6, mixed: after a step the mean force
     is removed exactly and the energies are finite; two calls of one step each
