@@ -399,6 +399,21 @@ def test_deterministic(precision, recip, order, ne):
     np.testing.assert_array_equal(outs[0][2], outs[1][2])
 
 
+@pytest.mark.parametrize("ne,ni,flags", [(1, 1, 0), (37, 100, KELBG), (333, 257, 0),
+                                         (5000, 3001, REG_EE), (21000, 19001, KELBG)])
+def test_mixed_deterministic_ragged(ne, ni, flags):
+    """Mixed forces, Ur and RDF repeat bit for bit on systems whose species
+    boundary and end fall inside tiles (the tiles there take the per-lane
+    path) and whose last block column is partial, on both block shapes."""
+    say("case mixed deterministic ragged", ne, ni, flags)
+    p, x, _ = make_plasma(ne, ni, seed=17, recip=RECIP_NONE)
+    p = p.replace(flags=flags)
+    outs = [direct(p, x, MIXED) for _ in range(2)]
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
+    np.testing.assert_array_equal(outs[0][2], outs[1][2])
+
+
 def test_mixed_vs_fp64_16k():
     """16384 + 16384 particles (the 64^3 SPME workload), mixed against FP64 pair forces."""
     say("case mixed 32k")
