@@ -931,7 +931,7 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
                                 N3Stage<kBT, kW>& stg)
 {
     constexpr int kBP = kBT * kTile;  // particles per block side
-    static_assert((kBT & (kBT - 1)) == 0 && kBT % kW == 0, "kBT: a power of two, kW divides it");
+    static_assert(kBT % kW == 0, "the waves share the i tiles of a block equally");
     const bool dblk = br == bc;
     const int n = a.n_i, ne = a.ne_i;
     for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
