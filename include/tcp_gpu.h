@@ -13,7 +13,9 @@
  * return, and the next call that returns host data waits for them (a fault of
  * theirs is reported there).  A context owns all device memory for its
  * lifetime, is used by one host thread at a time and runs on its own HIP
- * stream.
+ * streams (a second one, forked from and joined to the first within one
+ * tcp_md_steps step, carries the reciprocal part of systems of 65536 and
+ * more particles beside the force sum).
  *
  * Host arrays are AoS double[3*N] (x,y,z per particle) with the electrons first
  * and the ions after them, the order of qel(:) followed by qion(:)
