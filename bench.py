@@ -55,6 +55,7 @@ from tcp_allpair_amd.params import FP64, MIXED, NOTEMP, reduced_dt  # noqa: E402
 # Newton's third law (the reaction accumulate), which the mixed kernel does.
 FLOP_PER_UNORDERED_PAIR_NEWTON = 62.5 + 6.0
 FP32_VECTOR_PEAK_TFLOPS = 157.3
+FP32_NONPACKED_PEAK_TFLOPS = 78.6  # one FP32 FMA per lane per cycle, no packed math
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pair_mixed_n131072_pmc.md")
 
@@ -295,7 +296,11 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
                      "traffic": profiled_traffic("pair_mixed_n3_kernel<true, false>"),
                      "kernel": "pair_mixed_n3_kernel<true, false>",
-                     "kernel_ms": mx["pair_ms"], "flop_per_launch": flops},
+                     "kernel_ms": mx["pair_ms"], "flop_per_launch": flops,
+                     # packed FP32 (v_pk_*_f32) is disabled for all device code, so
+                     # the kernel issues non-packed FP32, whose ceiling is half the peak
+                     "peak_nonpacked": FP32_NONPACKED_PEAK_TFLOPS,
+                     "frac_nonpacked": achieved / FP32_NONPACKED_PEAK_TFLOPS},
         "roofline_fp64": {"bound": "valu", "achieved": achieved64, "peak": FP64_VECTOR_PEAK_TFLOPS,
                           "unit": "TFLOP/s", "frac": achieved64 / FP64_VECTOR_PEAK_TFLOPS,
                           "traffic": profiled_traffic("pair_f64_n3_kernel<true, false>"),
