@@ -199,7 +199,9 @@ int tcp_force_direct_noalloc(const tcp_params* p, int precision, const double* x
  * per job.  Each replica behaves like its own context (same state layout, same
  * histograms); one call steps all of them with one persistent pair launch per
  * step for all replicas and the step captured in a hipGraph.  Replicas must
- * share recip (TCP_RECIP_NONE or TCP_RECIP_EWALD), kmax and flags.  Every
+ * share recip and flags, and kmax (TCP_RECIP_EWALD) or the spline order
+ * (TCP_RECIP_SPME: grids may differ, every K[d] a multiple of 8, else
+ * TCP_ERR_UNSUPPORTED).  Every
  * replica reproduces tcp_md_steps on its own context bit for bit: in FP64
  * always, in TCP_MIXED when the batch's shape (tcp_batch_set_shape) is the
  * context's own, 0 above about 32000 particles and 3 below. */

@@ -39,6 +39,7 @@ struct tcp_batch {
     int maxn = 2;             // particles of the largest replica
     int recip = TCP_RECIP_NONE;
     int kmax = 0;
+    SpmeBatchPlan spme;       // TCP_RECIP_SPME: launch sizes and the FFT pass tables
     bool kelbg = false;
     // device tables
     std::vector<BatchRep> h_reps;
@@ -239,6 +240,7 @@ void launch_step(tcp_batch* b, int thermo, double dt, double tau, double fric, u
 {
     launch_vv1_batch(b->d_reps, b->nrep, b->maxblk, thermo, dt, fric, b->stream);
     if (b->recip == TCP_RECIP_EWALD) launch_ewald_batch(b->d_reps, b->nrep, b->kmax, b->maxn, b->stream);
+    if (b->recip == TCP_RECIP_SPME) launch_spme_batch(b->d_reps, b->spme, b->stream);
     launch_pair(b);
     launch_combine_batch(b->d_reps, b->nrep, b->maxblk, b->stream);
     launch_reduce_parts_batch(b->d_reps, b->nrep, 4, SC_FSUM, b->stream);
@@ -298,6 +300,7 @@ int tcp_batch_destroy(tcp_batch* b)
     for (auto e : b->ev)
         if (e) (void)hipEventDestroy(e);
     for (void* a : b->allocs) (void)hipFree(a);
+    spme_batch_release(&b->spme);
     for (tcp_ctx* c : b->reps) tcp_destroy(c);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->stream2) (void)hipStreamDestroy(b->stream2);
@@ -312,10 +315,17 @@ int tcp_batch_create(tcp_batch** out, int nrep, const tcp_params* p, int precisi
     *out = nullptr;
     for (int r = 1; r < nrep; ++r)
         if (p[r].recip != p[0].recip || p[r].flags != p[0].flags ||
-            (p[0].recip == TCP_RECIP_EWALD && p[r].kmax != p[0].kmax))
-            return bfail(nullptr, TCP_ERR_ARG, "replicas must share recip, kmax and flags");
-    if (p[0].recip == TCP_RECIP_SPME)
-        return bfail(nullptr, TCP_ERR_UNSUPPORTED, "batches support TCP_RECIP_NONE and TCP_RECIP_EWALD");
+            (p[0].recip == TCP_RECIP_EWALD && p[r].kmax != p[0].kmax) ||
+            (p[0].recip == TCP_RECIP_SPME && p[r].order != p[0].order))
+            return bfail(nullptr, TCP_ERR_ARG, "replicas must share recip, flags, kmax (k-sum) and order (SPME)");
+    // batched SPME takes brick-binned grids only: the unbinned spreading adds
+    // with FP64 atomics, whose order would break run-to-run repeatability
+    for (int r = 0; r < nrep && p[0].recip == TCP_RECIP_SPME; ++r)
+        for (int d = 0; d < 3; ++d)
+            if (p[r].K[d] % kBrick != 0)
+                return bfail(nullptr, TCP_ERR_UNSUPPORTED,
+                             "replica " + std::to_string(r) + ": batches run SPME on grids with every K[d] a multiple of " +
+                                 std::to_string(kBrick));
     for (int r = 0; r < nrep; ++r) {
         std::string msg;
         const int rc = validate_params(&p[r], precision, &msg);
@@ -394,6 +404,12 @@ int tcp_batch_create(tcp_batch** out, int nrep, const tcp_params* p, int precisi
         h.eps = c->p.eps;
         const double V = c->p.box[0] * c->p.box[1] * c->p.box[2];
         h.um_scale = c->p.eps / (2.0 * kPi * V);
+    }
+    if (b->recip == TCP_RECIP_SPME) {
+        std::vector<RecipArgs> ra(nrep);
+        for (int r = 0; r < nrep; ++r) ra[r] = b->h_reps[r].ra;
+        if ((rc = spme_batch_plan(ra.data(), nrep, &b->spme)))
+            return bail(bfail(b, rc, "SPME pass tables: hipMalloc failed"));
     }
     if ((rc = set_shape(b, 0))) return bail(rc);
     *out = b;

@@ -486,7 +486,8 @@ __global__ __launch_bounds__(kRedBlock) void um_batch_kernel(const BatchRep* rep
 {
     const BatchRep& r = reps[blockIdx.x];
     if (r.recip == TCP_RECIP_NONE) return;
-    um_body(r.ra.epart, 1, r.um_scale, r.sc + SC_UM);
+    // the k-sum leaves one energy partial, SPME one per z-pass block
+    um_body(r.ra.epart, r.recip == TCP_RECIP_SPME ? r.ra.nepart : 1, r.um_scale, r.sc + SC_UM);
 }
 
 // counter[0]: step counter of the Gaussian noise, counter[1]: trace row

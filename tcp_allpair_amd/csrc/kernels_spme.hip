@@ -24,6 +24,9 @@
 //            derivatives (force.f90:868-919).
 // With C the convolution of Q with the real, even kernel IDFT(gb):
 //   Um = eps/(2 pi V) sum_m gb(m) |S(m)|^2,  F_i = -eps/(pi V) sum dQ/dx_i C.
+#include <algorithm>
+#include <vector>
+
 #include "tcp_internal.h"
 
 namespace tcp {
@@ -193,11 +196,10 @@ __global__ __launch_bounds__(256) void gather_kernel(RecipArgs a)
 // binned order of the spreading) reads its support from there at offsets
 // inside the tile.
 template <int P>
-__global__ __launch_bounds__(64) void gather_brick_kernel(RecipArgs a)
+__device__ inline void gather_brick_body(const RecipArgs& a, int bid)
 {
     constexpr int E = kBrick + P - 1;
     __shared__ double tile[E * E * E];
-    const int bid = blockIdx.x;
     const int p0 = a.brick_start[bid], p1 = a.brick_start[bid + 1];
     if (p0 == p1) return;
     const int borg[3] = {(bid % a.nbrick[0]) * kBrick, ((bid / a.nbrick[0]) % a.nbrick[1]) * kBrick,
@@ -240,6 +242,12 @@ __global__ __launch_bounds__(64) void gather_brick_kernel(RecipArgs a)
     }
 }
 
+template <int P>
+__global__ __launch_bounds__(64) void gather_brick_kernel(RecipArgs a)
+{
+    gather_brick_body<P>(a, blockIdx.x);
+}
+
 // ------------------------------------------------- brick-binned spreading ---
 //
 // Device-scope FP64 atomics resolve beyond the per-XCD L2, so spreading p^3
@@ -263,9 +271,8 @@ __device__ inline int brick_axis(double u, int P, int K)
 }
 
 template <int P>
-__global__ __launch_bounds__(256) void bin_count_kernel(RecipArgs a)
+__device__ inline void bin_count_body(const RecipArgs& a, int m)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= a.n) return;
     const double pos[3] = {a.x[m], a.y[m], a.z[m]};
     int b[3];
@@ -279,8 +286,14 @@ __global__ __launch_bounds__(256) void bin_count_kernel(RecipArgs a)
     atomicAdd(a.brick_count + id, 1);
 }
 
+template <int P>
+__global__ __launch_bounds__(256) void bin_count_kernel(RecipArgs a)
+{
+    bin_count_body<P>(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // exclusive scan of the brick counts (one block), cursors reset
-__global__ __launch_bounds__(1024) void bin_scan_kernel(RecipArgs a, int nb)
+__device__ inline void bin_scan_body(const RecipArgs& a, int nb)
 {
     __shared__ int part[1024];
     const int per = (nb + 1023) / 1024;
@@ -304,16 +317,25 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(RecipArgs a, int nb)
     if (threadIdx.x == 1023) a.brick_start[nb] = part[1023];
 }
 
-__global__ __launch_bounds__(256) void bin_scatter_kernel(RecipArgs a)
+__global__ __launch_bounds__(1024) void bin_scan_kernel(RecipArgs a, int nb)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    bin_scan_body(a, nb);
+}
+
+__device__ inline void bin_scatter_body(const RecipArgs& a, int m)
+{
     if (m >= a.n) return;
     const int id = a.brick_of[m];
     a.sorted[a.brick_start[id] + atomicAdd(a.brick_cursor + id, 1)] = m;
 }
 
+__global__ __launch_bounds__(256) void bin_scatter_kernel(RecipArgs a)
+{
+    bin_scatter_body(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 template <int P>
-__global__ __launch_bounds__(256) void spread_brick_kernel(RecipArgs a)
+__device__ inline void spread_brick_body(const RecipArgs& a, int bid)
 {
     constexpr int E = kBrick + P - 1;      // tile edge
     constexpr int kBatch = 64;             // particles staged per pass
@@ -321,7 +343,6 @@ __global__ __launch_bounds__(256) void spread_brick_kernel(RecipArgs a)
     __shared__ double ws[kBatch][3][P];
     __shared__ int org[kBatch][3];
     __shared__ double qred[256];
-    const int bid = blockIdx.x;
     const int bx = bid % a.nbrick[0], by = (bid / a.nbrick[0]) % a.nbrick[1],
               bz = bid / (a.nbrick[0] * a.nbrick[1]);
     for (int t = threadIdx.x; t < E * E * E; t += blockDim.x) tile[t] = 0ull;
@@ -375,13 +396,18 @@ __global__ __launch_bounds__(256) void spread_brick_kernel(RecipArgs a)
         out[t] = ldexp((double)(long long)tile[t], -k);
 }
 
+template <int P>
+__global__ __launch_bounds__(256) void spread_brick_kernel(RecipArgs a)
+{
+    spread_brick_body<P>(a, blockIdx.x);
+}
+
 // Q(g) = sum over the tiles covering g, in a fixed order (x fastest grid index)
 template <int P>
-__global__ __launch_bounds__(256) void grid_from_tiles_kernel(RecipArgs a)
+__device__ inline void grid_from_tiles_body(const RecipArgs& a, size_t g)
 {
     constexpr int E = kBrick + P - 1;
     const size_t G = (size_t)a.K[0] * a.K[1] * a.K[2];
-    const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (g >= G) return;
     const int gx = (int)(g % a.K[0]), gy = (int)((g / a.K[0]) % a.K[1]),
               gz = (int)(g / ((size_t)a.K[0] * a.K[1]));
@@ -407,6 +433,12 @@ __global__ __launch_bounds__(256) void grid_from_tiles_kernel(RecipArgs a)
                 s += a.tiles[(size_t)bid * (E * E * E) + (ls[2][cz] * E + ls[1][cy]) * E + ls[0][cx]];
             }
     a.Q[g] = s;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void grid_from_tiles_kernel(RecipArgs a)
+{
+    grid_from_tiles_body<P>(a, blockIdx.x * (size_t)blockDim.x + threadIdx.x);
 }
 
 // -------------------------------------------------------------------- FFT ---
@@ -520,12 +552,12 @@ __device__ inline void fft_stages(double2 (*buf)[kFftTile], int& src, int K, int
 }
 
 template <int kMode>
-__global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
+__device__ inline void fft_pass_body(const FftPass& f, int blk)
 {
     __shared__ double2 buf[2][kFftTile];
     __shared__ double2 tws[kFftTile / 2];
     const int K = f.K, B = f.lines_per_block, half = K >> 1;
-    const int l0 = blockIdx.x * B;
+    const int l0 = blk * B;
     const int npts = B * K;
     // x lines: the block's lines are one contiguous run (of H or K1 points each)
     constexpr bool kX = kMode == kFftFwdX || kMode == kFftInvX;
@@ -586,7 +618,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
             if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0) f.epart[blockIdx.x] = red[0];
+        if (threadIdx.x == 0) f.epart[blk] = red[0];
         __syncthreads();
         fft_stages(buf, src, K, f.logK, npts, tws, true);
     }
@@ -602,6 +634,12 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
         if (kMode == kFftInvX) f.out_real[g] = v.x;
         else f.out[g] = v;
     }
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass f)
+{
+    fft_pass_body<kMode>(f, blockIdx.x);
 }
 
 int ilog2(int k)
@@ -1042,6 +1080,101 @@ __global__ __launch_bounds__(kKsumWaves * 64) void ewald_force_batch_kernel(cons
     ewald_force_body<kStaged>(r.ra, r.kmax, r.sk, blockIdx.x, kf_dyn);
 }
 
+// ---------------------------------------- batched SPME (tcp_batch_*) ---
+//
+// The bodies of the binned single-context kernels over a replica table:
+// blockIdx.y selects the replica, blocks past that replica's own count
+// return at once, so each replica's spreading, FFT and gather are its
+// standalone ones, bit for bit.  Every replica has a brick-binned grid and
+// the batch's spline order.
+
+__device__ inline int brick_total(const RecipArgs& a)
+{
+    return a.nbrick[0] * a.nbrick[1] * a.nbrick[2];
+}
+
+__global__ __launch_bounds__(256) void bin_clear_batch_kernel(const BatchRep* reps)
+{
+    const RecipArgs& a = reps[blockIdx.y].ra;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < brick_total(a)) a.brick_count[i] = 0;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void bin_count_batch_kernel(const BatchRep* reps)
+{
+    bin_count_body<P>(reps[blockIdx.y].ra, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(1024) void bin_scan_batch_kernel(const BatchRep* reps)
+{
+    const RecipArgs& a = reps[blockIdx.x].ra;
+    bin_scan_body(a, brick_total(a));
+}
+
+__global__ __launch_bounds__(256) void bin_scatter_batch_kernel(const BatchRep* reps)
+{
+    bin_scatter_body(reps[blockIdx.y].ra, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void spread_brick_batch_kernel(const BatchRep* reps)
+{
+    const RecipArgs& a = reps[blockIdx.y].ra;
+    if ((int)blockIdx.x >= brick_total(a)) return;
+    spread_brick_body<P>(a, blockIdx.x);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void grid_from_tiles_batch_kernel(const BatchRep* reps)
+{
+    grid_from_tiles_body<P>(reps[blockIdx.y].ra, blockIdx.x * (size_t)blockDim.x + threadIdx.x);
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kFftThreads) void fft_pass_batch_kernel(const FftPass* passes)
+{
+    const FftPass& f = passes[blockIdx.y];
+    if ((int)blockIdx.x >= (f.lines + f.lines_per_block - 1) / f.lines_per_block) return;
+    fft_pass_body<kMode>(f, blockIdx.x);
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void gather_brick_batch_kernel(const BatchRep* reps)
+{
+    const RecipArgs& a = reps[blockIdx.y].ra;
+    if ((int)blockIdx.x >= brick_total(a)) return;
+    gather_brick_body<P>(a, blockIdx.x);
+}
+
+// the five passes of launch_spme_convolve, in its order
+constexpr int kPassMode[kSpmeBatchPasses] = {kFftFwdX, kFftY, kFftZConv, kFftY, kFftInvX};
+constexpr int kPassSign[kSpmeBatchPasses] = {-1, -1, -1, +1, +1};
+
+template <int P>
+void spme_batch_order(const BatchRep* reps, const SpmeBatchPlan& pl, hipStream_t s)
+{
+    const int nr = pl.nrep;
+    const dim3 gp((pl.maxn + 255) / 256, nr), b256(256);
+    const FftPass* passes = (const FftPass*)pl.passes;
+    hipLaunchKernelGGL(bin_clear_batch_kernel, dim3((pl.maxnb + 255) / 256, nr), b256, 0, s, reps);
+    hipLaunchKernelGGL(bin_count_batch_kernel<P>, gp, b256, 0, s, reps);
+    hipLaunchKernelGGL(bin_scan_batch_kernel, dim3(nr), dim3(1024), 0, s, reps);
+    hipLaunchKernelGGL(bin_scatter_batch_kernel, gp, b256, 0, s, reps);
+    hipLaunchKernelGGL(spread_brick_batch_kernel<P>, dim3(pl.maxnb, nr), b256, 0, s, reps);
+    hipLaunchKernelGGL(grid_from_tiles_batch_kernel<P>, dim3(pl.maxgblk, nr), b256, 0, s, reps);
+    const dim3 bf(kFftThreads);
+    hipLaunchKernelGGL(fft_pass_batch_kernel<kFftFwdX>, dim3(pl.pass_blocks[0], nr), bf, 0, s, passes);
+    hipLaunchKernelGGL(fft_pass_batch_kernel<kFftY>, dim3(pl.pass_blocks[1], nr), bf, 0, s, passes + nr);
+    hipLaunchKernelGGL(fft_pass_batch_kernel<kFftZConv>, dim3(pl.pass_blocks[2], nr), bf, 0, s,
+                       passes + 2 * nr);
+    hipLaunchKernelGGL(fft_pass_batch_kernel<kFftY>, dim3(pl.pass_blocks[3], nr), bf, 0, s,
+                       passes + 3 * nr);
+    hipLaunchKernelGGL(fft_pass_batch_kernel<kFftInvX>, dim3(pl.pass_blocks[4], nr), bf, 0, s,
+                       passes + 4 * nr);
+    hipLaunchKernelGGL(gather_brick_batch_kernel<P>, dim3(pl.maxnb, nr), dim3(64), 0, s, reps);
+}
+
 }  // namespace
 
 int fft_epart_blocks(const int* K)
@@ -1179,6 +1312,55 @@ void launch_ewald_batch(const BatchRep* reps, int nrep, int kmax, int maxn, hipS
     else
         hipLaunchKernelGGL(ewald_force_batch_kernel<false>, fg, fb, flds, s, reps);
     launch_um_batch(reps, nrep, s);
+}
+
+
+int spme_batch_plan(const RecipArgs* ra, int nrep, SpmeBatchPlan* pl)
+{
+    *pl = SpmeBatchPlan{};
+    pl->nrep = nrep;
+    pl->order = ra[0].order;
+    std::vector<FftPass> tab((size_t)kSpmeBatchPasses * nrep);
+    for (int r = 0; r < nrep; ++r) {
+        const RecipArgs& a = ra[r];
+        const int nb = a.nbrick[0] * a.nbrick[1] * a.nbrick[2];
+        const size_t G = (size_t)a.K[0] * a.K[1] * a.K[2];
+        pl->maxn = std::max(pl->maxn, a.n);
+        pl->maxnb = std::max(pl->maxnb, nb);
+        pl->maxgblk = std::max(pl->maxgblk, (int)((G + 255) / 256));
+        for (int k = 0; k < kSpmeBatchPasses; ++k) {
+            const FftPass f = make_pass(a, kPassMode[k], kPassSign[k]);
+            tab[(size_t)k * nrep + r] = f;
+            pl->pass_blocks[k] =
+                std::max(pl->pass_blocks[k], (f.lines + f.lines_per_block - 1) / f.lines_per_block);
+        }
+    }
+    const size_t bytes = tab.size() * sizeof(FftPass);
+    if (hipMalloc(&pl->passes, bytes) != hipSuccess) {
+        pl->passes = nullptr;
+        return TCP_ERR_HIP;
+    }
+    if (hipMemcpy(pl->passes, tab.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+        return TCP_ERR_HIP;
+    return 0;
+}
+
+void spme_batch_release(SpmeBatchPlan* pl)
+{
+    if (pl->passes) (void)hipFree(pl->passes);
+    pl->passes = nullptr;
+}
+
+void launch_spme_batch(const BatchRep* reps, const SpmeBatchPlan& pl, hipStream_t s)
+{
+    switch (pl.order) {
+        case 4: spme_batch_order<4>(reps, pl, s); break;
+        case 6: spme_batch_order<6>(reps, pl, s); break;
+        case 8: spme_batch_order<8>(reps, pl, s); break;
+        case 10: spme_batch_order<10>(reps, pl, s); break;
+        default: spme_batch_order<12>(reps, pl, s); break;
+    }
+    launch_um_batch(reps, pl.nrep, s);
 }
 
 }  // namespace tcp

@@ -203,13 +203,13 @@ enum {
 struct BatchRep {
     MdArgs m;
     PairArgs pa;               // pair kernel arguments; pf/pu hold `slots` partial slots
-    RecipArgs ra;              // direct k-space sum arguments (recip == TCP_RECIP_EWALD)
+    RecipArgs ra;              // reciprocal-part arguments (k-sum or SPME)
     double* sk;                // k-sum work, 3 (2 kmax + 1)^3
     double* part;              // block partials
     double* sc;                // device scalars SC_*
     int slots;                 // pair partial slots summed by combine
     int pu_row;                // particles per block row of the pair kernel's shape
-    int recip;                 // TCP_RECIP_NONE or TCP_RECIP_EWALD
+    int recip;                 // TCP_RECIP_NONE, _EWALD or _SPME (ra holds its buffers)
     int kmax;
     double alpha;
     double Te, Ti;             // thermostat targets
@@ -236,6 +236,19 @@ void launch_step_advance(uint64_t* counter, hipStream_t s);
 // direct k-space sums of every replica (same kmax): S(m), forces, energy partial;
 // maxn: particles of the largest replica
 void launch_ewald_batch(const BatchRep* reps, int nrep, int kmax, int maxn, hipStream_t s);
+// SPME of every replica (brick-binned grids, one spline order): the launch
+// sizes and a device table of each replica's five FFT passes
+constexpr int kSpmeBatchPasses = 5;
+struct SpmeBatchPlan {
+    void* passes = nullptr;  // device FftPass[kSpmeBatchPasses][nrep]
+    int nrep = 0, order = 0, maxn = 0, maxnb = 0, maxgblk = 0;
+    int pass_blocks[kSpmeBatchPasses] = {};
+};
+// ra: host copies of the replicas' RecipArgs; 0 or TCP_ERR_HIP
+int spme_batch_plan(const RecipArgs* ra, int nrep, SpmeBatchPlan* plan);
+void spme_batch_release(SpmeBatchPlan* plan);
+// spreading, convolution and gather of every replica, then Um (sc[SC_UM])
+void launch_spme_batch(const BatchRep* reps, const SpmeBatchPlan& plan, hipStream_t s);
 
 // Pair-kernel shapes of the persistent Newton's-third-law kernel: tiles per
 // block side and wavefronts per block.  Shape 0 is the single-context kernel's.

@@ -126,7 +126,11 @@ def test_batch_argument_errors_before_device_use():
     assert rc == -1 and "share" in msg and not h.value
     rc, msg, h = _batch_create([p, p.replace(flags=1)])
     assert rc == -1 and "share" in msg and not h.value
-    rc, msg, h = _batch_create([p.replace(recip=1), p.replace(recip=1)])
+    # SPME replicas: one spline order, brick-binned grids (every K a multiple of 8)
+    s = p.replace(recip=1, K=(16, 16, 16))
+    rc, msg, h = _batch_create([s, s.replace(order=6)])
+    assert rc == -1 and "share" in msg and not h.value
+    rc, msg, h = _batch_create([s, s.replace(K=(16, 12, 16))])
     assert rc == -4 and "batches" in msg and not h.value
     # a replica's own parameter error is reported with its index
     rc, msg, h = _batch_create([p, p.replace(rc=0.51 * p.box[0])])

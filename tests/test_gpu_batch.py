@@ -13,7 +13,7 @@ import pytest
 from tcp_allpair_amd.gpu import Batch, Context, TcpError
 from tcp_allpair_amd.inputs import make_plasma
 from tcp_allpair_amd.params import (BEREND, EPS, FP64, ISOKIN, KELBG, MIXED, NOTEMP, RECIP_EWALD,
-                                    RECIP_NONE, REG_EE, STOCH, reduced_dt)
+                                    RECIP_NONE, RECIP_SPME, REG_EE, STOCH, reduced_dt)
 
 pytestmark = pytest.mark.gpu
 
@@ -220,6 +220,69 @@ def test_autotune_and_graph_reuse():
     np.testing.assert_array_equal(np.concatenate([t1, t2]), tr)
     for r in range(len(reps)):
         np.testing.assert_array_equal(s1[r][0], st[r][0])
+
+
+# SPME replicas: grids from 16^3 to 32^3, one anisotropic
+SPME_K = [(16, 16, 16), (32, 32, 32), (16, 32, 16), (32, 32, 32), (32, 16, 32)]
+
+
+def spme_replicas(order=4, flags=0):
+    out = []
+    for r, ((ne, ni), d, t) in enumerate(zip(SIZES, DENS, TEMP)):
+        p, x, v = make_plasma(ne, ni, density=d, T=t * EPS, seed=140 + r, recip=RECIP_SPME,
+                              K=SPME_K[r], order=order)
+        out.append((p.replace(flags=flags), x, v))
+    return out
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_fp64_batch_spme_bitwise(order):
+    """SPME in a batch: every replica's binning, spreading, five FFT passes and
+    gather run over the replica table with the single-context kernel bodies,
+    so each replica equals its own context's tcp_md_steps bit for bit."""
+    say("case batch spme fp64 order", order)
+    assert_bitwise(spme_replicas(order), FP64, 4, NOTEMP, reduced_dt(0.005))
+
+
+def test_mixed_batch_spme_matches_standalone():
+    say("case batch spme mixed")
+    assert_bitwise(spme_replicas(flags=KELBG), MIXED, 3, NOTEMP, reduced_dt(0.005), shape=3)
+
+
+def test_batch_spme_vs_oracle():
+    """One SPME batch step with dt = 0 (positions stay, forces of main.f90:100-141):
+    every replica's FP64 forces against the oracle's real space + SPME + mean-force
+    removal <= 1e-10 per particle, Ur and Um <= 1e-12."""
+    from oracle import cpu
+    say("case batch spme vs oracle")
+    reps = spme_replicas(order=6)
+    with Batch([p for p, _, _ in reps], FP64) as b:
+        for r, (p, x, v) in enumerate(reps):
+            b.set_state(r, x=x, v=np.zeros_like(v), f=np.zeros_like(x))
+        tr = b.md_steps(1, NOTEMP, 0.0)
+        st = [b.get_state(r) for r in range(len(reps))]
+    for r, (p, x, _) in enumerate(reps):
+        fo, uro, _ = cpu.force_allpair(p, x)
+        fso, umo = cpu.spme(p, x)
+        fo = cpu.remove_rigidmtn(p, fo + fso)
+        f = st[r][2]
+        nref = np.linalg.norm(fo, axis=1)
+        err = np.linalg.norm(f - fo, axis=1) / np.maximum(nref, 1e-3 * np.sqrt(np.mean(nref ** 2)))
+        assert err.max() <= 1e-10, (r, err.max())
+        assert abs(tr[0, r, 0] - uro) <= 1e-12 * abs(uro), r
+        assert abs(tr[0, r, 2] - umo) <= 1e-12 * abs(umo), r
+
+
+def test_batch_spme_needs_binned_grids():
+    """Batched SPME takes brick-binned grids only (every K a multiple of 8)."""
+    say("case batch spme grid")
+    p, _, _ = make_plasma(200, 200, K=(4, 16, 8), recip=RECIP_SPME)
+    q, _, _ = make_plasma(300, 300, K=(16, 16, 16), recip=RECIP_SPME)
+    with pytest.raises(TcpError):
+        Batch([q, p], FP64)
+    r, _, _ = make_plasma(300, 300, K=(16, 16, 16), order=6, recip=RECIP_SPME)
+    with pytest.raises(TcpError):
+        Batch([q, r], FP64)  # one order per batch
 
 
 def test_batch_errors():
