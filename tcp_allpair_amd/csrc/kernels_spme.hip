@@ -888,9 +888,11 @@ __device__ inline void ewald_sk_reduce_body(const RecipArgs& a, int kmax, double
     decode_m(mi, kmax, &mx, &my, &mz);
     const double kx = mx / a.box[0], ky = my / a.box[1], kz = mz / a.box[2];
     const double m2 = kx * kx + ky * ky + kz * kz;
-    sk[3 * mi + 0] = sr;
-    sk[3 * mi + 1] = si;
-    sk[3 * mi + 2] = exp(-kPi * kPi * m2 / (alpha * alpha)) / m2;
+    const double G = exp(-kPi * kPi * m2 / (alpha * alpha)) / m2;
+    // G S (the force and potential sums take S only with G) and G |S|^2
+    sk[3 * mi + 0] = G * sr;
+    sk[3 * mi + 1] = G * si;
+    sk[3 * mi + 2] = G * (sr * sr + si * si);
 }
 
 // Forces F_j = (2 eps q_j / V) sum_m G m/L Im(conj(S) e^{2 pi i m.r_j}) (all m),
@@ -914,13 +916,13 @@ __device__ inline double2 cpowi(double2 u, int k)
     return e;
 }
 
-// Dynamic LDS of ewald_force_body, in doubles: the staged table, then reused
-// for the reductions.
+// Dynamic LDS of ewald_force_body, in doubles: the staged G S pairs, then
+// reused for the reductions.
 __host__ __device__ inline int ewald_force_lds(int kmax, bool staged)
 {
     const int nk = 2 * kmax + 1, nh = (nk * nk * nk - 1) / 2;
     const int red = kKsumWaves * 4 * 64;
-    return staged && 3 * nh > red ? 3 * nh : red;
+    return staged && 2 * nh > red ? 2 * nh : red;
 }
 
 template <bool kStaged>
@@ -932,11 +934,13 @@ __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const doub
     const int j = blk * 64 + lane;
     const int nk = 2 * kmax + 1, nh = (nk * nk * nk - 1) / 2;
     if (kStaged) {
-        // entry mi of the half space at lds[3 (mi - nh - 1) ..]
-        for (int t = threadIdx.x; t < 3 * nh; t += blockDim.x) lds[t] = skg[3 * (nh + 1) + t];
+        // G S of half-space entry mi at lds[2 (mi - nh - 1) ..]: one 16-byte read
+        for (int t = threadIdx.x; t < 2 * nh; t += blockDim.x)
+            lds[t] = skg[3 * (nh + 1) + 3 * (t >> 1) + (t & 1)];
         __syncthreads();
     }
-    // half-space entry h = mi - nh - 1
+    // half-space entry h = mi - nh - 1: G Re S and G Im S at sk[kS h], sk[kS h + 1]
+    constexpr int kS = kStaged ? 2 : 3;
     const double* sk = kStaged ? lds : skg + 3 * (nh + 1);
     const int h0 = wave * nh / kKsumWaves, h1 = (wave + 1) * nh / kKsumWaves;
     double fx = 0.0, fy = 0.0, fz = 0.0, pt = 0.0;
@@ -955,12 +959,13 @@ __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const doub
             // the share's part of row (my, mz): mx up to kmax, branch-free
             const int hend = min(h1, h + kmax - mx + 1);
             double gx = 0.0, gi = 0.0, mxd = (double)mx;
+#pragma unroll 2
             for (; h < hend; ++h) {
-                const double Sr = sk[3 * h], Si = sk[3 * h + 1], G = sk[3 * h + 2];
-                const double im = G * (Sr * e.y - Si * e.x);  // G Im(conj(S) e)
+                const double A = sk[kS * h], B = sk[kS * h + 1];  // G S
+                const double im = fma(A, e.y, -B * e.x);           // G Im(conj(S) e)
                 gx = fma(im, mxd, gx);
                 gi += im;
-                pt = fma(G, fma(Sr, e.x, Si * e.y), pt);      // G Re(conj(S) e)
+                pt = fma(A, e.x, fma(B, e.y, pt));                 // G Re(conj(S) e)
                 e = cmul(e, ux);
                 mxd += 1.0;
             }
@@ -1009,7 +1014,7 @@ __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const doub
         double en = 0.0;
         for (int h = threadIdx.x; h < nh; h += blockDim.x) {
             const int mi = nh + 1 + h;
-            en += skg[3 * mi + 2] * (skg[3 * mi] * skg[3 * mi] + skg[3 * mi + 1] * skg[3 * mi + 1]);
+            en += skg[3 * mi + 2];  // G |S|^2
         }
         esum[threadIdx.x] = en;
         __syncthreads();
