@@ -779,12 +779,13 @@ __device__ inline void ewald_sk_part_body(const RecipArgs& a, int kmax, int chun
     }
 }
 
-// The same partials by rows: a row is the half-space line (my, mz), walked
-// along mx (up to kLen = 2 kmax + 1 values) by the recurrence e <- e e^{i theta_x},
-// so a particle costs a row two complex products to start and one per m, and
-// the row's S(m) partials stay in registers.  Row 0 is (my, mz) = (0, 0) with
-// mx = 1..kmax; rows 1..kmax are mz = 0, my = 1..kmax; then mz = 1..kmax with
-// my = -kmax..kmax.  The block's threads form ewald_sk_groups(kmax) groups of
+// The same partials by rows: a row is the line (my, mz), walked along
+// mx = -kmax..kmax (kLen = 2 kmax + 1 values, a compile-time count, so the
+// walk has no per-m test) by the recurrence e <- e e^{i theta_x}; a particle
+// costs a row two complex products to start and one per m, and the row's S(m)
+// partials stay in registers.  Row 0 is (my, mz) = (0, 0), whose half-space
+// part mx = 1..kmax is the only part stored; rows 1..kmax are mz = 0,
+// my = 1..kmax; then mz = 1..kmax with my = -kmax..kmax.  The block's threads form ewald_sk_groups(kmax) groups of
 // `rows` threads; group g walks its own contiguous share of the chunk's
 // particles, and the groups' partials are joined in LDS in group order, so the
 // sum is fixed by kmax and chunk.  ux: e^{i theta_x} of the chunk.
@@ -808,55 +809,52 @@ __device__ inline void ewald_sk_rows_body(const RecipArgs& a, int kmax, int chun
     const int rows = ewald_sk_rows(kmax), groups = ewald_sk_groups(kmax);
     const int g = threadIdx.x / rows, t = threadIdx.x - g * rows;
     const bool live = g < groups;
-    int my, mz, mx0;
+    int my, mz;
     if (t <= kmax) {
         mz = 0;
         my = t;
-        mx0 = t == 0 ? 1 : -kmax;
     } else {
         mz = 1 + (t - kmax - 1) / nk;
         my = (t - kmax - 1) % nk - kmax;
-        mx0 = -kmax;
     }
-    const int len = kmax - mx0 + 1;
+    // every row walks mx = -kmax .. kmax, so the mx loop has one length for all
+    // threads; of row (0, 0) only mx > 0 lies in the half space and is stored
     double2 acc[kLen];
 #pragma unroll
     for (int k = 0; k < kLen; ++k) acc[k] = make_double2(0.0, 0.0);
     if (live) {
         const double2* Y = ph + (1 * K1 + abs(my)) * chunk;
         const double2* Z = ph + (2 * K1 + mz) * chunk;
-        const double2* X = ph + (0 * K1 + (t == 0 ? 1 : kmax)) * chunk;  // q u^1, or q u^kmax to conjugate
+        const double2* X = ph + (0 * K1 + kmax) * chunk;  // q u^kmax, conjugated: q u^-kmax
         const int p1 = (g + 1) * chunk / groups;
         for (int p = g * chunk / groups; p < p1; ++p) {
             const double2 y = my < 0 ? conjd(Y[p]) : Y[p];
-            const double2 x = t == 0 ? X[p] : conjd(X[p]);
+            const double2 x = conjd(X[p]);
             double2 e = cmul(cmul(x, y), Z[p]);
             const double2 u = ux[p];
 #pragma unroll
             for (int k = 0; k < kLen; ++k) {
-                if (k < len) {
-                    acc[k].x += e.x;
-                    acc[k].y += e.y;
-                    e = cmul(e, u);
-                }
+                acc[k].x += e.x;
+                acc[k].y += e.y;
+                e = cmul(e, u);
             }
         }
     }
-    const int h0 = ((mz + kmax) * nk + (my + kmax)) * nk + (mx0 + kmax) - (nh + 1);
+    const int h0 = ((mz + kmax) * nk + (my + kmax)) * nk - (nh + 1);  // mx = -kmax
     if (groups > 1) {
         // groups 1.. park their partials over the phase table, group 0 adds them in order
         __syncthreads();
         if (live && g > 0) {
 #pragma unroll
             for (int k = 0; k < kLen; ++k)
-                if (k < len) ph[(size_t)(g - 1) * nh + h0 + k] = acc[k];
+                if (h0 + k >= 0) ph[(size_t)(g - 1) * nh + h0 + k] = acc[k];
         }
         __syncthreads();
         if (g == 0) {
             for (int o = 0; o < groups - 1; ++o) {
 #pragma unroll
                 for (int k = 0; k < kLen; ++k)
-                    if (k < len) {
+                    if (h0 + k >= 0) {
                         const double2 v = ph[(size_t)o * nh + h0 + k];
                         acc[k].x += v.x;
                         acc[k].y += v.y;
@@ -867,7 +865,7 @@ __device__ inline void ewald_sk_rows_body(const RecipArgs& a, int kmax, int chun
     if (g == 0) {
 #pragma unroll
         for (int k = 0; k < kLen; ++k)
-            if (k < len) a.skpart[(size_t)c * nh + h0 + k] = acc[k];
+            if (h0 + k >= 0) a.skpart[(size_t)c * nh + h0 + k] = acc[k];
     }
 }
 
@@ -1026,17 +1024,22 @@ __device__ inline void ewald_force_body(const RecipArgs& a, int kmax, const doub
     }
 }
 
-// the row form for kmax <= 8 (2 kmax + 1 <= 17 accumulators), per-m otherwise
+// the row form for kmax <= 8 (2 kmax + 1 <= 17 accumulators), per m otherwise
 __device__ inline void ewald_sk_part_any(const RecipArgs& a, int kmax, int chunk, int c,
                                          double2* dyn)
 {
     double2* ux = dyn + 3 * (kmax + 1) * chunk;
-    if (kmax <= 6)
-        ewald_sk_rows_body<13>(a, kmax, chunk, c, dyn, ux);
-    else if (kmax <= 8)
-        ewald_sk_rows_body<17>(a, kmax, chunk, c, dyn, ux);
-    else
-        ewald_sk_part_body(a, kmax, chunk, c, dyn);
+    switch (kmax) {  // the row length 2 kmax + 1 as a constant
+        case 1: ewald_sk_rows_body<3>(a, kmax, chunk, c, dyn, ux); break;
+        case 2: ewald_sk_rows_body<5>(a, kmax, chunk, c, dyn, ux); break;
+        case 3: ewald_sk_rows_body<7>(a, kmax, chunk, c, dyn, ux); break;
+        case 4: ewald_sk_rows_body<9>(a, kmax, chunk, c, dyn, ux); break;
+        case 5: ewald_sk_rows_body<11>(a, kmax, chunk, c, dyn, ux); break;
+        case 6: ewald_sk_rows_body<13>(a, kmax, chunk, c, dyn, ux); break;
+        case 7: ewald_sk_rows_body<15>(a, kmax, chunk, c, dyn, ux); break;
+        case 8: ewald_sk_rows_body<17>(a, kmax, chunk, c, dyn, ux); break;
+        default: ewald_sk_part_body(a, kmax, chunk, c, dyn); break;
+    }
 }
 
 __global__ __launch_bounds__(256) void ewald_sk_part_kernel(RecipArgs a, int kmax, int chunk)
