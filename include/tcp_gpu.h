@@ -204,7 +204,7 @@ int tcp_force_direct_noalloc(const tcp_params* p, int precision, const double* x
  * TCP_ERR_UNSUPPORTED).  Every
  * replica reproduces tcp_md_steps on its own context bit for bit: in FP64
  * always, in TCP_MIXED when the batch's shape (tcp_batch_set_shape) is the
- * context's own, 0 above about 32000 particles and 3 below. */
+ * context's own, 0 above about 46000 particles and 3 below. */
 typedef struct tcp_batch tcp_batch;
 
 int tcp_batch_create(tcp_batch** out, int nrep, const tcp_params* p /* [nrep] */, int precision);

@@ -9,7 +9,7 @@
 // are those of the single-context path with the same block decomposition, so
 // every replica reproduces its standalone run bit for bit: always in FP64, and
 // in mixed precision at the standalone run's tile shape (4 x 4 tiles below
-// about 32000 particles, 8 x 8 above).  The autotuner may pick another shape
+// about 46000 particles, 8 x 8 above).  The autotuner may pick another shape
 // for the persistent mixed kernel (a different, equally valid FP32 grouping).
 #include <algorithm>
 #include <cmath>

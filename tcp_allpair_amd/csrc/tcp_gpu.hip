@@ -682,11 +682,12 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
             return bail(fail(c, TCP_ERR_HIP, "no multiprocessor count"));
         const bool kel = (p->flags & TCP_KELBG) != 0;
-        // 4 x 4 tile blocks when the 8 x 8 ones are fewer than twice the
-        // resident blocks (a function of N and the CU count only)
+        // 4 x 4 tile blocks for small systems (n3_self_shape: a function of
+        // N, the precision and the CU count only)
         const bool mixed = precision == TCP_MIXED;
         c->n3_shape = n3_self_shape(c->N, cus * (mixed ? pair_n3_occupancy(true, kel, 0)
-                                                       : pair_f64_n3_occupancy(true, kel, 0)));
+                                                       : pair_f64_n3_occupancy(true, kel, 0)),
+                                    mixed);
         c->n3_grid = cus * (mixed ? pair_n3_occupancy(true, kel, c->n3_shape)
                                   : pair_f64_n3_occupancy(true, kel, c->n3_shape));
         // upper triangle of block pairs, off-diagonal (full) blocks first

@@ -117,10 +117,10 @@ def test_fp64_batch_two_shapes_bitwise():
 def test_mixed_batch_matches_standalone(shape, large):
     """A single context runs the same kernel body on 4 x 4 tile blocks for
     systems as small as the five replicas (shape 3) and on 8 x 8 tile blocks for
-    a 40000-particle one (shape 0); the batch at the context's shape matches it
+    a 60000-particle one (shape 0); the batch at the context's shape matches it
     bit for bit (the waves of a block add their reactions in fixed turns)."""
     say("case batch mixed vs standalone, shape", shape)
-    reps = [make_plasma(20000, 20000, seed=120, recip=RECIP_NONE)] if large else replicas()
+    reps = [make_plasma(30000, 30000, seed=120, recip=RECIP_NONE)] if large else replicas()
     assert_bitwise(reps, MIXED, 5 if not large else 2, NOTEMP, reduced_dt(0.005), shape=shape)
 
 
