@@ -130,8 +130,7 @@ extern "C" int tcp_force_direct_noalloc(const tcp_params* p, int precision, cons
     const bool kel = (p->flags & TCP_KELBG) != 0;
     // the block shape and resident blocks tcp_create picks for this system
     const int shape = n3_self_shape(n, cus * (mixed ? pair_n3_occupancy(true, kel, 0)
-                                                    : pair_f64_n3_occupancy(true, kel, 0)),
-                                    mixed);
+                                                    : pair_f64_n3_occupancy(true, kel, 0)));
     const int grid = cus * (mixed ? pair_n3_occupancy(rdf != nullptr, kel, shape)
                                   : pair_f64_n3_occupancy(rdf != nullptr, kel, shape));
     const int rows = n3_block_rows_shape(n, shape);

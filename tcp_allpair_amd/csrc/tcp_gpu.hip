@@ -686,8 +686,7 @@ int tcp_create(tcp_ctx** out, const tcp_params* p, int precision)
         // N, the precision and the CU count only)
         const bool mixed = precision == TCP_MIXED;
         c->n3_shape = n3_self_shape(c->N, cus * (mixed ? pair_n3_occupancy(true, kel, 0)
-                                                       : pair_f64_n3_occupancy(true, kel, 0)),
-                                    mixed);
+                                                       : pair_f64_n3_occupancy(true, kel, 0)));
         c->n3_grid = cus * (mixed ? pair_n3_occupancy(true, kel, c->n3_shape)
                                   : pair_f64_n3_occupancy(true, kel, c->n3_shape));
         // upper triangle of block pairs, off-diagonal (full) blocks first

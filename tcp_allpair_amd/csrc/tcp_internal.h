@@ -259,14 +259,14 @@ struct N3Shape {
 N3Shape n3_shape(int shape);
 int n3_block_rows_shape(int n, int shape);
 // the 4 x 4 tile, four-wave shape that a single context takes when the 8 x 8
-// tile blocks are fewer than four (mixed) or two (FP64) times its resident
-// blocks: their few rounds of long items would leave the chip idle at the end
+// tile blocks are fewer than four times its resident blocks: their few rounds
+// of long items would leave the chip idle at the end
 constexpr int kFineN3Shape = 3;
 // that rule, from the particle count and the resident blocks of shape 0
-inline int n3_self_shape(int n, int grid0, bool mixed)
+inline int n3_self_shape(int n, int grid0)
 {
     const int64_t rows0 = n3_block_rows_shape(n, 0);
-    return rows0 * (rows0 + 1) / 2 < (mixed ? 4 : 2) * (int64_t)grid0 ? kFineN3Shape : 0;
+    return rows0 * (rows0 + 1) / 2 < 4 * (int64_t)grid0 ? kFineN3Shape : 0;
 }
 // Persistent launches over a work list; queue[2] must be zero before the first
 // launch and is left zero by every launch.  grid: resident blocks to start.
