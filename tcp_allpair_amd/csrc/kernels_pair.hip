@@ -1374,6 +1374,108 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
     uo = u;
 }
 
+// n3_tile_f64 for a tile pair whose two tiles are full and hold one species
+// each (si, sj wave-uniform): every per-species constant is a scalar, the
+// regularisation branch is uniform, no live flag is read, and the sums are
+// charge-free (eps qi qj and qi qj applied once at the end).  The minimum image
+// is dx - L rint(dx/L) as one fma, which equals the separately rounded product
+// and difference while L rint(dx/L) is exact (|dx| < 2.5 L; wrapped positions
+// have |dx| <= L), so r^2, the cut-off and the RDF bins stay the reference's.
+template <bool kRdf, bool kKelbg, bool kDiag>
+__device__ inline void n3_tile_f64_uni(const PairArgs& a, unsigned* hist, const double* etab,
+                                       const double* e2tab, double xi, double yi, double zi,
+                                       int si, int sj, const F64J* jt, double& fxo, double& fyo,
+                                       double& fzo, double& uo, double& rx, double& ry, double& rz)
+{
+    const char* jb = (const char*)jt;
+    uint32_t off = (uint32_t)(threadIdx.x % kTile) * (uint32_t)sizeof(F64J);
+    double qx = *(const double*)(jb + off), qy = *(const double*)(jb + off + 8),
+           qz = *(const double*)(jb + off + 16);
+    const double Lx = a.box[0], Ly = a.box[1], Lz = a.box[2];
+    const double iLx = a.inv_box[0], iLy = a.inv_box[1], iLz = a.inv_box[2];
+    const double alpha = a.alpha, rc2 = a.rc2;
+    const double two_sqrtpi = 2.0 / kSqrtPi;
+    const double inv_rs = 1.0 / a.rs;
+    const int ht = si + sj;
+    const double dxb = a.dx[ht], idxb = a.inv_dx[ht];
+    const bool reg = sj == 0 ? (si != 0 || (a.flags & TCP_REG_EE) != 0) : si == 0;
+    unsigned* hrow = hist + ht * kHistStride;
+    double fx = 0.0, fy = 0.0, fz = 0.0, u = 0.0;
+    rx = ry = rz = 0.0;
+#pragma unroll 2
+    for (int s = 0; s < kTile; ++s) {  // by two: the carried offset and j alternate registers
+        const uint32_t noff = rot_u(off);
+        const double nx = *(const double*)(jb + noff), ny = *(const double*)(jb + noff + 8),
+                     nz = *(const double*)(jb + noff + 16);
+        double xr, yr, zr, r2;
+        {
+#pragma clang fp contract(off)
+            const double dx = xi - qx;
+            const double dy = yi - qy;
+            const double dz = zi - qz;
+            xr = fma(-Lx, rint(dx * iLx), dx);
+            yr = fma(-Ly, rint(dy * iLy), dy);
+            zr = fma(-Lz, rint(dz * iLz), dz);
+            r2 = xr * xr + yr * yr + zr * zr;
+        }
+        if (r2 < rc2 && r2 > 0.0) {
+            const double inv_r = rsq_f64(r2);
+            const double r = r2 * inv_r;
+            if (kRdf) {
+                const double q = r * idxb;
+                int b = (int)q;
+                const double fr = q - (double)b;
+                if (fr < 1e-12 || fr > 1.0 - 1e-12) b = (int)(sqrt(r2) / dxb);
+                if (b < kNbin) atomicAdd(&hrow[b], kDiag ? 1u : 2u);
+            }
+            const double x = alpha * r;
+            const double e = exp_mx2(x, e2tab);
+            const double ec = e * erfcx_tab(x, etab);
+            double num = fma(two_sqrtpi * x, e, ec), en = ec;
+            if (reg) {
+                const double sr = r * inv_rs;
+                if (kKelbg) {
+                    const double y2 = exp_mx2(sr, e2tab);
+                    num -= y2;
+                    en += y2 * fma(kSqrtPi * sr, erfcx_tab(sr, etab), -1.0);
+                } else {
+                    const double y = exp_tab(-sr, e2tab);
+                    num -= y * (1.0 + sr);
+                    en -= y;
+                }
+            }
+            const double df = num * (inv_r * inv_r) * inv_r;
+            u = fma(en, inv_r, u);
+            fx = fma(df, xr, fx);
+            fy = fma(df, yr, fy);
+            fz = fma(df, zr, fz);
+            if (!kDiag) {
+                rx = fma(-df, xr, rx);
+                ry = fma(-df, yr, ry);
+                rz = fma(-df, zr, rz);
+            }
+        }
+        off = noff;
+        qx = nx;
+        qy = ny;
+        qz = nz;
+        if (!kDiag) {
+            rx = rot_d(rx);
+            ry = rot_d(ry);
+            rz = rot_d(rz);
+        }
+    }
+    const double qq = a.q[si] * a.q[sj];
+    const double eqq = a.eps * qq;
+    fxo = eqq * fx;
+    fyo = eqq * fy;
+    fzo = eqq * fz;
+    uo = (kDiag ? 0.5 * qq : qq) * u;
+    rx *= eqq;
+    ry *= eqq;
+    rz *= eqq;
+}
+
 // LDS of one FP64 block: j positions and flags, the block's reaction sums
 // (slot br), the per-wave reaction buffer and the RDF histogram.
 template <int kBT>
@@ -1409,7 +1511,7 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds<kB
     }
     if (kRdf) hist_clear(L.hist);
     __syncthreads();
-    const int wave = threadIdx.x / kTile, lane = threadIdx.x % kTile;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile), lane = threadIdx.x % kTile;
     const int ntiles = (n + kTile - 1) / kTile;
     const int I = br * kBT + wave;
     const bool exists = I < ntiles;
@@ -1418,6 +1520,13 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds<kB
     const int ic = min(ip, n - 1);
     const double xi = a.pi.x[ic], yi = a.pi.y[ic], zi = a.pi.z[ic];
     const int si = ip >= ne;
+    // a full tile of one species
+    auto uniform = [&](int T) {
+        const int p0 = T * kTile, p1 = p0 + kTile - 1;
+        return p1 < n && (p0 >= ne || p1 < ne);
+    };
+    const bool iuni = uniform(I);
+    const int siu = I * kTile >= ne;
     double Fx = 0.0, Fy = 0.0, Fz = 0.0, U = 0.0;
     for (int tj = 0; tj < kBT; ++tj) {
         const int J = bc * kBT + tj;
@@ -1425,7 +1534,15 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds<kB
         if (exists && J < ntiles && (!dblk || tj >= wave)) {
             const F64J* jt = L.jp + tj * kTile;
             double fx, fy, fz, u;
-            if (dblk && tj == wave)
+            const int sju = J * kTile >= ne;
+            if (iuni && uniform(J)) {
+                if (dblk && tj == wave)
+                    n3_tile_f64_uni<kRdf, kKelbg, true>(a, L.hist, L.etab, L.e2tab, xi, yi, zi, siu,
+                                                        sju, jt, fx, fy, fz, u, rx, ry, rz);
+                else
+                    n3_tile_f64_uni<kRdf, kKelbg, false>(a, L.hist, L.etab, L.e2tab, xi, yi, zi,
+                                                         siu, sju, jt, fx, fy, fz, u, rx, ry, rz);
+            } else if (dblk && tj == wave)
                 n3_tile_f64<kRdf, kKelbg, true>(a, L.hist, L.etab, L.e2tab, xi, yi, zi, si, ilive,
                                                 jt, fx, fy, fz, u, rx, ry, rz);
             else
