@@ -1,22 +1,25 @@
 """Writes f64_tables.h, the constant tables of the FP64 pair kernel:
 
-kErfcxTab: piecewise polynomials of the scaled complementary error function
-erfcx(x) = exp(x^2) erfc(x).  The kernel forms erfc(x) = exp(-x^2) erfcx(x)
-from the exp(-x^2) it needs anyway for the Gaussian term of the Ewald force.
-Intervals: width 1/8 on [0, 28) (224 intervals, one uniform grid so that the
-kernel finds the interval with one multiply); on each, erfcx(m + h t) for t
-in [-1, 1] is the degree-9
-polynomial interpolating erfcx at the 10 Chebyshev nodes, computed with
-40-digit arithmetic (mpmath) and rounded to double.  Past x = 28, exp(-x^2)
-underflows to zero, so the value there does not matter.  Maximum relative
-error of the double Horner evaluation: 1.9e-16.
+kErfcTab: piecewise polynomials of the complementary error function erfc(x).
+Intervals: width 1/16 on [0, 7) (112 intervals, one uniform grid so that the
+kernel finds the interval from one multiply and one conversion); on each,
+erfc(m + h t) for t in [-1, 1] is the degree-9 polynomial interpolating erfc
+at the 10 Chebyshev nodes, computed with 40-digit arithmetic (mpmath) and
+rounded to double.  A last row of zeros stands for x >= 7, where
+erfc(x) < 4.2e-23 and (2/sqrt(pi)) exp(-x^2) < 6e-22.  The kernel evaluates
+the polynomial P and its derivative P' by one Horner pass; erfc'(x) =
+-(2/sqrt(pi)) exp(-x^2) = 32 P'(t), so the Gaussian term of the Ewald force,
+(2x/sqrt(pi)) exp(-x^2) = -32 x P'(t), and Kelbg's exp(-s^2) come from the
+same row.  Maximum absolute error of the double Horner evaluation: erfc
+1.1e-16, (2/sqrt(pi)) exp(-x^2) 2.9e-16 (both about an ulp of values near 1;
+the reference's erfc, 1 - DERF(x), is itself good to about 1.1e-16 absolute).
 
 kExp2Tab: 2^(j/64), j = 0..63, correctly rounded, for exp(y) = 2^n 2^(j/64)
 e^r with y = (64 n + j) ln2/64 + r, |r| <= ln2/128, where e^r is the Taylor
 polynomial of degree 5 (remainder below r^6/720 < 4e-17).
 
-Both are checked by tests/test_f64_tables.py (against scipy.special.erfcx and
-numpy's exp).
+Both are checked by tests/test_f64_tables.py (against scipy.special.erfc,
+the exact derivative, and numpy's exp).
 
     python tcp_allpair_amd/csrc/gen_f64_tables.py      (rewrites f64_tables.h)
 """
@@ -27,7 +30,7 @@ import os
 import mpmath as mp
 
 DEG = 9
-XMAX, PER_UNIT = 28, 8  # intervals of width 1/8 on [0, 28)
+XMAX, PER_UNIT = 7, 16  # intervals of width 1/16 on [0, 7), then a row of zeros
 EXP2_N = 64
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "f64_tables.h")
@@ -37,16 +40,12 @@ def intervals():
     return [(mp.mpf(i) / PER_UNIT, mp.mpf(i + 1) / PER_UNIT) for i in range(XMAX * PER_UNIT)]
 
 
-def erfcx(x):
-    return mp.exp(x * x) * mp.erfc(x)
-
-
 def interval_poly(a, b, deg=DEG):
-    """Monomial coefficients (in t = (x - m)/h) of the Chebyshev interpolant."""
+    """Monomial coefficients (in t = (x - m)/h) of the Chebyshev interpolant of erfc."""
     n = deg + 1
     m, h = (a + b) / 2, (b - a) / 2
     nodes = [mp.cos(mp.pi * (k + mp.mpf(1) / 2) / n) for k in range(n)]
-    vals = [erfcx(m + h * t) for t in nodes]
+    vals = [mp.erfc(m + h * t) for t in nodes]
     cheb = [2 * mp.fsum(vals[k] * mp.cos(mp.pi * j * (k + mp.mpf(1) / 2) / n) for k in range(n)) / n
             for j in range(n)]
     cheb[0] /= 2
@@ -68,21 +67,24 @@ def interval_poly(a, b, deg=DEG):
 def main():
     mp.mp.dps = 40
     rows = [interval_poly(a, b) for a, b in intervals()]
+    rows.append([0.0] * (DEG + 1))
     exp2 = [float(mp.power(2, mp.mpf(j) / EXP2_N)) for j in range(EXP2_N)]
     lines = [
         "// Generated by gen_f64_tables.py: do not edit.",
-        "// kErfcxTab: erfcx(x) = exp(x^2) erfc(x) on interval k: sum_j kErfcxTab[k][j] t^j,",
-        "// t = 16 x - (2k + 1) in [-1, 1], intervals of width 1/8 on [0, 28).",
+        "// kErfcTab: erfc(x) on interval k < kErfcN: sum_j kErfcTab[k][j] t^j,",
+        f"// t = {2 * PER_UNIT} x - (2k + 1) in [-1, 1], intervals of width 1/{PER_UNIT} on [0, {XMAX});",
+        f"// row kErfcN is zero (x >= {XMAX}).",
         "// kExp2Tab: 2^(j/64), j = 0..63.",
         "#pragma once",
         "",
         "namespace tcp {",
         "",
-        f"constexpr int kErfcxDeg = {DEG};",
-        f"constexpr int kErfcxN = {XMAX * PER_UNIT};  // intervals of width 1/{PER_UNIT} on [0, {XMAX})",
+        f"constexpr int kErfcDeg = {DEG};",
+        f"constexpr int kErfcPerUnit = {PER_UNIT};",
+        f"constexpr int kErfcN = {XMAX * PER_UNIT};  // intervals of width 1/{PER_UNIT} on [0, {XMAX})",
         f"constexpr int kExp2N = {EXP2_N};",
         "",
-        "__device__ __constant__ const double kErfcxTab[kErfcxN][kErfcxDeg + 1] = {",
+        "__device__ __constant__ const double kErfcTab[kErfcN + 1][kErfcDeg + 1] = {",
     ]
     for r in rows:
         lines.append("    {" + ", ".join(c.hex() for c in r) + "},")
@@ -92,7 +94,8 @@ def main():
     lines += ["};", "", "}  // namespace tcp", ""]
     with open(OUT, "w") as f:
         f.write("\n".join(lines))
-    print(f"wrote {OUT}: {len(rows)} erfcx intervals of degree {DEG}, {EXP2_N} powers of 2")
+    print(f"wrote {OUT}: {len(rows) - 1} erfc intervals of degree {DEG} and a zero row, "
+          f"{EXP2_N} powers of 2")
 
 
 if __name__ == "__main__":

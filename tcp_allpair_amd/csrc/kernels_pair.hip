@@ -1205,18 +1205,28 @@ __device__ inline double rot_d(double v)
     return __longlong_as_double((long long)((hi << 32) | lo));
 }
 
-// erfcx(x) = exp(x^2) erfc(x), x >= 0, from the LDS copy of kErfcxTab: the
-// interval of width 1/8 holding x (the last one past 28) and the degree-9
-// polynomial in t = 16 x - (2k + 1).  The row offset k (kErfcxDeg + 1) is a
-// 24-bit multiply (full rate; a 32-bit one issues at quarter rate).
-__device__ inline double erfcx_tab(double x, const double* tab)
+// erfc(x), x >= 0, from the LDS copy of kErfcTab, given xs = 32 x, and in dp
+// the derivative dP/dt of the interval's polynomial: erfc'(x) = 32 dP/dt =
+// -(2/sqrt(pi)) e^{-x^2}, so the Gaussian term of the Ewald force,
+// (2x/sqrt(pi)) e^{-x^2} = -xs dP/dt, and Kelbg's e^{-s^2} need no exponential.
+// The interval is k = floor(xs/2) (the zero row kErfcN for x >= 7; xs is
+// clamped before the conversion, so any finite x is safe) and t = xs - (2k + 1)
+// in [-1, 1), both exact; the row offset is a 24-bit multiply (full rate; a
+// 32-bit one issues at quarter rate).  P and dP/dt come from one Horner pass.
+__device__ inline double erfc_tab(double xs, const double* tab, double& dp)
 {
-    const int k = min((int)(x * 8.0), kErfcxN - 1);
-    const double t = fmin(fma(x, 16.0, -(double)(2 * k + 1)), 1.0);
-    const double* c = tab + __umul24((unsigned)k, kErfcxDeg + 1);
-    double p = c[kErfcxDeg];
+    const int ki = (int)fmin(xs, (double)(2 * kErfcN + 1));
+    const double t = xs - (double)(ki | 1);
+    const double* c = tab + __umul24((unsigned)(ki >> 1), kErfcDeg + 1);
+    double p = c[kErfcDeg];
+    double d = p;
+    p = fma(p, t, c[kErfcDeg - 1]);
 #pragma unroll
-    for (int j = kErfcxDeg - 1; j >= 0; --j) p = fma(p, t, c[j]);
+    for (int j = kErfcDeg - 2; j >= 0; --j) {
+        d = fma(d, t, p);
+        p = fma(p, t, c[j]);
+    }
+    dp = d;
     return p;
 }
 
@@ -1240,16 +1250,6 @@ __device__ inline double exp_tab(double y, const double* e2tab)
     q *= r;  // e^r - 1
     const double t = e2tab[ki & (kExp2N - 1)];
     return ldexp(fma(t, q, t), ki >> 6);
-}
-
-// exp(-x^2) with x^2 carried as hi + lo, so the result keeps the accuracy of
-// exp (the rounding of x^2 would cost up to 2 x^2 ulp)
-__device__ inline double exp_mx2(double x, const double* e2tab)
-{
-    const double hi = x * x;
-    const double lo = fma(x, x, -hi);
-    const double e = exp_tab(-hi, e2tab);
-    return fma(-e, lo, e);
 }
 
 // 1/sqrt(r2) to about one ulp: the hardware estimate and two Newton steps
@@ -1278,7 +1278,7 @@ struct __align__(32) F64J {
 // and bin = int(sqrt(r2) / dx) with both operations correctly rounded, which
 // r * (1/dx) (r = r2 / sqrt(r2) to an ulp or two) decides unless it lies within
 // 1e-12 of a bin edge; there the exact sqrt and division decide.
-// erfc(x) = exp(-x^2) erfcx(x) shares exp(-x^2) with the Gaussian term.
+// erfc(x) and the Gaussian term come from one row of the erfc table.
 template <bool kRdf, bool kKelbg, bool kDiag>
 __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const double* etab,
                                    const double* e2tab, double xi, double yi, double zi, int si,
@@ -1290,9 +1290,8 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
     F64J q = *(const F64J*)(jb + off);
     const double Lx = a.box[0], Ly = a.box[1], Lz = a.box[2];
     const double iLx = a.inv_box[0], iLy = a.inv_box[1], iLz = a.inv_box[2];
-    const double alpha = a.alpha, rc2 = a.rc2;
-    const double two_sqrtpi = 2.0 / kSqrtPi;
-    const double inv_rs = 1.0 / a.rs;
+    const double alpha32 = 32.0 * a.alpha, rc2 = a.rc2;
+    const double inv_rs = 1.0 / a.rs, inv_rs32 = 32.0 / a.rs;
     // per-i constants for a j electron (0) / a j ion (1)
     const double qi = si ? a.q[1] : a.q[0];
     const double eqi = a.eps * qi;
@@ -1331,18 +1330,22 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
             }
             const double qq = qi * (sj ? qj1 : qj0);
             const double inv_r2 = inv_r * inv_r;
-            const double x = alpha * r;
-            const double e = exp_mx2(x, e2tab);
-            const double ec = e * erfcx_tab(x, etab);
-            double num = fma(two_sqrtpi * x, e, ec), en = ec;
+            // num = erfc(x) + (2x/sqrt(pi)) e^{-x^2}, x = alpha r
+            double d;
+            const double xs = alpha32 * r;
+            const double ec = erfc_tab(xs, etab, d);
+            double num = fma(-xs, d, ec), en = ec;
             if (sj ? reg1 : reg0) {
-                const double sr = r * inv_rs;
                 if (kKelbg) {
-                    // R = -e^{-s^2} + sqrt(pi) s erfc(s)
-                    const double y2 = exp_mx2(sr, e2tab);
+                    // R = -e^{-s^2} + sqrt(pi) s erfc(s), s = r/lambda
+                    double ds;
+                    const double ss = r * inv_rs32;
+                    const double es = erfc_tab(ss, etab, ds);
+                    const double y2 = ds * (-16.0 * kSqrtPi);
                     num -= y2;
-                    en += y2 * fma(kSqrtPi * sr, erfcx_tab(sr, etab), -1.0);
+                    en += fma((kSqrtPi / 32.0) * ss, es, -y2);
                 } else {
+                    const double sr = r * inv_rs;
                     // R = -e^{-s} (Deutsch, force.f90:136-139)
                     const double y = exp_tab(-sr, e2tab);
                     num -= y * (1.0 + sr);
@@ -1393,9 +1396,8 @@ __device__ inline void n3_tile_f64_uni(const PairArgs& a, unsigned* hist, const 
            qz = *(const double*)(jb + off + 16);
     const double Lx = a.box[0], Ly = a.box[1], Lz = a.box[2];
     const double iLx = a.inv_box[0], iLy = a.inv_box[1], iLz = a.inv_box[2];
-    const double alpha = a.alpha, rc2 = a.rc2;
-    const double two_sqrtpi = 2.0 / kSqrtPi;
-    const double inv_rs = 1.0 / a.rs;
+    const double alpha32 = 32.0 * a.alpha, rc2 = a.rc2;
+    const double inv_rs = 1.0 / a.rs, inv_rs32 = 32.0 / a.rs;
     const int ht = si + sj;
     const double dxb = a.dx[ht], idxb = a.inv_dx[ht];
     const bool reg = sj == 0 ? (si != 0 || (a.flags & TCP_REG_EE) != 0) : si == 0;
@@ -1428,17 +1430,20 @@ __device__ inline void n3_tile_f64_uni(const PairArgs& a, unsigned* hist, const 
                 if (fr < 1e-12 || fr > 1.0 - 1e-12) b = (int)(sqrt(r2) / dxb);
                 if (b < kNbin) atomicAdd(&hrow[b], kDiag ? 1u : 2u);
             }
-            const double x = alpha * r;
-            const double e = exp_mx2(x, e2tab);
-            const double ec = e * erfcx_tab(x, etab);
-            double num = fma(two_sqrtpi * x, e, ec), en = ec;
+            double d;
+            const double xs = alpha32 * r;
+            const double ec = erfc_tab(xs, etab, d);
+            double num = fma(-xs, d, ec), en = ec;
             if (reg) {
-                const double sr = r * inv_rs;
                 if (kKelbg) {
-                    const double y2 = exp_mx2(sr, e2tab);
+                    double ds;
+                    const double ss = r * inv_rs32;
+                    const double es = erfc_tab(ss, etab, ds);
+                    const double y2 = ds * (-16.0 * kSqrtPi);
                     num -= y2;
-                    en += y2 * fma(kSqrtPi * sr, erfcx_tab(sr, etab), -1.0);
+                    en += fma((kSqrtPi / 32.0) * ss, es, -y2);
                 } else {
+                    const double sr = r * inv_rs;
                     const double y = exp_tab(-sr, e2tab);
                     num -= y * (1.0 + sr);
                     en -= y;
@@ -1484,15 +1489,15 @@ struct F64Lds {
     double racc[4][kBT * kTile];
     double wbuf[kBT][3][kTile];
     unsigned hist[3 * kHistStride];
-    double etab[kErfcxN * (kErfcxDeg + 1)] __attribute__((aligned(16)));
+    double etab[(kErfcN + 1) * (kErfcDeg + 1)] __attribute__((aligned(16)));
     double e2tab[kExp2N];
 };
 
 template <int kBT>
 __device__ inline void load_tables(F64Lds<kBT>& L)
 {
-    const double* src = &kErfcxTab[0][0];
-    for (int i = threadIdx.x; i < kErfcxN * (kErfcxDeg + 1); i += blockDim.x) L.etab[i] = src[i];
+    const double* src = &kErfcTab[0][0];
+    for (int i = threadIdx.x; i < (kErfcN + 1) * (kErfcDeg + 1); i += blockDim.x) L.etab[i] = src[i];
     for (int i = threadIdx.x; i < kExp2N; i += blockDim.x) L.e2tab[i] = kExp2Tab[i];
 }
 

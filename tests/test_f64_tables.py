@@ -1,11 +1,13 @@
 """The constant tables of the FP64 pair kernel (tcp_allpair_amd/csrc/f64_tables.h).
 
-erfcx: its piecewise polynomials, evaluated with the kernel's interval mapping
-(erfcx_tab in kernels_pair.hip), against scipy.special.erfcx, and erfc =
-exp(-x^2) erfcx against scipy.special.erfc over the range the Ewald and Kelbg
-terms use.  exp: the powers 2^(j/64) against mpmath at 40 digits, and the
-table-driven reduction of exp_tab (the same steps in numpy) against numpy's exp
-over the arguments the kernel meets (-x^2 and -r/lambda, down to underflow).
+erfc: its piecewise polynomials, evaluated with the kernel's interval mapping
+and one Horner pass for P and dP/dt (erfc_tab in kernels_pair.hip), against
+scipy.special.erfc, and the derivative against its closed form
+erfc'(x) = -(2/sqrt(pi)) exp(-x^2) (scipy/numpy), over the range the Ewald and
+Kelbg terms use; past x = 7 both are zero.  exp: the powers 2^(j/64) against
+mpmath at 40 digits, and the table-driven reduction of exp_tab (the same steps
+in numpy) against numpy's exp over the arguments the kernel meets (-r/lambda,
+down to underflow).
 """
 import os
 import re
@@ -22,14 +24,14 @@ def _text():
     return open(HEADER).read()
 
 
-def load_erfcx():
+def load_erfc():
     text = _text()
-    deg = int(re.search(r"kErfcxDeg = (\d+)", text).group(1))
-    n = int(re.search(r"kErfcxN = (\d+)", text).group(1))
-    body = text[text.index("kErfcxTab"):text.index("kExp2Tab[kExp2N]")]
+    deg = int(re.search(r"kErfcDeg = (\d+)", text).group(1))
+    n = int(re.search(r"kErfcN = (\d+)", text).group(1))
+    body = text[text.index("kErfcTab"):text.index("kExp2Tab[kExp2N]")]
     rows = re.findall(r"\{([^{}]*)\},", body)
     tab = np.array([[float.fromhex(v.strip()) for v in r.split(",")] for r in rows])
-    assert tab.shape == (n, deg + 1)
+    assert tab.shape == (n + 1, deg + 1)
     return tab
 
 
@@ -43,47 +45,102 @@ def load_exp2():
     return tab
 
 
-def erfcx_tab(x, tab):
-    """The kernel's mapping: intervals of width 1/8, the last one past 28."""
-    x = np.asarray(x, dtype=np.float64)
-    k = np.minimum((x * 8.0).astype(np.int64), len(tab) - 1)
-    t = np.minimum(x * 16.0 - (2 * k + 1), 1.0)
-    c = tab[k]
+def _fma(a, b, c):
+    """a b + c rounded once to double, as the device's fma (the 64-bit-mantissa
+    intermediate leaves an error far below an ulp of the result)."""
+    return (np.longdouble(a) * np.longdouble(b) + np.longdouble(c)).astype(np.float64)
+
+
+def erfc_tab(x, tab):
+    """The kernel's mapping (intervals of width 1/16, the zero row past 7) and
+    its Horner pass with fused multiply-adds: returns erfc(x) and
+    erfc'(x) = 32 dP/dt."""
+    xs = np.asarray(x, dtype=np.float64) * 32.0
+    ki = np.minimum(xs, 2.0 * (len(tab) - 1) + 1.0).astype(np.int64)
+    t = xs - (ki | 1)
+    c = tab[ki >> 1]
     p = c[:, -1].copy()
-    for j in range(tab.shape[1] - 2, -1, -1):
-        p = p * t + c[:, j]
-    return p
+    d = p.copy()
+    p = _fma(p, t, c[:, -2])
+    for j in range(tab.shape[1] - 3, -1, -1):
+        d = _fma(d, t, p)
+        p = _fma(p, t, c[:, j])
+    return p, 32.0 * d
 
 
-def test_table_matches_scipy_erfcx():
-    tab = load_erfcx()
+def _points(xmax):
     rng = np.random.default_rng(5)
-    x = np.concatenate([rng.uniform(0.0, 28.0, 200000), np.arange(0, 28 * 8 + 1) / 8.0,
-                        np.arange(0, 28 * 8) / 8.0 + 1e-12, [1e-300, 1e-8, 0.5, 7.999999999999, 8.0]])
-    got = erfcx_tab(x, tab)
-    ref = special.erfcx(x)
-    rel = np.abs(got - ref) / ref
-    assert rel.max() <= 1e-15, (x[rel.argmax()], rel.max())
-    assert abs(erfcx_tab([0.0], tab)[0] - 1.0) <= 2e-16
+    grid = np.arange(0, int(xmax * 16) + 1) / 16.0
+    return np.concatenate([rng.uniform(0.0, xmax, 200000), grid, grid + 1e-12,
+                           [1e-300, 1e-8, 0.5, 6.999999999999]])
 
 
-@pytest.mark.parametrize("xmax", [4.0, 8.0, 27.0])
-def test_erfc_from_table(xmax):
-    tab = load_erfcx()
+def test_erfc_table_matches_mpmath():
+    """Against erfc at 40 digits: absolute to about an ulp of values near 1
+    (the reference's 1 - DERF(x) is good to about the same), relative to
+    3e-16 where erfc > 1e-5 (x < 3.1, the Ewald range alpha r < 3.1 of
+    alpha = 3.1/rc)."""
+    tab = load_erfc()
+    rng = np.random.default_rng(11)
+    grid = np.arange(0, 7 * 16) / 16.0
+    x = np.concatenate([rng.uniform(0.0, 7.0, 3000), grid, grid + 1e-12, [1e-300, 6.999999999999]])
+    p, _ = erfc_tab(x, tab)
+    mp.mp.dps = 40
+    ref = np.array([float(mp.erfc(v)) for v in x])
+    err = np.abs(p - ref)
+    assert err.max() <= 1.5e-16, (x[err.argmax()], err.max())
+    big = ref > 1e-5
+    assert (err[big] / ref[big]).max() <= 3e-16
+    assert abs(erfc_tab([0.0], tab)[0][0] - 1.0) <= 2e-16
+
+
+def test_erfc_table_matches_scipy():
+    """A dense sweep against scipy.special.erfc, whose own error reaches
+    3e-16 absolute near x = 0.9."""
+    tab = load_erfc()
+    x = _points(7.0)
+    p, _ = erfc_tab(x, tab)
+    assert np.abs(p - special.erfc(x)).max() <= 5e-16
+
+
+def test_erfc_derivative_is_the_gaussian():
+    tab = load_erfc()
+    x = _points(7.0)
+    _, d = erfc_tab(x, tab)
+    g = 2.0 / np.sqrt(np.pi) * np.exp(-x * x)
+    assert np.abs(-d - g).max() <= 5e-16
+    big = g > 1e-4
+    assert (np.abs(-d - g)[big] / g[big]).max() <= 1e-14
+
+
+@pytest.mark.parametrize("xmax", [3.1, 4.5, 7.0])
+def test_ewald_force_term(xmax):
+    """num = erfc(x) + (2x/sqrt(pi)) exp(-x^2) as the kernel forms it, x P - xs dP/dt."""
+    tab = load_erfc()
     x = np.linspace(0.0, xmax, 100001)
-    got = np.exp(-x * x) * erfcx_tab(x, tab)
-    ref = special.erfc(x)
-    ok = ref > 1e-300
-    rel = np.abs(got[ok] - ref[ok]) / ref[ok]
-    # exp(-x^2) of a rounded x^2 costs up to 2 x^2 ulp here; the kernel splits x^2 in hi + lo
-    assert rel.max() <= 4e-16 * (1 + 2 * xmax * xmax)
+    p, d = erfc_tab(x, tab)
+    num = _fma(-32.0 * x, d / 32.0, p)
+    ref = special.erfc(x) + 2.0 * x / np.sqrt(np.pi) * np.exp(-x * x)
+    assert np.abs(num - ref).max() <= 1e-15
 
 
-def test_past_the_table_is_finite():
-    tab = load_erfcx()
-    x = np.array([28.0, 30.0, 1e3, 1e6])
-    p = erfcx_tab(x, tab)
-    assert np.all(np.isfinite(p)) and np.all(np.exp(-x * x) * p == 0.0)
+def test_past_the_table_is_zero():
+    tab = load_erfc()
+    assert np.all(tab[-1] == 0.0)
+    x = np.array([7.0, 7.5, 30.0, 1e3, 1e6, 1e300])
+    p, d = erfc_tab(x, tab)
+    assert np.all(p == 0.0) and np.all(d == 0.0)
+    # erfc and (2/sqrt(pi)) exp(-x^2) dropped there are below 4.2e-23 and 6e-22
+    assert special.erfc(7.0) < 4.2e-23 and 2.0 / np.sqrt(np.pi) * np.exp(-49.0) < 6e-22
+
+
+def test_table_rows_continuous():
+    """Neighbouring intervals agree at their common end (t = 1 of row k, t = -1
+    of row k + 1) to an ulp or two: no row is misplaced or mis-scaled."""
+    tab = load_erfc()[:-1]
+    right = tab.sum(1)
+    left = (tab * (-1.0) ** np.arange(tab.shape[1])).sum(1)
+    assert np.abs(right[:-1] - left[1:]).max() <= 4e-16
 
 
 def test_exp2_table_correctly_rounded():
