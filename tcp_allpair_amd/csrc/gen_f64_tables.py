@@ -14,9 +14,9 @@ same row.  Maximum absolute error of the double Horner evaluation: erfc
 1.1e-16, (2/sqrt(pi)) exp(-x^2) 2.9e-16 (both about an ulp of values near 1;
 the reference's erfc, 1 - DERF(x), is itself good to about 1.1e-16 absolute).
 
-kExp2Tab: 2^(j/64), j = 0..63, correctly rounded, for exp(y) = 2^n 2^(j/64)
-e^r with y = (64 n + j) ln2/64 + r, |r| <= ln2/128, where e^r is the Taylor
-polynomial of degree 5 (remainder below r^6/720 < 4e-17).
+kExp2Tab: 2^(j/256), j = 0..255, correctly rounded, for exp(y) = 2^n 2^(j/256)
+e^r with y = (256 n + j) ln2/256 + r, |r| <= ln2/512, where e^r is the Taylor
+polynomial of degree 4 (remainder below r^5/120 < 4e-17).
 
 Both are checked by tests/test_f64_tables.py (against scipy.special.erfc,
 the exact derivative, and numpy's exp).
@@ -31,7 +31,7 @@ import mpmath as mp
 
 DEG = 9
 XMAX, PER_UNIT = 7, 16  # intervals of width 1/16 on [0, 7), then a row of zeros
-EXP2_N = 64
+EXP2_N, EXP2_BITS = 256, 8
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "f64_tables.h")
 
@@ -74,7 +74,7 @@ def main():
         "// kErfcTab: erfc(x) on interval k < kErfcN: sum_j kErfcTab[k][j] t^j,",
         f"// t = {2 * PER_UNIT} x - (2k + 1) in [-1, 1], intervals of width 1/{PER_UNIT} on [0, {XMAX});",
         f"// row kErfcN is zero (x >= {XMAX}).",
-        "// kExp2Tab: 2^(j/64), j = 0..63.",
+        "// kExp2Tab: 2^(j/256), j = 0..255.",
         "#pragma once",
         "",
         "namespace tcp {",
@@ -82,7 +82,8 @@ def main():
         f"constexpr int kErfcDeg = {DEG};",
         f"constexpr int kErfcPerUnit = {PER_UNIT};",
         f"constexpr int kErfcN = {XMAX * PER_UNIT};  // intervals of width 1/{PER_UNIT} on [0, {XMAX})",
-        f"constexpr int kExp2N = {EXP2_N};",
+        f"constexpr int kExp2Bits = {EXP2_BITS};",
+        f"constexpr int kExp2N = {EXP2_N};  // 1 << kExp2Bits",
         "",
         "__device__ __constant__ const double kErfcTab[kErfcN + 1][kErfcDeg + 1] = {",
     ]

@@ -1231,25 +1231,24 @@ __device__ inline double erfc_tab(double xs, const double* tab, double& dp)
 }
 
 // e^y for y <= 0 by Tang's table-driven reduction (ACM TOMS 15, 1989):
-// y = (64 n + j) ln2/64 + r with |r| <= ln2/128, e^y = 2^n 2^(j/64) e^r,
-// 2^(j/64) from the LDS copy of kExp2Tab and e^r - 1 by its Taylor polynomial
-// to r^5 (remainder below 4e-17); ln2/64 as a sum of two doubles keeps r to an
-// ulp.  About 1.5 ulp; underflows to 0 past -745.
+// y = (256 n + j) ln2/256 + r with |r| <= ln2/512, e^y = 2^n 2^(j/256) e^r,
+// 2^(j/256) from the LDS copy of kExp2Tab and e^r - 1 by its Taylor polynomial
+// to r^4 (remainder below r^5/120 < 4e-17); ln2/256 as a sum of two doubles
+// keeps r to an ulp.  About 1.5 ulp; underflows to 0 past -745.
 __device__ inline double exp_tab(double y, const double* e2tab)
 {
-    constexpr double k64_ln2 = 92.332482616893656;              // 64 / ln 2
-    constexpr double ln2_64_hi = 0x1.62e42fefa39efp-7;          // ln 2 / 64
-    constexpr double ln2_64_lo = 0x1.abc9e3b39803fp-62;         // ln 2 / 64 - ln2_64_hi
-    const double k = rint(y * k64_ln2);
-    const double r = fma(-k, ln2_64_lo, fma(-k, ln2_64_hi, y));
+    constexpr double k256_ln2 = 0x1.71547652b82fep+8;  // 256 / ln 2
+    constexpr double ln2_256_hi = 0x1.62e42fefa39efp-9;  // ln 2 / 256
+    constexpr double ln2_256_lo = 0x1.abc9e3b39803fp-64; // ln 2 / 256 - ln2_256_hi
+    const double k = rint(y * k256_ln2);
+    const double r = fma(-k, ln2_256_lo, fma(-k, ln2_256_hi, y));
     const int ki = (int)k;
-    double q = fma(r, 1.0 / 120.0, 1.0 / 24.0);
-    q = fma(q, r, 1.0 / 6.0);
+    double q = fma(r, 1.0 / 24.0, 1.0 / 6.0);
     q = fma(q, r, 0.5);
     q = fma(q, r, 1.0);
     q *= r;  // e^r - 1
     const double t = e2tab[ki & (kExp2N - 1)];
-    return ldexp(fma(t, q, t), ki >> 6);
+    return ldexp(fma(t, q, t), ki >> kExp2Bits);
 }
 
 // 1/sqrt(r2) to about one ulp: the hardware estimate and two Newton steps
@@ -1276,8 +1275,8 @@ struct __align__(32) F64J {
 // the i side only, half energy, RDF weight 1.  The cut-off test and the RDF bin
 // are the reference's bit for bit: minimum image and r^2 without contraction,
 // and bin = int(sqrt(r2) / dx) with both operations correctly rounded, which
-// r * (1/dx) (r = r2 / sqrt(r2) to an ulp or two) decides unless it lies within
-// 1e-12 of a bin edge; there the exact sqrt and division decide.
+// r * (1/dx) (r = r2 / sqrt(r2) to an ulp or two) decides unless its fraction
+// lies within 1e-12 of a bin edge; there the exact sqrt and division decide.
 // erfc(x) and the Gaussian term come from one row of the erfc table.
 template <bool kRdf, bool kKelbg, bool kDiag>
 __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const double* etab,
@@ -1324,8 +1323,8 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
             if (kRdf) {
                 const double q = r * (sj ? idx1 : idx0);
                 int b = (int)q;
-                const double fr = q - (double)b;
-                if (fr < 1e-12 || fr > 1.0 - 1e-12) b = (int)(sqrt(r2) / (sj ? dx1 : dx0));
+                if (fabs(__builtin_amdgcn_fract(q) - 0.5) > 0.5 - 1e-12)
+                    b = (int)(sqrt(r2) / (sj ? dx1 : dx0));
                 if (b < kNbin) atomicAdd(&hist[(si + sj) * kHistStride + b], kDiag ? 1u : 2u);
             }
             const double qq = qi * (sj ? qj1 : qj0);
@@ -1426,8 +1425,7 @@ __device__ inline void n3_tile_f64_uni(const PairArgs& a, unsigned* hist, const 
             if (kRdf) {
                 const double q = r * idxb;
                 int b = (int)q;
-                const double fr = q - (double)b;
-                if (fr < 1e-12 || fr > 1.0 - 1e-12) b = (int)(sqrt(r2) / dxb);
+                if (fabs(__builtin_amdgcn_fract(q) - 0.5) > 0.5 - 1e-12) b = (int)(sqrt(r2) / dxb);
                 if (b < kNbin) atomicAdd(&hrow[b], kDiag ? 1u : 2u);
             }
             double d;

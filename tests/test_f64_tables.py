@@ -4,7 +4,7 @@ erfc: its piecewise polynomials, evaluated with the kernel's interval mapping
 and one Horner pass for P and dP/dt (erfc_tab in kernels_pair.hip), against
 scipy.special.erfc, and the derivative against its closed form
 erfc'(x) = -(2/sqrt(pi)) exp(-x^2) (scipy/numpy), over the range the Ewald and
-Kelbg terms use; past x = 7 both are zero.  exp: the powers 2^(j/64) against
+Kelbg terms use; past x = 7 both are zero.  exp: the powers 2^(j/256) against
 mpmath at 40 digits, and the table-driven reduction of exp_tab (the same steps
 in numpy) against numpy's exp over the arguments the kernel meets (-r/lambda,
 down to underflow).
@@ -153,17 +153,18 @@ def test_exp2_table_correctly_rounded():
 
 
 def exp_tab(y, tab):
-    """The steps of exp_tab (kernels_pair.hip) in numpy.  The reduction's fused
-    multiply-add y - k ln2/64 is formed in extended precision (as the device's
-    fma forms it exactly); the polynomial without fma, a rounding or two looser."""
-    k = np.rint(y * 92.332482616893656)
-    hi = np.longdouble(float.fromhex("0x1.62e42fefa39efp-7"))
-    r = (np.longdouble(y) - np.longdouble(k) * hi).astype(np.float64)
-    r = r - k * float.fromhex("0x1.abc9e3b39803fp-62")
+    """The steps of exp_tab (kernels_pair.hip) in numpy, fused multiply-adds
+    as _fma forms them."""
+    k = np.rint(y * float.fromhex("0x1.71547652b82fep+8"))
+    r = _fma(-k, float.fromhex("0x1.62e42fefa39efp-9"), y)
+    r = _fma(-k, float.fromhex("0x1.abc9e3b39803fp-64"), r)
     ki = k.astype(np.int64)
-    q = ((((r * (1.0 / 120.0) + 1.0 / 24.0) * r + 1.0 / 6.0) * r + 0.5) * r + 1.0) * r
-    t = tab[ki & 63]
-    return np.ldexp(t + t * q, (ki >> 6).astype(np.int32))
+    q = _fma(r, 1.0 / 24.0, 1.0 / 6.0)
+    q = _fma(q, r, 0.5)
+    q = _fma(q, r, 1.0)
+    q = q * r
+    t = tab[ki & (len(tab) - 1)]
+    return np.ldexp(_fma(t, q, t), (ki >> 8).astype(np.int32))
 
 
 def test_exp_reduction_accuracy():
