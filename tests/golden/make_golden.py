@@ -4,7 +4,7 @@ Runs oracle/_ref/ref_harness -- data.f90, force.f90 and vverlet.f90 compiled
 from the reference sources by oracle/Makefile -- on seeded inputs and stores
 inputs and full-precision REAL8 outputs as compressed .npz (no pickles).
 
-    python tests/golden/make_golden.py          # needs oracle/_ref/ref_harness
+    python tests/golden/make_golden.py [case ...]   # needs oracle/_ref/ref_harness
 
 Cases (reference routine, file:line):
   force_tiny   Force_direct_self + EWALD_SELF, 24 e + 20 i        force.f90:4,52
@@ -18,6 +18,8 @@ Cases (reference routine, file:line):
   vv2_notemp / vv2_berend / vv2_isokin                             vverlet.f90:174,227,301
   traj_nve     40 steps of main.f90:87-153 without the reciprocal part, 250+250
   traj_isokin  20 steps, isokinetic thermostat
+  traj_c0      200 NVE steps at dt = 0.005 tau* of 1000 e + 1000 i (C0's size,
+               step and run length) without the reciprocal part
 """
 from __future__ import annotations
 
@@ -54,6 +56,17 @@ def save(name, p, inputs, out, mode, **extra):
     d.update(extra)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
     print("wrote", name)
+
+
+def traj_c0():
+    """C0's NVE run (1000 e + 1000 i at reduced density 1, dt = 0.005 tau*,
+    200 steps) through main.f90:87-153 without the reciprocal part (the
+    reference's SPME needs FFTE, which is not here)."""
+    dt = reduced_dt(0.005)
+    p, x, v = make_plasma(1000, 1000, seed=29)
+    f0 = rh.run(rh.MODE_FORCE, p, x, v)["f"]
+    out = rh.run(rh.MODE_TRAJ, p, x, v, f0, dt=dt, nsteps=200, thermo=NOTEMP)
+    save("traj_c0", p, dict(x=x, v=v, f=f0), out, rh.MODE_TRAJ, dt=dt, thermo=NOTEMP, nsteps=200)
 
 
 def main():
@@ -102,4 +115,11 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    cases = sys.argv[1:]
+    if not cases:
+        main()
+        traj_c0()
+    elif cases == ["traj_c0"]:
+        traj_c0()
+    else:
+        sys.exit("cases: none (all) or traj_c0")

@@ -251,6 +251,40 @@ def test_md_steps_vs_reference_trajectory(name):
     np.testing.assert_array_equal(vdf, d["out_vdf"])
 
 
+def test_c0_nve_energy_and_drift_vs_reference():
+    """C0's NVE run (1000 e + 1000 i, dt = 0.005 tau*, 200 steps; main.f90:87-153
+    without the reciprocal part, whose FFTE the reference build lacks) against
+    the reference's own run from the same state: every step's Ur, Epot, mv2el,
+    mv2ion to 1e-10 relative and E_tot to 1e-12 of |E_tot(0)| in FP64, the
+    energy drift (least-squares slope of E_tot per step over |E_tot(0)|, about
+    1.06e-9 here) to 1e-15 per step; mixed precision: E_tot to 1e-7 of
+    |E_tot(0)| at every step and the drift to 1e-11 per step (1% of it)."""
+    p, d = load("traj_c0")
+    p = p.replace(recip=RECIP_NONE)
+    ref = d["out_trace"]  # Ur, Epot, mv2el, mv2ion
+    e_ref = ref[:, 1] + 0.5 * (ref[:, 2] + ref[:, 3])
+
+    def drift(e):
+        return np.polyfit(np.arange(len(e)), e, 1)[0] / abs(e[0])
+
+    for prec in (FP64, MIXED):
+        say("case", "precision", prec)
+        with Context(p, prec) as c:
+            c.set_state(x=d["in_x"], v=d["in_v"], f=d["in_f"])
+            tr = c.md_steps(int(d["nsteps"]), NOTEMP, float(d["dt"]))
+            x = c.get_state()[0]
+        e = tr[:, 3] + 0.5 * (tr[:, 4] + tr[:, 5]) + tr[:, 2]
+        de = np.abs(e - e_ref).max() / abs(e_ref[0])
+        dd = abs(drift(e) - drift(e_ref))
+        say("max |dE_tot|/|E_tot(0)|", de, "drift", drift(e), "reference", drift(e_ref))
+        if prec == FP64:
+            np.testing.assert_allclose(tr[:, [0, 3, 4, 5]], ref, rtol=1e-10, atol=0)
+            assert de <= 1e-12 and dd <= 1e-15
+            np.testing.assert_allclose(x, d["out_x"], rtol=0, atol=1e-10)
+        else:
+            assert de <= 1e-7 and dd <= 1e-11
+
+
 @pytest.mark.parametrize("order,K", [(4, 32), (6, 32), (8, 16)])
 def test_spme_vs_oracle(order, K):
     say("case spme", order, K)

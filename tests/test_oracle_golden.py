@@ -164,9 +164,10 @@ def test_stoch2_closed_form():
     assert vdf.sum() == np.sum((mv2 / 3.0 / dv).astype(int) + 1 <= NBIN)
 
 
-@pytest.mark.parametrize("name", ["traj_nve", "traj_isokin"])
+@pytest.mark.parametrize("name", ["traj_nve", "traj_isokin", "traj_c0"])
 def test_trajectory(name):
-    """main.f90:87-153 without the reciprocal part, step by step against the reference."""
+    """main.f90:87-153 without the reciprocal part, step by step against the
+    reference (traj_c0: C0's 200 NVE steps of 2000 particles)."""
     p, d = load(name)
     p = p.replace(recip=0)
     x, v, f = (d[k].copy() for k in ("in_x", "in_v", "in_f"))
