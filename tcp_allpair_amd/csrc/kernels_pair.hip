@@ -1195,8 +1195,12 @@ __device__ inline void queue_done(int* queue)
 // wave order after each j tile, the i sums of a diagonal block join once every
 // j tile is done, so the result does not depend on scheduling.
 
-constexpr int kF64Waves = 8;
-constexpr int kFineTiles = 4;  // tiles per block side of the small-system shape
+constexpr int kF64Tiles = 8;   // tiles per block side
+constexpr int kF64Waves = 4;   // waves per 8 x 8 block: two i tiles each
+constexpr int kFineTiles = 4;  // tiles per block side of the small-system shape (a wave per i tile)
+
+template <int kBT>
+constexpr int f64_waves() { return kBT == kFineTiles ? kFineTiles : kF64Waves; }
 
 __device__ inline double rot_d(double v)
 {
@@ -1479,82 +1483,237 @@ __device__ inline void n3_tile_f64_uni(const PairArgs& a, unsigned* hist, const 
     rz *= eqq;
 }
 
-// LDS of one FP64 block: j positions and flags, the block's reaction sums
-// (slot br), the per-wave reaction buffer and the RDF histogram.
-template <int kBT>
+// n3_tile_f64_uni for two i tiles of one species against one j tile, off the
+// diagonal: each step's j coordinates and the rotation of the reactions serve
+// both pairs, and each pair is the single-tile path's arithmetic.
+template <bool kRdf, bool kKelbg>
+__device__ inline void n3_tile_f64_uni2(const PairArgs& a, unsigned* hist, const double* etab,
+                                       const double* e2tab, const double (&xi)[2],
+                                       const double (&yi)[2], const double (&zi)[2], int si,
+                                       int sj, const F64J* jt, double (&fxo)[2],
+                                       double (&fyo)[2], double (&fzo)[2], double (&uo)[2],
+                                       double& rx, double& ry, double& rz)
+{
+    constexpr int kNI = 2;
+    constexpr bool kDiag = false;
+    const char* jb = (const char*)jt;
+    uint32_t off = (uint32_t)(threadIdx.x % kTile) * (uint32_t)sizeof(F64J);
+    double qx = *(const double*)(jb + off), qy = *(const double*)(jb + off + 8),
+           qz = *(const double*)(jb + off + 16);
+    const double Lx = a.box[0], Ly = a.box[1], Lz = a.box[2];
+    const double iLx = a.inv_box[0], iLy = a.inv_box[1], iLz = a.inv_box[2];
+    const double alpha32 = 32.0 * a.alpha, rc2 = a.rc2;
+    const double inv_rs = 1.0 / a.rs, inv_rs32 = 32.0 / a.rs;
+    const int ht = si + sj;
+    const double dxb = a.dx[ht], idxb = a.inv_dx[ht];
+    const bool reg = sj == 0 ? (si != 0 || (a.flags & TCP_REG_EE) != 0) : si == 0;
+    unsigned* hrow = hist + ht * kHistStride;
+    double fx[kNI], fy[kNI], fz[kNI], u[kNI];
+#pragma unroll
+    for (int h = 0; h < kNI; ++h) fx[h] = fy[h] = fz[h] = u[h] = 0.0;
+    rx = ry = rz = 0.0;
+#pragma unroll 2
+    for (int s = 0; s < kTile; ++s) {  // by two: the carried offset and j alternate registers
+        const uint32_t noff = rot_u(off);
+        const double nx = *(const double*)(jb + noff), ny = *(const double*)(jb + noff + 8),
+                     nz = *(const double*)(jb + noff + 16);
+#pragma unroll
+        for (int h = 0; h < kNI; ++h) {
+            double xr, yr, zr, r2;
+            {
+#pragma clang fp contract(off)
+                const double dx = xi[h] - qx;
+                const double dy = yi[h] - qy;
+                const double dz = zi[h] - qz;
+                xr = fma(-Lx, rint(dx * iLx), dx);
+                yr = fma(-Ly, rint(dy * iLy), dy);
+                zr = fma(-Lz, rint(dz * iLz), dz);
+                r2 = xr * xr + yr * yr + zr * zr;
+            }
+            if (r2 < rc2 && r2 > 0.0) {
+                const double inv_r = rsq_f64(r2);
+                const double r = r2 * inv_r;
+                if (kRdf) {
+                    const double q = r * idxb;
+                    int b = (int)q;
+                    if (fabs(__builtin_amdgcn_fract(q) - 0.5) > 0.5 - 1e-12) b = (int)(sqrt(r2) / dxb);
+                    if (b < kNbin) atomicAdd(&hrow[b], kDiag ? 1u : 2u);
+                }
+                double d;
+                const double xs = alpha32 * r;
+                const double ec = erfc_tab(xs, etab, d);
+                double num = fma(-xs, d, ec), en = ec;
+                if (reg) {
+                    if (kKelbg) {
+                        double ds;
+                        const double ss = r * inv_rs32;
+                        const double es = erfc_tab(ss, etab, ds);
+                        const double y2 = ds * (-16.0 * kSqrtPi);
+                        num -= y2;
+                        en += fma((kSqrtPi / 32.0) * ss, es, -y2);
+                    } else {
+                        const double sr = r * inv_rs;
+                        const double y = exp_tab(-sr, e2tab);
+                        num -= y * (1.0 + sr);
+                        en -= y;
+                    }
+                }
+                const double df = num * (inv_r * inv_r) * inv_r;
+                u[h] = fma(en, inv_r, u[h]);
+                fx[h] = fma(df, xr, fx[h]);
+                fy[h] = fma(df, yr, fy[h]);
+                fz[h] = fma(df, zr, fz[h]);
+                if (!kDiag) {
+                    rx = fma(-df, xr, rx);
+                    ry = fma(-df, yr, ry);
+                    rz = fma(-df, zr, rz);
+                }
+            }
+        }
+        off = noff;
+        qx = nx;
+        qy = ny;
+        qz = nz;
+        if (!kDiag) {
+            rx = rot_d(rx);
+            ry = rot_d(ry);
+            rz = rot_d(rz);
+        }
+    }
+    const double qq = a.q[si] * a.q[sj];
+    const double eqq = a.eps * qq;
+#pragma unroll
+    for (int h = 0; h < kNI; ++h) {
+        fxo[h] = eqq * fx[h];
+        fyo[h] = eqq * fy[h];
+        fzo[h] = eqq * fz[h];
+        uo[h] = (kDiag ? 0.5 * qq : qq) * u[h];
+    }
+    rx *= eqq;
+    ry *= eqq;
+    rz *= eqq;
+}
+
+// LDS of one FP64 block of kBT x kBT tiles worked by kW waves: j positions and
+// flags, the block's reaction sums (slot br), the per-wave reaction buffer and
+// the RDF histogram.
+template <int kBT, int kW>
 struct F64Lds {
     F64J jp[kBT * kTile];
-    double racc[4][kBT * kTile];
-    double wbuf[kBT][3][kTile];
+    double racc[3][kBT * kTile];
+    double wbuf[kW][3][kTile];
     unsigned hist[3 * kHistStride];
     double etab[(kErfcN + 1) * (kErfcDeg + 1)] __attribute__((aligned(16)));
     double e2tab[kExp2N];
 };
 
-template <int kBT>
-__device__ inline void load_tables(F64Lds<kBT>& L)
+template <int kBT, int kW>
+__device__ inline void load_tables(F64Lds<kBT, kW>& L)
 {
     const double* src = &kErfcTab[0][0];
     for (int i = threadIdx.x; i < (kErfcN + 1) * (kErfcDeg + 1); i += blockDim.x) L.etab[i] = src[i];
     for (int i = threadIdx.x; i < kExp2N; i += blockDim.x) L.e2tab[i] = kExp2Tab[i];
 }
 
-// kBT x kBT tiles, one i tile per wave (kBT waves)
-template <bool kRdf, bool kKelbg, int kBT>
-__device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds<kBT>& L)
+// kBT x kBT tiles by kW waves, kNI = kBT / kW i tiles per wave: wave w holds
+// the block's i tiles w, w + kW, ..  For each j tile the wave's i tiles walk
+// it together when the tile pairs are uniform and off the diagonal (every
+// step's j read and reaction rotation serving kNI pairs), else one after the
+// other, their reactions added in i-tile order; then the waves' reactions for
+// the j tile are summed in wave order.  A diagonal block adds its i sums to
+// those after the last j tile.  Every sum has a fixed order.
+template <bool kRdf, bool kKelbg, int kBT, int kW>
+__device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds<kBT, kW>& L)
 {
     constexpr int kBP = kBT * kTile;
+    constexpr int kNI = kBT / kW;
+    static_assert(kBT % kW == 0 && kNI <= 2, "one or two i tiles per wave");
     const bool dblk = br == bc;
     const int n = a.n_i, ne = a.ne_i;
     for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
         const int jp = bc * kBP + p;
         const int jc = min(jp, n - 1);
         L.jp[p] = F64J{a.pi.x[jc], a.pi.y[jc], a.pi.z[jc], (jp >= ne ? 1 : 0) | (jp < n ? 2 : 0), 0};
-        L.racc[0][p] = L.racc[1][p] = L.racc[2][p] = L.racc[3][p] = 0.0;
     }
     if (kRdf) hist_clear(L.hist);
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile), lane = threadIdx.x % kTile;
     const int ntiles = (n + kTile - 1) / kTile;
-    const int I = br * kBT + wave;
-    const bool exists = I < ntiles;
-    const int ip = I * kTile + lane;
-    const bool ilive = exists && ip < n;
-    const int ic = min(ip, n - 1);
-    const double xi = a.pi.x[ic], yi = a.pi.y[ic], zi = a.pi.z[ic];
-    const int si = ip >= ne;
     // a full tile of one species
     auto uniform = [&](int T) {
         const int p0 = T * kTile, p1 = p0 + kTile - 1;
         return p1 < n && (p0 >= ne || p1 < ne);
     };
-    const bool iuni = uniform(I);
-    const int siu = I * kTile >= ne;
-    double Fx = 0.0, Fy = 0.0, Fz = 0.0, U = 0.0;
+    int tl[kNI], ip[kNI], si[kNI], siu[kNI];
+    bool exists[kNI], ilive[kNI], iuni[kNI];
+    double xi[kNI], yi[kNI], zi[kNI], Fx[kNI], Fy[kNI], Fz[kNI], U[kNI];
+#pragma unroll
+    for (int h = 0; h < kNI; ++h) {
+        tl[h] = wave + h * kW;  // i tile within the block
+        const int I = br * kBT + tl[h];
+        exists[h] = I < ntiles;
+        ip[h] = I * kTile + lane;
+        ilive[h] = exists[h] && ip[h] < n;
+        const int ic = min(ip[h], n - 1);
+        xi[h] = a.pi.x[ic];
+        yi[h] = a.pi.y[ic];
+        zi[h] = a.pi.z[ic];
+        si[h] = ip[h] >= ne;
+        iuni[h] = uniform(I);
+        siu[h] = I * kTile >= ne;
+        Fx[h] = Fy[h] = Fz[h] = U[h] = 0.0;
+    }
     for (int tj = 0; tj < kBT; ++tj) {
         const int J = bc * kBT + tj;
+        const F64J* jt = L.jp + tj * kTile;
+        const bool juni = J < ntiles && uniform(J);
+        const int sju = J * kTile >= ne;
         double rx = 0.0, ry = 0.0, rz = 0.0;
-        if (exists && J < ntiles && (!dblk || tj >= wave)) {
-            const F64J* jt = L.jp + tj * kTile;
-            double fx, fy, fz, u;
-            const int sju = J * kTile >= ne;
-            if (iuni && uniform(J)) {
-                if (dblk && tj == wave)
-                    n3_tile_f64_uni<kRdf, kKelbg, true>(a, L.hist, L.etab, L.e2tab, xi, yi, zi, siu,
-                                                        sju, jt, fx, fy, fz, u, rx, ry, rz);
+        bool both = kNI == 2 && !dblk && juni;
+#pragma unroll
+        for (int h = 0; h < kNI; ++h) both = both && exists[h] && iuni[h] && siu[h] == siu[0];
+        if (both) {
+            double fx[2], fy[2], fz[2], u[2];
+            const double x2[2] = {xi[0], xi[kNI - 1]}, y2[2] = {yi[0], yi[kNI - 1]},
+                         z2[2] = {zi[0], zi[kNI - 1]};
+            n3_tile_f64_uni2<kRdf, kKelbg>(a, L.hist, L.etab, L.e2tab, x2, y2, z2, siu[0], sju, jt,
+                                           fx, fy, fz, u, rx, ry, rz);
+#pragma unroll
+            for (int h = 0; h < kNI; ++h) {
+                Fx[h] += fx[h];
+                Fy[h] += fy[h];
+                Fz[h] += fz[h];
+                U[h] += u[h];
+            }
+        } else {
+#pragma unroll
+            for (int h = 0; h < kNI; ++h) {
+                if (!(exists[h] && J < ntiles && (!dblk || tj >= tl[h]))) continue;
+                double fx, fy, fz, u, hx, hy, hz;
+                const bool diag = dblk && tj == tl[h];
+                if (iuni[h] && juni) {
+                    if (diag)
+                        n3_tile_f64_uni<kRdf, kKelbg, true>(a, L.hist, L.etab, L.e2tab, xi[h], yi[h],
+                                                            zi[h], siu[h], sju, jt, fx, fy, fz, u, hx,
+                                                            hy, hz);
+                    else
+                        n3_tile_f64_uni<kRdf, kKelbg, false>(a, L.hist, L.etab, L.e2tab, xi[h], yi[h],
+                                                             zi[h], siu[h], sju, jt, fx, fy, fz, u, hx,
+                                                             hy, hz);
+                } else if (diag)
+                    n3_tile_f64<kRdf, kKelbg, true>(a, L.hist, L.etab, L.e2tab, xi[h], yi[h], zi[h],
+                                                    si[h], ilive[h], jt, fx, fy, fz, u, hx, hy, hz);
                 else
-                    n3_tile_f64_uni<kRdf, kKelbg, false>(a, L.hist, L.etab, L.e2tab, xi, yi, zi,
-                                                         siu, sju, jt, fx, fy, fz, u, rx, ry, rz);
-            } else if (dblk && tj == wave)
-                n3_tile_f64<kRdf, kKelbg, true>(a, L.hist, L.etab, L.e2tab, xi, yi, zi, si, ilive,
-                                                jt, fx, fy, fz, u, rx, ry, rz);
-            else
-                n3_tile_f64<kRdf, kKelbg, false>(a, L.hist, L.etab, L.e2tab, xi, yi, zi, si, ilive,
-                                                 jt, fx, fy, fz, u, rx, ry, rz);
-            Fx += fx;
-            Fy += fy;
-            Fz += fz;
-            U += u;
+                    n3_tile_f64<kRdf, kKelbg, false>(a, L.hist, L.etab, L.e2tab, xi[h], yi[h], zi[h],
+                                                     si[h], ilive[h], jt, fx, fy, fz, u, hx, hy, hz);
+                Fx[h] += fx;
+                Fy[h] += fy;
+                Fz[h] += fz;
+                U[h] += u;
+                rx += hx;
+                ry += hy;
+                rz += hz;
+            }
         }
         L.wbuf[wave][0][lane] = rx;
         L.wbuf[wave][1][lane] = ry;
@@ -1562,26 +1721,30 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds<kB
         __syncthreads();
         for (int t = threadIdx.x; t < 3 * kTile; t += blockDim.x) {
             const int c = t / kTile, p = t % kTile;
-            double s = L.racc[c][tj * kTile + p];
-            for (int w = 0; w < kBT; ++w) s += L.wbuf[w][c][p];
+            double s = 0.0;
+            for (int w = 0; w < kW; ++w) s += L.wbuf[w][c][p];
             L.racc[c][tj * kTile + p] = s;
         }
         __syncthreads();
     }
-    if (dblk) {
-        if (exists) {
-            const int q = wave * kTile + lane;
-            L.racc[0][q] += Fx;
-            L.racc[1][q] += Fy;
-            L.racc[2][q] += Fz;
-            L.racc[3][q] += U;
+    // the i sums: slot bc, or on the diagonal added to the reactions (slot br;
+    // the energy slots below the diagonal hold zeros and are not stored)
+#pragma unroll
+    for (int h = 0; h < kNI; ++h) {
+        if (!exists[h]) continue;
+        if (dblk) {
+            const int q = tl[h] * kTile + lane;
+            L.racc[0][q] += Fx[h];
+            L.racc[1][q] += Fy[h];
+            L.racc[2][q] += Fz[h];
+            if (ilive[h]) a.pu[(size_t)br * n + ip[h]] = U[h];
+        } else if (ilive[h]) {
+            double* pf = a.pf + (size_t)bc * 3 * n;
+            pf[ip[h]] = Fx[h];
+            pf[n + ip[h]] = Fy[h];
+            pf[2 * n + ip[h]] = Fz[h];
+            a.pu[(size_t)bc * n + ip[h]] = U[h];
         }
-    } else if (ilive) {
-        double* pf = a.pf + (size_t)bc * 3 * n;
-        pf[ip] = Fx;
-        pf[n + ip] = Fy;
-        pf[2 * n + ip] = Fz;
-        a.pu[(size_t)bc * n + ip] = U;
     }
     __syncthreads();
     for (int p = threadIdx.x; p < kBP; p += blockDim.x) {
@@ -1591,7 +1754,6 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds<kB
         pf[jp] = L.racc[0][p];
         pf[n + jp] = L.racc[1][p];
         pf[2 * n + jp] = L.racc[2][p];
-        if (dblk) a.pu[(size_t)br * n + jp] = L.racc[3][p];  // zero off the diagonal: not stored
     }
     if (kRdf) {
         __syncthreads();
@@ -1599,56 +1761,58 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds<kB
     }
 }
 
-// four waves per SIMD (at most 128 VGPRs): two resident 8-wave blocks per CU
+// 8 x 8 tile blocks by four waves, two i tiles each (51 KB of LDS, at most 168
+// VGPRs: three blocks per CU, three waves per SIMD, each with two independent
+// pair chains)
 template <bool kRdf, bool kKelbg>
-__global__ __launch_bounds__(kF64Waves * kTile, 4) void pair_f64_n3_kernel(PairArgs a,
+__global__ __launch_bounds__(kF64Waves * kTile, 3) void pair_f64_n3_kernel(PairArgs a,
                                                                         const int2* blkmap,
                                                                         int nblk, int* queue)
 {
-    __shared__ F64Lds<kF64Waves> lds;
+    __shared__ F64Lds<kF64Tiles, kF64Waves> lds;
     __shared__ int slot;
     load_tables(lds);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nblk) break;
         const int2 rc = blkmap[it];
-        n3_block_f64<kRdf, kKelbg, kF64Waves>(a, rc.x, rc.y, lds);
+        n3_block_f64<kRdf, kKelbg, kF64Tiles, kF64Waves>(a, rc.x, rc.y, lds);
     }
     queue_done(queue);
 }
 
-// The same over blocks of 4 x 4 tiles by four waves, for small systems (47 KB
-// of LDS: three blocks, three waves per SIMD).
+// The same over blocks of 4 x 4 tiles by four waves, one i tile each, for small
+// systems (three blocks per CU, three waves per SIMD).
 template <bool kRdf, bool kKelbg>
 __global__ __launch_bounds__(kFineTiles * kTile, 3) void pair_f64_n3_fine_kernel(PairArgs a,
                                                                               const int2* blkmap,
                                                                               int nblk, int* queue)
 {
-    __shared__ F64Lds<kFineTiles> lds;
+    __shared__ F64Lds<kFineTiles, kFineTiles> lds;
     __shared__ int slot;
     load_tables(lds);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nblk) break;
         const int2 rc = blkmap[it];
-        n3_block_f64<kRdf, kKelbg, kFineTiles>(a, rc.x, rc.y, lds);
+        n3_block_f64<kRdf, kKelbg, kFineTiles, kFineTiles>(a, rc.x, rc.y, lds);
     }
     queue_done(queue);
 }
 
 template <bool kRdf, bool kKelbg, int kBT>
-__global__ __launch_bounds__(kBT * kTile, kBT == kFineTiles ? 3 : 4) void pair_f64_n3_batch_kernel(const BatchRep* reps,
+__global__ __launch_bounds__(f64_waves<kBT>() * kTile, 3) void pair_f64_n3_batch_kernel(const BatchRep* reps,
                                                                         const PairItem* items,
                                                                         int nitems, int* queue)
 {
-    __shared__ F64Lds<kBT> lds;
+    __shared__ F64Lds<kBT, f64_waves<kBT>()> lds;
     __shared__ int slot;
     load_tables(lds);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nitems) break;
         const PairItem w = items[it];
-        n3_block_f64<kRdf, kKelbg, kBT>(reps[w.rep].pa, w.a, w.b, lds);
+        n3_block_f64<kRdf, kKelbg, kBT, f64_waves<kBT>()>(reps[w.rep].pa, w.a, w.b, lds);
     }
     queue_done(queue);
 }
@@ -1905,7 +2069,7 @@ template <int kBT>
 void launch_f64_batch_bt(const BatchRep* reps, const PairItem* items, int nitems, int* queue,
                          bool rdf, bool kelbg, int grid, hipStream_t s)
 {
-    const dim3 g(grid), b(kBT * kTile);
+    const dim3 g(grid), b(f64_waves<kBT>() * kTile);
     if (rdf) {
         if (kelbg) hipLaunchKernelGGL((pair_f64_n3_batch_kernel<true, true, kBT>), g, b, 0, s, reps, items, nitems, queue);
         else hipLaunchKernelGGL((pair_f64_n3_batch_kernel<true, false, kBT>), g, b, 0, s, reps, items, nitems, queue);
@@ -1930,7 +2094,7 @@ void launch_pair_f64_batch(const BatchRep* reps, const PairItem* items, int nite
                            bool rdf, bool kelbg, int grid, int shape, hipStream_t s)
 {
     if (shape == kFineN3Shape) launch_f64_batch_bt<kFineTiles>(reps, items, nitems, queue, rdf, kelbg, grid, s);
-    else launch_f64_batch_bt<kF64Waves>(reps, items, nitems, queue, rdf, kelbg, grid, s);
+    else launch_f64_batch_bt<kF64Tiles>(reps, items, nitems, queue, rdf, kelbg, grid, s);
 }
 
 int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg)
@@ -1943,7 +2107,7 @@ int pair_batch_occupancy(bool mixed, int shape, bool rdf, bool kelbg)
             nb = 1;
     } else {
         const void* f = shape == kFineN3Shape ? f64_batch_fn<kFineTiles>(rdf, kelbg)
-                                              : f64_batch_fn<kF64Waves>(rdf, kelbg);
+                                              : f64_batch_fn<kF64Tiles>(rdf, kelbg);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, f64_threads(shape), 0) != hipSuccess)
             nb = 1;
     }
