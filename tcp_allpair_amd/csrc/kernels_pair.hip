@@ -1209,26 +1209,40 @@ __device__ inline double rot_d(double v)
     return __longlong_as_double((long long)((hi << 32) | lo));
 }
 
-// erfc(x), x >= 0, from the LDS copy of kErfcTab, given xs = 32 x, and in dp
-// the derivative dP/dt of the interval's polynomial: erfc'(x) = 32 dP/dt =
-// -(2/sqrt(pi)) e^{-x^2}, so the Gaussian term of the Ewald force,
-// (2x/sqrt(pi)) e^{-x^2} = -xs dP/dt, and Kelbg's e^{-s^2} need no exponential.
-// The interval is k = floor(xs/2) (the zero row kErfcN for x >= 7; xs is
-// clamped before the conversion, so any finite x is safe) and t = xs - (2k + 1)
-// in [-1, 1), both exact; the row offset is a 24-bit multiply (full rate; a
-// 32-bit one issues at quarter rate).  P and dP/dt come from one Horner pass.
-__device__ inline double erfc_tab(double xs, const double* tab, double& dp)
+// The LDS copy of kErfcTab by coefficient pairs: pair j (coefficients 2j and
+// 2j + 1) of row k at [j][k], 16 bytes apart from row to row.  A lane's four
+// 16-byte reads of its row then fall on bank groups spread by its row index,
+// where rows 64 bytes apart would put every lane's read of one pair on a
+// quarter of the banks.
+constexpr int kErfcPairs = (kErfcDeg + 1) / 2;
+static_assert((kErfcDeg + 1) % 2 == 0, "the erfc rows split into coefficient pairs");
+struct __align__(16) ErfcPair {
+    double lo, hi;  // coefficients 2j and 2j + 1
+};
+
+// erfc(x), x >= 0, from the LDS copy of kErfcTab, given xs = kErfcScale x
+// (64 x), and in dp the derivative dP/dt of the interval's polynomial:
+// erfc'(x) = kErfcScale dP/dt = -(2/sqrt(pi)) e^{-x^2}, so the Gaussian term of
+// the Ewald force, (2x/sqrt(pi)) e^{-x^2} = -xs dP/dt, and Kelbg's e^{-s^2}
+// need no exponential.  The interval is k = floor(xs/2) (the zero row kErfcN
+// for x >= 6; xs is clamped before the conversion, so any finite x is safe) and
+// t = xs - (2k + 1) in [-1, 1), both exact.  P and dP/dt come from one Horner
+// pass (degree 7: 13 fused multiply-adds).
+__device__ inline double erfc_tab(double xs, const ErfcPair* tab, double& dp)
 {
     const int ki = (int)fmin(xs, (double)(2 * kErfcN + 1));
     const double t = xs - (double)(ki | 1);
-    const double* c = tab + __umul24((unsigned)(ki >> 1), kErfcDeg + 1);
-    double p = c[kErfcDeg];
+    const ErfcPair* row = tab + (ki >> 1);
+    ErfcPair c[kErfcPairs];
+#pragma unroll
+    for (int j = 0; j < kErfcPairs; ++j) c[j] = row[j * (kErfcN + 1)];
+    double p = c[kErfcPairs - 1].hi;
     double d = p;
-    p = fma(p, t, c[kErfcDeg - 1]);
+    p = fma(p, t, c[kErfcPairs - 1].lo);
 #pragma unroll
     for (int j = kErfcDeg - 2; j >= 0; --j) {
         d = fma(d, t, p);
-        p = fma(p, t, c[j]);
+        p = fma(p, t, (j & 1) ? c[j >> 1].hi : c[j >> 1].lo);
     }
     dp = d;
     return p;
@@ -1236,7 +1250,8 @@ __device__ inline double erfc_tab(double xs, const double* tab, double& dp)
 
 // e^y for y <= 0 by Tang's table-driven reduction (ACM TOMS 15, 1989):
 // y = (256 n + j) ln2/256 + r with |r| <= ln2/512, e^y = 2^n 2^(j/256) e^r,
-// 2^(j/256) from the LDS copy of kExp2Tab and e^r - 1 by its Taylor polynomial
+// 2^(j/256) from kExp2Tab (read through the vector cache: only the Deutsch
+// form calls this, so the table stays out of LDS) and e^r - 1 by its Taylor polynomial
 // to r^4 (remainder below r^5/120 < 4e-17); ln2/256 as a sum of two doubles
 // keeps r to an ulp.  About 1.5 ulp; underflows to 0 past -745.
 __device__ inline double exp_tab(double y, const double* e2tab)
@@ -1283,7 +1298,7 @@ struct __align__(32) F64J {
 // lies within 1e-12 of a bin edge; there the exact sqrt and division decide.
 // erfc(x) and the Gaussian term come from one row of the erfc table.
 template <bool kRdf, bool kKelbg, bool kDiag>
-__device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const double* etab,
+__device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const ErfcPair* etab,
                                    const double* e2tab, double xi, double yi, double zi, int si,
                                    bool ilive, const F64J* jt, double& fxo, double& fyo,
                                    double& fzo, double& uo, double& rx, double& ry, double& rz)
@@ -1293,8 +1308,8 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
     F64J q = *(const F64J*)(jb + off);
     const double Lx = a.box[0], Ly = a.box[1], Lz = a.box[2];
     const double iLx = a.inv_box[0], iLy = a.inv_box[1], iLz = a.inv_box[2];
-    const double alpha32 = 32.0 * a.alpha, rc2 = a.rc2;
-    const double inv_rs = 1.0 / a.rs, inv_rs32 = 32.0 / a.rs;
+    const double alpha_s = kErfcScale * a.alpha, rc2 = a.rc2;
+    const double inv_rs = 1.0 / a.rs, inv_rs_s = kErfcScale / a.rs;
     // per-i constants for a j electron (0) / a j ion (1)
     const double qi = si ? a.q[1] : a.q[0];
     const double eqi = a.eps * qi;
@@ -1335,18 +1350,18 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
             const double inv_r2 = inv_r * inv_r;
             // num = erfc(x) + (2x/sqrt(pi)) e^{-x^2}, x = alpha r
             double d;
-            const double xs = alpha32 * r;
+            const double xs = alpha_s * r;
             const double ec = erfc_tab(xs, etab, d);
             double num = fma(-xs, d, ec), en = ec;
             if (sj ? reg1 : reg0) {
                 if (kKelbg) {
                     // R = -e^{-s^2} + sqrt(pi) s erfc(s), s = r/lambda
                     double ds;
-                    const double ss = r * inv_rs32;
+                    const double ss = r * inv_rs_s;
                     const double es = erfc_tab(ss, etab, ds);
-                    const double y2 = ds * (-16.0 * kSqrtPi);
+                    const double y2 = ds * (-0.5 * kErfcScale * kSqrtPi);
                     num -= y2;
-                    en += fma((kSqrtPi / 32.0) * ss, es, -y2);
+                    en += fma((kSqrtPi / kErfcScale) * ss, es, -y2);
                 } else {
                     const double sr = r * inv_rs;
                     // R = -e^{-s} (Deutsch, force.f90:136-139)
@@ -1388,7 +1403,7 @@ __device__ inline void n3_tile_f64(const PairArgs& a, unsigned* hist, const doub
 // and difference while L rint(dx/L) is exact (|dx| < 2.5 L; wrapped positions
 // have |dx| <= L), so r^2, the cut-off and the RDF bins stay the reference's.
 template <bool kRdf, bool kKelbg, bool kDiag>
-__device__ inline void n3_tile_f64_uni(const PairArgs& a, unsigned* hist, const double* etab,
+__device__ inline void n3_tile_f64_uni(const PairArgs& a, unsigned* hist, const ErfcPair* etab,
                                        const double* e2tab, double xi, double yi, double zi,
                                        int si, int sj, const F64J* jt, double& fxo, double& fyo,
                                        double& fzo, double& uo, double& rx, double& ry, double& rz)
@@ -1399,8 +1414,8 @@ __device__ inline void n3_tile_f64_uni(const PairArgs& a, unsigned* hist, const 
            qz = *(const double*)(jb + off + 16);
     const double Lx = a.box[0], Ly = a.box[1], Lz = a.box[2];
     const double iLx = a.inv_box[0], iLy = a.inv_box[1], iLz = a.inv_box[2];
-    const double alpha32 = 32.0 * a.alpha, rc2 = a.rc2;
-    const double inv_rs = 1.0 / a.rs, inv_rs32 = 32.0 / a.rs;
+    const double alpha_s = kErfcScale * a.alpha, rc2 = a.rc2;
+    const double inv_rs = 1.0 / a.rs, inv_rs_s = kErfcScale / a.rs;
     const int ht = si + sj;
     const double dxb = a.dx[ht], idxb = a.inv_dx[ht];
     const bool reg = sj == 0 ? (si != 0 || (a.flags & TCP_REG_EE) != 0) : si == 0;
@@ -1433,17 +1448,17 @@ __device__ inline void n3_tile_f64_uni(const PairArgs& a, unsigned* hist, const 
                 if (b < kNbin) atomicAdd(&hrow[b], kDiag ? 1u : 2u);
             }
             double d;
-            const double xs = alpha32 * r;
+            const double xs = alpha_s * r;
             const double ec = erfc_tab(xs, etab, d);
             double num = fma(-xs, d, ec), en = ec;
             if (reg) {
                 if (kKelbg) {
                     double ds;
-                    const double ss = r * inv_rs32;
+                    const double ss = r * inv_rs_s;
                     const double es = erfc_tab(ss, etab, ds);
-                    const double y2 = ds * (-16.0 * kSqrtPi);
+                    const double y2 = ds * (-0.5 * kErfcScale * kSqrtPi);
                     num -= y2;
-                    en += fma((kSqrtPi / 32.0) * ss, es, -y2);
+                    en += fma((kSqrtPi / kErfcScale) * ss, es, -y2);
                 } else {
                     const double sr = r * inv_rs;
                     const double y = exp_tab(-sr, e2tab);
@@ -1487,7 +1502,7 @@ __device__ inline void n3_tile_f64_uni(const PairArgs& a, unsigned* hist, const 
 // diagonal: each step's j coordinates and the rotation of the reactions serve
 // both pairs, and each pair is the single-tile path's arithmetic.
 template <bool kRdf, bool kKelbg>
-__device__ inline void n3_tile_f64_uni2(const PairArgs& a, unsigned* hist, const double* etab,
+__device__ inline void n3_tile_f64_uni2(const PairArgs& a, unsigned* hist, const ErfcPair* etab,
                                        const double* e2tab, const double (&xi)[2],
                                        const double (&yi)[2], const double (&zi)[2], int si,
                                        int sj, const F64J* jt, double (&fxo)[2],
@@ -1502,8 +1517,8 @@ __device__ inline void n3_tile_f64_uni2(const PairArgs& a, unsigned* hist, const
            qz = *(const double*)(jb + off + 16);
     const double Lx = a.box[0], Ly = a.box[1], Lz = a.box[2];
     const double iLx = a.inv_box[0], iLy = a.inv_box[1], iLz = a.inv_box[2];
-    const double alpha32 = 32.0 * a.alpha, rc2 = a.rc2;
-    const double inv_rs = 1.0 / a.rs, inv_rs32 = 32.0 / a.rs;
+    const double alpha_s = kErfcScale * a.alpha, rc2 = a.rc2;
+    const double inv_rs = 1.0 / a.rs, inv_rs_s = kErfcScale / a.rs;
     const int ht = si + sj;
     const double dxb = a.dx[ht], idxb = a.inv_dx[ht];
     const bool reg = sj == 0 ? (si != 0 || (a.flags & TCP_REG_EE) != 0) : si == 0;
@@ -1540,17 +1555,17 @@ __device__ inline void n3_tile_f64_uni2(const PairArgs& a, unsigned* hist, const
                     if (b < kNbin) atomicAdd(&hrow[b], kDiag ? 1u : 2u);
                 }
                 double d;
-                const double xs = alpha32 * r;
+                const double xs = alpha_s * r;
                 const double ec = erfc_tab(xs, etab, d);
                 double num = fma(-xs, d, ec), en = ec;
                 if (reg) {
                     if (kKelbg) {
                         double ds;
-                        const double ss = r * inv_rs32;
+                        const double ss = r * inv_rs_s;
                         const double es = erfc_tab(ss, etab, ds);
-                        const double y2 = ds * (-16.0 * kSqrtPi);
+                        const double y2 = ds * (-0.5 * kErfcScale * kSqrtPi);
                         num -= y2;
-                        en += fma((kSqrtPi / 32.0) * ss, es, -y2);
+                        en += fma((kSqrtPi / kErfcScale) * ss, es, -y2);
                     } else {
                         const double sr = r * inv_rs;
                         const double y = exp_tab(-sr, e2tab);
@@ -1595,24 +1610,28 @@ __device__ inline void n3_tile_f64_uni2(const PairArgs& a, unsigned* hist, const
 }
 
 // LDS of one FP64 block of kBT x kBT tiles worked by kW waves: j positions and
-// flags, the block's reaction sums (slot br), the per-wave reaction buffer and
-// the RDF histogram.
+// flags, the block's reaction sums (slot br), the per-wave reaction buffer, the
+// RDF histogram and the erfc table (52 KiB for 8 x 8 tiles: three blocks per
+// CU).
 template <int kBT, int kW>
 struct F64Lds {
     F64J jp[kBT * kTile];
     double racc[3][kBT * kTile];
     double wbuf[kW][3][kTile];
     unsigned hist[3 * kHistStride];
-    double etab[(kErfcN + 1) * (kErfcDeg + 1)] __attribute__((aligned(16)));
-    double e2tab[kExp2N];
+    ErfcPair etab[kErfcPairs * (kErfcN + 1)];
 };
+
+static_assert(sizeof(F64Lds<kF64Tiles, kF64Waves>) + 64 <= 160 * 1024 / 3,
+              "three 8 x 8 FP64 blocks fit in one CU's LDS");
 
 template <int kBT, int kW>
 __device__ inline void load_tables(F64Lds<kBT, kW>& L)
 {
-    const double* src = &kErfcTab[0][0];
-    for (int i = threadIdx.x; i < (kErfcN + 1) * (kErfcDeg + 1); i += blockDim.x) L.etab[i] = src[i];
-    for (int i = threadIdx.x; i < kExp2N; i += blockDim.x) L.e2tab[i] = kExp2Tab[i];
+    for (int i = threadIdx.x; i < kErfcPairs * (kErfcN + 1); i += blockDim.x) {
+        const int j = i / (kErfcN + 1), k = i - j * (kErfcN + 1);
+        L.etab[i] = ErfcPair{kErfcTab[k][2 * j], kErfcTab[k][2 * j + 1]};
+    }
 }
 
 // kBT x kBT tiles by kW waves, kNI = kBT / kW i tiles per wave: wave w holds
@@ -1676,7 +1695,7 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds<kB
             double fx[2], fy[2], fz[2], u[2];
             const double x2[2] = {xi[0], xi[kNI - 1]}, y2[2] = {yi[0], yi[kNI - 1]},
                          z2[2] = {zi[0], zi[kNI - 1]};
-            n3_tile_f64_uni2<kRdf, kKelbg>(a, L.hist, L.etab, L.e2tab, x2, y2, z2, siu[0], sju, jt,
+            n3_tile_f64_uni2<kRdf, kKelbg>(a, L.hist, L.etab, kExp2Tab, x2, y2, z2, siu[0], sju, jt,
                                            fx, fy, fz, u, rx, ry, rz);
 #pragma unroll
             for (int h = 0; h < kNI; ++h) {
@@ -1693,18 +1712,18 @@ __device__ inline void n3_block_f64(const PairArgs& a, int br, int bc, F64Lds<kB
                 const bool diag = dblk && tj == tl[h];
                 if (iuni[h] && juni) {
                     if (diag)
-                        n3_tile_f64_uni<kRdf, kKelbg, true>(a, L.hist, L.etab, L.e2tab, xi[h], yi[h],
+                        n3_tile_f64_uni<kRdf, kKelbg, true>(a, L.hist, L.etab, kExp2Tab, xi[h], yi[h],
                                                             zi[h], siu[h], sju, jt, fx, fy, fz, u, hx,
                                                             hy, hz);
                     else
-                        n3_tile_f64_uni<kRdf, kKelbg, false>(a, L.hist, L.etab, L.e2tab, xi[h], yi[h],
+                        n3_tile_f64_uni<kRdf, kKelbg, false>(a, L.hist, L.etab, kExp2Tab, xi[h], yi[h],
                                                              zi[h], siu[h], sju, jt, fx, fy, fz, u, hx,
                                                              hy, hz);
                 } else if (diag)
-                    n3_tile_f64<kRdf, kKelbg, true>(a, L.hist, L.etab, L.e2tab, xi[h], yi[h], zi[h],
+                    n3_tile_f64<kRdf, kKelbg, true>(a, L.hist, L.etab, kExp2Tab, xi[h], yi[h], zi[h],
                                                     si[h], ilive[h], jt, fx, fy, fz, u, hx, hy, hz);
                 else
-                    n3_tile_f64<kRdf, kKelbg, false>(a, L.hist, L.etab, L.e2tab, xi[h], yi[h], zi[h],
+                    n3_tile_f64<kRdf, kKelbg, false>(a, L.hist, L.etab, kExp2Tab, xi[h], yi[h], zi[h],
                                                      si[h], ilive[h], jt, fx, fy, fz, u, hx, hy, hz);
                 Fx[h] += fx;
                 Fy[h] += fy;
