@@ -19,6 +19,7 @@
 //
 // RDF counts go to a per-block LDS histogram (3 x 512 uint32) flushed with one
 // global 64-bit atomic per non-empty bin.
+#include "f32_tables.h"
 #include "f64_tables.h"
 #include "tcp_internal.h"
 
@@ -701,10 +702,11 @@ struct MixUnits {
     float rc2, rfix2;        // u^2
     float rnear2;            // u^2: max(rfix2, rclose^2)
     float half_alpha, two_a_sqrtpi, inv_rs, nl2rs;  // x u
-    float inv_half_alpha, two_rs;  // 2/alpha, 2 lambda: / u
+    float two_rs;            // 2 lambda: / u
     float na2l;              // x u^2
     float rclose;            // / u
     float idx_ee, idx_ei, idx_ii;  // RDF bin widths (reciprocal) x u
+    float xrow;              // kErfcxPerUnit alpha: x u (erfcx table row coordinate of r)
     double f_scale, e_scale; // u^-2 (num d / r^3), u^-1 (en / r)
 };
 
@@ -720,7 +722,6 @@ __device__ inline MixUnits mix_units(const PairArgs& a, bool cubic)
     const double lmax = fmax(fmax(a.box[0], a.box[1]), a.box[2]);
     w.rfix2 = (float)(lmax * lmax * 0x1p-16 / (u * u));
     w.half_alpha = (float)(0.5 * a.alpha * u);
-    w.inv_half_alpha = (float)(2.0 / (a.alpha * u));
     w.two_rs = (float)(2.0 * a.rs / u);
     w.two_a_sqrtpi = (float)(2.0 * a.alpha / kSqrtPi * u);
     w.inv_rs = (float)(u / a.rs);
@@ -732,6 +733,7 @@ __device__ inline MixUnits mix_units(const PairArgs& a, bool cubic)
     w.idx_ee = (float)(a.inv_dx[0] * u);
     w.idx_ei = (float)(a.inv_dx[1] * u);
     w.idx_ii = (float)(a.inv_dx[2] * u);
+    w.xrow = (float)(kErfcxPerUnit * a.alpha * u);
     w.f_scale = 1.0 / (u * u);
     w.e_scale = 1.0 / u;
     return w;
@@ -760,20 +762,32 @@ struct MixUni {
     bool in;
 };
 
+// erfcx(x) = e^{x^2} erfc(x) from the LDS copy of kErfcxTab, given
+// u = kErfcxPerUnit x >= 0 below kErfcxRows: row int(u) (the conversion is
+// nearly free beside FMAs), t = fract(u), a cubic by three FMAs; 1.2e-7
+// relative on [0, 4.5].  A NaN u (coincident particles) converts to row 0.
+__device__ inline float erfcx_row(const float4* xtab, float u)
+{
+    const float4 c = xtab[(int)u];
+    const float t = __builtin_amdgcn_fractf(u);
+    return fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+}
+
 // The terms of one pair from its separation o.dx, o.dy, o.dz and r2 (unit u):
 // o.r, o.in, o.er and o.fs.  kClose: a close electron-ion pair (r < rclose)
-// takes num and en from the cancellation-free series.
+// takes num and en from the cancellation-free series.  r is taken as r2 times
+// the masked 1/r, so it is 0 beyond rc, where every term is masked anyway, and
+// inside rc alpha r < alpha rc <= 4.5 keeps the erfcx row in the table.
 template <bool kKelbg, bool kReg, bool kClose>
-__device__ inline void mix_terms(const MixUnits& w, float r2, MixUni& o)
+__device__ inline void mix_terms(const MixUnits& w, const float4* xtab, float r2, MixUni& o)
 {
     o.in = r2 < w.rc2;
     const float rinv = __builtin_amdgcn_rsqf(r2);
-    const float r = r2 * rinv;
     const float rm = o.in ? rinv : 0.f;  // 1/r inside rc, else 0
+    const float r = r2 * rm;             // r inside rc, else 0
     const float t = rm * rinv;           // 1/r^2 inside rc
     const float e2 = __builtin_amdgcn_exp2f(w.na2l * r2);
-    // w = x/(2 + x), x = alpha r, from 1/r: one rcp and no multiply by r
-    const float ec = e2 * erfcx_poly7(__builtin_amdgcn_rcpf(fmaf(rinv, w.inv_half_alpha, 1.0f)));
+    const float ec = e2 * erfcx_row(xtab, r * w.xrow);
     float er, inner;  // en/r and num/r (inside rc)
     if (kReg) {
         const float sr = r * w.inv_rs;
@@ -810,7 +824,8 @@ __device__ inline void mix_terms(const MixUnits& w, float r2, MixUni& o)
 }
 
 template <bool kKelbg, bool kCubic, bool kReg>
-__device__ inline MixUni mix_uni(const MixUnits& w, const FixPos& pi, FixU3 qu, const FixPos& q)
+__device__ inline MixUni mix_uni(const MixUnits& w, const float4* xtab, const FixPos& pi, FixU3 qu,
+                                 const FixPos& q)
 {
     MixUni o;
     const float ix = (float)(int)(pi.ux - qu.x), iy = (float)(int)(pi.uy - qu.y),
@@ -819,7 +834,7 @@ __device__ inline MixUni mix_uni(const MixUnits& w, const FixPos& pi, FixU3 qu, 
     o.dy = kCubic ? iy : iy * w.sy;
     o.dz = kCubic ? iz : iz * w.sz;
     const float r2 = fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz));
-    mix_terms<kKelbg, kReg, false>(w, r2, o);
+    mix_terms<kKelbg, kReg, false>(w, xtab, r2, o);
     // A close pair (rare) is evaluated again: with the FP32 residuals below
     // L 2^-8, where the 32-bit quantisation would exceed FP32 rounding, and,
     // for a regularised pair below rclose, with the series.  One test covers
@@ -828,7 +843,7 @@ __device__ inline MixUni mix_uni(const MixUnits& w, const FixPos& pi, FixU3 qu, 
         o.dx = kCubic ? fmaf(pi.rx - q.rx, w.inv_u, ix) : fmaf(ix, w.sx, pi.rx - q.rx);
         o.dy = kCubic ? fmaf(pi.ry - q.ry, w.inv_u, iy) : fmaf(iy, w.sy, pi.ry - q.ry);
         o.dz = kCubic ? fmaf(pi.rz - q.rz, w.inv_u, iz) : fmaf(iz, w.sz, pi.rz - q.rz);
-        mix_terms<kKelbg, kReg, kReg>(w, fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz)), o);
+        mix_terms<kKelbg, kReg, kReg>(w, xtab, fmaf(o.dx, o.dx, fmaf(o.dy, o.dy, o.dz * o.dz)), o);
     }
     return o;
 }
@@ -838,7 +853,8 @@ __device__ inline MixUni mix_uni(const MixUnits& w, const FixPos& pi, FixU3 qu, 
 // en / r in the unit of w; the caller applies the charges and powers of u.
 template <bool kRdf, bool kKelbg, int kNI, bool kClamp, bool kCubic, bool kReg>
 __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsigned* hist,
-                                     const FixPos (&pi)[kNI], int si, int sj, const FixPos* jpos,
+                                     const float4* xtab, const FixPos (&pi)[kNI], int si, int sj,
+                                     const FixPos* jpos,
                                      int tj, float (&fxo)[kNI], float (&fyo)[kNI],
                                      float (&fzo)[kNI], float (&uo)[kNI], float& rx, float& ry,
                                      float& rz)
@@ -866,7 +882,7 @@ __device__ inline void n3_tile_multi(const MixUnits& w, const PairArgs& a, unsig
         const FixPos& q = *(const FixPos*)(jb + off);
 #pragma unroll
         for (int n = 0; n < kNI; ++n) {
-            const MixUni m = mix_uni<kKelbg, kCubic, kReg>(w, pi[n], qu, q);
+            const MixUni m = mix_uni<kKelbg, kCubic, kReg>(w, xtab, pi[n], qu, q);
             const float g = m.fs;
             fx[n] = fmaf(g, m.dx, fx[n]);
             fy[n] = fmaf(g, m.dy, fy[n]);
@@ -927,7 +943,7 @@ struct N3Stage {
 //  The diagonal block's i sums join racc after its last step.
 template <bool kRdf, bool kKelbg, int kBT, int kW>
 __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
-                                double (*racc)[kBT * kTile], unsigned* hist,
+                                double (*racc)[kBT * kTile], unsigned* hist, const float4* xtab,
                                 N3Stage<kBT, kW>& stg)
 {
     constexpr int kBP = kBT * kTile;  // particles per block side
@@ -1013,8 +1029,8 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
                 // the tiles are single-species: regularised or not for the whole tile pair
                 const bool reg = s0 != sj || ((a.flags & TCP_REG_EE) != 0 && s0 == 0 && sj == 0);
 #define TCP_N3_MULTI(CLAMP, CUBIC, REG)                                                          \
-    n3_tile_multi<kRdf, kKelbg, kNI, CLAMP, CUBIC, REG>(w, a, hist, pi, s0, sj, jpos, tj, fx, fy, \
-                                                        fz, u, rx, ry, rz)
+    n3_tile_multi<kRdf, kKelbg, kNI, CLAMP, CUBIC, REG>(w, a, hist, xtab, pi, s0, sj, jpos, tj, fx, \
+                                                        fy, fz, u, rx, ry, rz)
                 if (cubic) {
                     if (k.rdf_clamp) {
                         if (reg) TCP_N3_MULTI(true, true, true);
@@ -1180,6 +1196,16 @@ __device__ inline void queue_done(int* queue)
             atomicExch(queue + 1, 0);
         }
     }
+}
+
+static_assert(kErfcxRows > kErfcxPerUnit * kMixedErfcMax + 1,
+              "the erfcx rows cover alpha r < alpha rc <= kMixedErfcMax with rounding to spare");
+
+// The LDS copy of kErfcxTab of a mixed block (1.2 KB)
+__device__ inline void load_erfcx(float4* xtab)
+{
+    for (int i = threadIdx.x; i < kErfcxRows; i += blockDim.x)
+        xtab[i] = make_float4(kErfcxTab[i][0], kErfcxTab[i][1], kErfcxTab[i][2], kErfcxTab[i][3]);
 }
 
 // ------------------------------------------- FP64, Newton's third law ---
@@ -1847,12 +1873,14 @@ __global__ __launch_bounds__(kN3Waves * kTile, 4) void pair_mixed_n3_kernel(Pair
     __shared__ double racc[4][kBlkTiles * kTile];
     __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
     __shared__ N3Stage<kBlkTiles, kN3Waves> stg;
+    __shared__ float4 xtab[kErfcxRows];
     __shared__ int slot;
+    load_erfcx(xtab);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nblk) break;
         const int2 rc = blkmap[it];
-        n3_block<kRdf, kKelbg, kBlkTiles, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist, stg);
+        n3_block<kRdf, kKelbg, kBlkTiles, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist, xtab, stg);
     }
     queue_done(queue);
 }
@@ -1869,12 +1897,14 @@ __global__ __launch_bounds__(kN3Waves * kTile, 4) void pair_mixed_n3_fine_kernel
     __shared__ double racc[4][kBT * kTile];
     __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
     __shared__ N3Stage<kBT, kN3Waves> stg;
+    __shared__ float4 xtab[kErfcxRows];
     __shared__ int slot;
+    load_erfcx(xtab);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nblk) break;
         const int2 rc = blkmap[it];
-        n3_block<kRdf, kKelbg, kBT, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist, stg);
+        n3_block<kRdf, kKelbg, kBT, kN3Waves>(a, rc.x, rc.y, jpos, racc, hist, xtab, stg);
     }
     queue_done(queue);
 }
@@ -1899,12 +1929,14 @@ __global__ __launch_bounds__(kW * kTile, (kBT / kW) > 2 ? 2 : 4) void pair_mixed
     __shared__ double racc[4][kBT * kTile];
     __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
     __shared__ N3Stage<kBT, kW> stg;
+    __shared__ float4 xtab[kErfcxRows];
     __shared__ int slot;
+    load_erfcx(xtab);  // the first queue_next orders it before use
     for (;;) {
         const int it = queue_next(queue, &slot);
         if (it >= nitems) break;
         const PairItem w = items[it];
-        n3_block<kRdf, kKelbg, kBT, kW>(reps[w.rep].pa, w.a, w.b, jpos, racc, hist, stg);
+        n3_block<kRdf, kKelbg, kBT, kW>(reps[w.rep].pa, w.a, w.b, jpos, racc, hist, xtab, stg);
     }
     queue_done(queue);
 }
