@@ -245,34 +245,40 @@ __global__ __launch_bounds__(256) void adjoint_reduce_kernel(const double* part,
 // float(int(u_i - u_j)) s; for pairs closer than L 2^-8 the FP32 residuals
 // restore the FP64 positions to FP32 rounding, dx = float(int(u_i - u_j)) s +
 // (r_i - r_j), so there is no cancellation even for pairs 0.01 A apart in a
-// 50 A box.  erfc(x) = exp(-x^2) P(w), w = (x/2)/(1 + x/2), with P a
+// 50 A box.  The Ewald erfc(x) = exp(-x^2) erfcx(x), erfcx from the LDS table
+// of kErfcxTab (erfcx_row); the Kelbg erfc(r/lambda), whose argument can run
+// far past that table, = exp(-s^2) P(w), w = s/(2 + s), with P a degree-9
 // polynomial with P(0) = 1 fitted here (weighted least squares plus Lawson
-// iterations against exp(x^2) erfc(x) on [0, 4.5]): degree 7 for the Ewald
-// term (erfcx_poly7), degree 9 (relative error 5.6e-9 in exact arithmetic, 3e-8
-// near x = 0 and below 7.2e-7 up to x = 3.2 once evaluated in FP32) for the
-// Kelbg erfc(r/lambda).  exp(-x^2) is shared with the 2x exp(-x^2)/sqrt(pi)
-// force term.
+// iterations against exp(s^2) erfc(s); relative error 5.6e-9 in exact
+// arithmetic, 3e-8 near s = 0 and below 7.2e-7 up to s = 3.2 once evaluated in
+// FP32).  exp(-x^2) is shared with the 2x exp(-x^2)/sqrt(pi) force term.
 // Electron-ion pairs closer than 2 lambda take a rare, divergent branch
 // (close_ei) where the Deutsch and Ewald parts are evaluated without their
 // mutual cancellation.
 // Per-lane sums run in FP32 over tiles of 32 j and go to FP64 after each tile.
 
-// e^{x^2} erfc(x) = P7(w) on the Ewald range: degree 7 with P7(0) = 1, fitted
-// here like erfcx_poly on x in [0, 4.5] (relative error 2.2e-7 in exact
-// arithmetic, 2.3e-7 RMS and 7.5e-7 at most on [0, 3.1] evaluated in FP32).
-__device__ inline float erfcx_poly7(float w)
+static_assert(kErfcxRows > kErfcxPerUnit * kMixedErfcMax + 1,
+              "the erfcx rows cover alpha r < alpha rc <= kMixedErfcMax with rounding to spare");
+
+// The LDS copy of kErfcxTab of a mixed block (1.2 KB)
+__device__ inline void load_erfcx(float4* xtab)
 {
-    float p = -8.387557e-02f;
-    p = fmaf(p, w, 2.5590158e-01f);
-    p = fmaf(p, w, -1.0481428e-01f);
-    p = fmaf(p, w, -2.7200407e-01f);
-    p = fmaf(p, w, -2.820583e-01f);
-    p = fmaf(p, w, 1.7439e+00f);
-    p = fmaf(p, w, -2.2567792e+00f);
-    p = fmaf(p, w, 1.0f);
-    return p;
+    for (int i = threadIdx.x; i < kErfcxRows; i += blockDim.x)
+        xtab[i] = make_float4(kErfcxTab[i][0], kErfcxTab[i][1], kErfcxTab[i][2], kErfcxTab[i][3]);
 }
 
+// erfcx(x) = e^{x^2} erfc(x) from the LDS copy of kErfcxTab, given
+// u = kErfcxPerUnit x >= 0 below kErfcxRows: row int(u) (the conversion is
+// nearly free beside FMAs), t = fract(u), a cubic by three FMAs; 1.2e-7
+// relative on [0, 4.5].  A NaN u (coincident particles) converts to row 0.
+__device__ inline float erfcx_row(const float4* xtab, float u)
+{
+    const float4 c = xtab[(int)u];
+    const float t = __builtin_amdgcn_fractf(u);
+    return fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+}
+
+// e^{s^2} erfc(s) = P(w), w = s/(2 + s), for the Kelbg erfc(r/lambda).
 __device__ inline float erfcx_poly(float w)
 {
     float p = 2.674967051e-02f;
@@ -391,7 +397,7 @@ struct MixConst {
     float sx, sy, sz;          // L_d 2^-32
     float rc2;
     float half_alpha;          // alpha/2
-    float inv_half_alpha;      // 2/alpha: w = x/(2 + x) = 1/(1 + (2/alpha)/r) for x = alpha r
+    float xrow;                // kErfcxPerUnit alpha: erfcx table row coordinate of r
     float two_rs;              // 2 lambda: the same for Kelbg's erfc(r/lambda)
     float na2l;                // -alpha^2 log2(e)
     float two_a_sqrtpi;        // 2 alpha/sqrt(pi)
@@ -412,7 +418,7 @@ __device__ inline MixConst mix_const(const PairArgs& a)
     k.sz = (float)(a.box[2] * 0x1p-32);
     k.rc2 = (float)a.rc2;
     k.half_alpha = (float)(0.5 * a.alpha);
-    k.inv_half_alpha = (float)(2.0 / a.alpha);
+    k.xrow = (float)(kErfcxPerUnit * a.alpha);
     k.two_rs = (float)(2.0 * a.rs);
     k.na2l = (float)(-a.alpha * a.alpha * 1.4426950408889634);
     k.two_a_sqrtpi = (float)(2.0 * a.alpha / kSqrtPi);
@@ -449,8 +455,8 @@ struct MixRaw {
 // kCheck: the pair may be a particle with itself (r = 0) or involve a padding
 // lane (live false); without it only the cut-off masks the pair.
 template <bool kKelbg, bool kCheck>
-__device__ inline MixRaw mix_raw(const MixConst& k, const FixPos& pi, const FixPos& pj, bool reg,
-                                 bool live)
+__device__ inline MixRaw mix_raw(const MixConst& k, const float4* xtab, const FixPos& pi,
+                                 const FixPos& pj, bool reg, bool live)
 {
     MixRaw o;
     o.dx = (float)(int)(pi.ux - pj.ux) * k.sx;
@@ -467,8 +473,8 @@ __device__ inline MixRaw mix_raw(const MixConst& k, const FixPos& pi, const FixP
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
     const float e2 = __builtin_amdgcn_exp2f(k.na2l * r2);
-    // w = x/(2 + x), x = alpha r, from 1/r: one rcp and no multiply by r
-    const float ec = e2 * erfcx_poly7(__builtin_amdgcn_rcpf(fmaf(rinv, k.inv_half_alpha, 1.0f)));
+    // the table row from r inside rc (alpha r < alpha rc <= 4.5), row 0 outside
+    const float ec = e2 * erfcx_row(xtab, (o.in ? r : 0.f) * k.xrow);
     float num = fmaf(k.two_a_sqrtpi * r, e2, ec);
     float en = ec;
     if (reg) {
@@ -497,10 +503,10 @@ __device__ inline MixRaw mix_raw(const MixConst& k, const FixPos& pi, const FixP
 }
 
 template <bool kKelbg, bool kCheck = true>
-__device__ inline MixPair mix_pair(const MixConst& k, const FixPos& pi, const FixPos& pj,
-                                   float qq, float eqq, bool reg, bool live)
+__device__ inline MixPair mix_pair(const MixConst& k, const float4* xtab, const FixPos& pi,
+                                   const FixPos& pj, float qq, float eqq, bool reg, bool live)
 {
-    const MixRaw m = mix_raw<kKelbg, kCheck>(k, pi, pj, reg, live);
+    const MixRaw m = mix_raw<kKelbg, kCheck>(k, xtab, pi, pj, reg, live);
     MixPair o;
     o.dx = m.dx;
     o.dy = m.dy;
@@ -517,10 +523,10 @@ template <bool kRdf, bool kKelbg>
 __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
 {
     __shared__ unsigned hist[kRdf ? 3 * kHistStride : 1];
-    if (kRdf) {
-        hist_clear(hist);
-        __syncthreads();
-    }
+    __shared__ float4 xtab[kErfcxRows];
+    if (kRdf) hist_clear(hist);
+    load_erfcx(xtab);
+    __syncthreads();
     const MixConst k = mix_const(a);
     const int i = blockIdx.x * kPairBlock + threadIdx.x;
     const bool live = i < a.n_i;
@@ -543,7 +549,7 @@ __global__ __launch_bounds__(kPairBlock) void pair_mixed_kernel(PairArgs a)
             float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
 #pragma unroll 4
             for (int j = jt; j < je; ++j) {
-                const MixPair o = mix_pair<kKelbg>(k, pi, a.uj[j], qq, eqq, reg, live);
+                const MixPair o = mix_pair<kKelbg>(k, xtab, pi, a.uj[j], qq, eqq, reg, live);
                 fx = fmaf(o.df, o.dx, fx);
                 fy = fmaf(o.df, o.dy, fy);
                 fz = fmaf(o.df, o.dz, fz);
@@ -613,7 +619,8 @@ __device__ inline float rot_f(float v)
 // wave-uniform and the flags do not travel with j.
 template <bool kRdf, bool kKelbg, bool kDiag, bool kUni>
 __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* hist,
-                               const FixPos& pi, int si, bool ilive, int sj_uni, FixPos pj,
+                               const float4* xtab, const FixPos& pi, int si, bool ilive,
+                               int sj_uni, FixPos pj,
                                float& fxo, float& fyo, float& fzo, float& uo,
                                float& rx, float& ry, float& rz)
 {
@@ -642,7 +649,7 @@ __device__ inline void n3_tile(const MixConst& k, const PairArgs& a, unsigned* h
             live = ilive & ((pj.pad0 & 2u) != 0u);
             reg = si != sj || (regee && si == 0 && sj == 0);
         }
-        const MixPair o = mix_pair<kKelbg>(k, pi, pj, qi * qj, eqi * qj, reg, live);
+        const MixPair o = mix_pair<kKelbg>(k, xtab, pi, pj, qi * qj, eqi * qj, reg, live);
         const float gx = o.df * o.dx, gy = o.df * o.dy, gz = o.df * o.dz;
         fx += gx;
         fy += gy;
@@ -761,17 +768,6 @@ struct MixUni {
     float dx, dy, dz, r, er, fs;
     bool in;
 };
-
-// erfcx(x) = e^{x^2} erfc(x) from the LDS copy of kErfcxTab, given
-// u = kErfcxPerUnit x >= 0 below kErfcxRows: row int(u) (the conversion is
-// nearly free beside FMAs), t = fract(u), a cubic by three FMAs; 1.2e-7
-// relative on [0, 4.5].  A NaN u (coincident particles) converts to row 0.
-__device__ inline float erfcx_row(const float4* xtab, float u)
-{
-    const float4 c = xtab[(int)u];
-    const float t = __builtin_amdgcn_fractf(u);
-    return fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
-}
 
 // The terms of one pair from its separation o.dx, o.dy, o.dz and r2 (unit u):
 // o.r, o.in, o.er and o.fs.  kClose: a close electron-ion pair (r < rclose)
@@ -1069,9 +1065,9 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
                     const int si = ip[h] >= ne;
                     float fx, fy, fz, u, rx, ry, rz;
                     if (uniform_tile(I) && juni)
-                        n3_tile<kRdf, kKelbg, false, true>(k, a, hist, pi[h], si, ilive, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                        n3_tile<kRdf, kKelbg, false, true>(k, a, hist, xtab, pi[h], si, ilive, sj, pj, fx, fy, fz, u, rx, ry, rz);
                     else
-                        n3_tile<kRdf, kKelbg, false, false>(k, a, hist, pi[h], si, ilive, sj, pj, fx, fy, fz, u, rx, ry, rz);
+                        n3_tile<kRdf, kKelbg, false, false>(k, a, hist, xtab, pi[h], si, ilive, sj, pj, fx, fy, fz, u, rx, ry, rz);
                     Fx[h] += (double)fx;
                     Fy[h] += (double)fy;
                     Fz[h] += (double)fz;
@@ -1124,14 +1120,14 @@ __device__ inline void n3_block(const PairArgs& a, int br, int bc, FixPos* jpos,
                 float fx, fy, fz, u, rx, ry, rz;
                 if (tj == ti) {
                     if (uni)
-                        n3_tile<kRdf, kKelbg, true, true>(k, a, hist, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                        n3_tile<kRdf, kKelbg, true, true>(k, a, hist, xtab, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
                     else
-                        n3_tile<kRdf, kKelbg, true, false>(k, a, hist, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                        n3_tile<kRdf, kKelbg, true, false>(k, a, hist, xtab, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
                 } else {
                     if (uni)
-                        n3_tile<kRdf, kKelbg, false, true>(k, a, hist, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                        n3_tile<kRdf, kKelbg, false, true>(k, a, hist, xtab, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
                     else
-                        n3_tile<kRdf, kKelbg, false, false>(k, a, hist, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
+                        n3_tile<kRdf, kKelbg, false, false>(k, a, hist, xtab, pi[h], si, ilive, sj_uni, pj, fx, fy, fz, u, rx, ry, rz);
                     react(tj, rx, ry, rz, 1.0);
                 }
                 Fx[h] += (double)fx;
@@ -1198,15 +1194,6 @@ __device__ inline void queue_done(int* queue)
     }
 }
 
-static_assert(kErfcxRows > kErfcxPerUnit * kMixedErfcMax + 1,
-              "the erfcx rows cover alpha r < alpha rc <= kMixedErfcMax with rounding to spare");
-
-// The LDS copy of kErfcxTab of a mixed block (1.2 KB)
-__device__ inline void load_erfcx(float4* xtab)
-{
-    for (int i = threadIdx.x; i < kErfcxRows; i += blockDim.x)
-        xtab[i] = make_float4(kErfcxTab[i][0], kErfcxTab[i][1], kErfcxTab[i][2], kErfcxTab[i][3]);
-}
 
 // ------------------------------------------- FP64, Newton's third law ---
 //
