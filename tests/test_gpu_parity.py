@@ -98,6 +98,22 @@ def test_mixed_vs_fp64_c0():
     assert np.abs(rdf32 - rdf64).sum() <= 1e-3 * rdf64.sum()
 
 
+@pytest.mark.parametrize("ni", [4096, 20000])
+def test_mixed_energy_bias_single_species(ni):
+    """Ions only: every pair energy erfc(alpha r)/r is positive, so Ur cancels
+    nothing and the mixed Ur's error against FP64 is the systematic per-pair
+    bias of the FP32 evaluation (the erfcx rows, the FP32 constants), which in
+    a near-neutral plasma would add up across the many pairs whose signed sum
+    Ur is.  Held to 4e-8 (measured 1.3e-8 with the erfcx table; the degree-7
+    polynomial it replaced gave 9.7e-8)."""
+    say("case mixed energy bias, ions", ni)
+    p, x, _ = make_plasma(0, ni, seed=11, recip=RECIP_NONE)
+    u64 = direct(p, x, FP64)[1]
+    u32 = direct(p, x, MIXED)[1]
+    say("Ur", u64, "relative error", (u32 - u64) / u64)
+    assert abs(u32 - u64) <= 4e-8 * abs(u64)
+
+
 def test_direct_ext_vs_reference():
     say("case direct_ext")
     p, d = load("ext")
