@@ -90,7 +90,7 @@ def test_parameter_errors(change, code):
     (float("nan"), MIXED, -1)])
 def test_free_alpha_range(arc, precision, code):
     """alpha > 0 overrides 3.1/rc (force.f90:23); a mixed context's alpha*rc must stay in
-    [0, 4.5], the range of its FP32 erfc fit (FP64 has no such bound)."""
+    [0, 4.5], the range of its FP32 erfcx table (FP64 has no such bound)."""
     p, _, _ = make_plasma(10, 10)
     rc, msg, h = _create(p.replace(alpha=arc / p.rc), precision)
     if code == 0:  # past validation: created, or stopped at the first device call

@@ -19,7 +19,7 @@ import pytest
 from oracle import cpu
 from tcp_allpair_amd.gpu import Context
 from tcp_allpair_amd.inputs import make_plasma
-from tcp_allpair_amd.params import FP64, MIXED, RECIP_EWALD, RECIP_SPME, STOCH, reduced_dt
+from tcp_allpair_amd.params import FP64, MIXED, RECIP_EWALD, RECIP_NONE, RECIP_SPME, STOCH, reduced_dt
 
 pytestmark = pytest.mark.gpu
 
@@ -92,6 +92,46 @@ def test_free_alpha_spme_arc35():
     fm, _, _, _, _ = forces(p, x, MIXED)
     say("mixed rms rel err", rms_rel(fm, f))
     assert rms_rel(fm, f) <= 1e-6
+
+
+@pytest.mark.parametrize("cubic", [True, False])
+def test_mixed_at_the_alpha_rc_limit(cubic):
+    """alpha rc = 4.5, the largest a mixed context takes: pairs near rc read the
+    top rows of the mixed kernels' erfcx table (x up to 4.5).  3000 + 3000
+    particles, rc = L/2 (the shortest edge's half in the box of aspect
+    1 : 1.25 : 0.8), no reciprocal part: FP64 against the oracle, mixed against
+    FP64 to 1e-6 RMS and Ur to 1e-7 of the sum of |qi qj|/r."""
+    say("case alpha rc 4.5, cubic", cubic)
+    p, x, _ = make_plasma(3000, 3000, seed=4500, recip=RECIP_NONE)
+    if not cubic:
+        box = np.asarray(p.box) * np.array([1.0, 1.25, 0.8])
+        x = x * np.array([1.0, 1.25, 0.8])
+        p = p.replace(box=tuple(box), rc=0.5 * box.min())
+    p = p.replace(alpha=4.5 / p.rc)
+    fo, uro, rdfo = cpu.force_allpair(p, x)
+    with Context(p, FP64) as c:
+        c.set_state(x=x, f=np.zeros_like(x))
+        ur = c.force_direct()
+        f = c.get_state()[2]
+        rdf, _ = c.histograms()
+    assert rel_err(f, fo).max() <= 1e-10
+    assert abs(ur - uro) <= 1e-12 * abs(uro)
+    np.testing.assert_array_equal(rdf, rdfo)
+    with Context(p, MIXED) as c:
+        c.set_state(x=x, f=np.zeros_like(x))
+        urm = c.force_direct()
+        fm = c.get_state()[2]
+    # sum over pairs inside rc of |qi qj|/r, which the near-neutral Ur cancels
+    L = np.asarray(p.box)
+    scale = 0.0
+    for i0 in range(0, p.n, 512):
+        d = x[i0:i0 + 512, None, :] - x[None, :, :]
+        d -= L * np.rint(d / L)
+        r = np.sqrt((d * d).sum(-1))
+        scale += 0.5 * (1.0 / r[(r < p.rc) & (r > 0)]).sum()
+    say("mixed rms rel err", rms_rel(fm, f), "dUr", urm - ur, "scale", scale)
+    assert rms_rel(fm, f) <= 1e-6
+    assert abs(urm - ur) <= 1e-7 * scale
 
 
 def test_fullsize_fp64_rows_vs_oracle():
