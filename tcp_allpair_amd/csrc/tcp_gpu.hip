@@ -597,13 +597,14 @@ int validate_params(const tcp_params* p, int precision, std::string* msg)
     if (p->rc > rc_cut) return vfail(msg, TCP_ERR_CUTOFF, " *** error at cut-off radius *** ");
     if (std::isnan(p->alpha) || std::isinf(p->alpha))
         return vfail(msg, TCP_ERR_ARG, "alpha must be finite (<= 0 selects 3.1/rc)");
-    // the mixed pair kernel's erfc(x) = exp(-x^2) P7(w) is fitted on x = alpha r in
-    // [0, kMixedErfcMax]; every pair it evaluates has r < rc
+    // the mixed pair kernels' erfc(x) = exp(-x^2) erfcx(x) reads erfcx from a table
+    // of rows up to just past x = alpha r = kMixedErfcMax; every pair they evaluate
+    // has r < rc
     if (precision == TCP_MIXED) {
         const double a = p->alpha > 0 ? p->alpha : 3.1 / p->rc;
         if (a * p->rc > kMixedErfcMax)
             return vfail(msg, TCP_ERR_UNSUPPORTED,
-                         "mixed precision needs alpha*rc <= 4.5, the range of its erfc fit");
+                         "mixed precision needs alpha*rc <= 4.5, the range of its erfcx table");
     }
     if (p->recip == TCP_RECIP_SPME) {
         for (int d = 0; d < 3; ++d)

@@ -14,7 +14,7 @@ constexpr int kHistStride = 512;       // padded bins per histogram in device me
 constexpr int kPairBlock = 256;        // threads (i particles) per pair-kernel block
 constexpr int kRedBlock = 256;         // threads per block of the O(N) kernels
 constexpr int64_t kMaxParticles = int64_t(1) << 22;  // per context (block-row work items stay in int)
-constexpr double kMixedErfcMax = 4.5;  // upper end of the mixed kernel's erfc fit in x = alpha r
+constexpr double kMixedErfcMax = 4.5;  // the mixed kernels' erfcx table covers x = alpha r up to it
 
 // Fixed-point position for the mixed pair kernel: u = round((x/L + 1/2) 2^32)
 // modulo 2^32 per axis and the FP32 residual r = x - (u 2^-32 - 1/2) L of the
