@@ -275,13 +275,15 @@ def test_c0_nve_energy_and_drift_vs_reference():
     energy drift (least-squares slope of E_tot per step over |E_tot(0)|, about
     1.06e-9 here) to 1e-15 per step; mixed precision: E_tot to 1e-7 of
     |E_tot(0)| at every step and the drift to 5e-11 per step (5% of it).  The
-    FP32 pair energies carry systematic biases of a few 1e-8 (the FP32
-    constants 16 alpha, alpha^2 log2(e), 1/lambda, the erfcx rows), and while
-    this run's electron-ion pairs form they add a trend to E_tot against the
-    FP64 run: 3.0e-11 per step with the erfcx table (the scatter of E_tot about
-    its trend, 2.9e-9, puts 3.5e-12 of noise on the slope), 2.8e-13 with the
-    earlier degree-7 polynomial, whose biases happened to cancel here while
-    its Ur error at 131072 particles was twenty times the table's."""
+    mixed run's slope differs from the reference's by 3.0e-11 per step (the
+    scatter of E_tot about its trend, 2.9e-9, puts 3.5e-12 of noise on it): its
+    Ur evaluated on the FP64 run's own positions is off by a nearly constant
+    -3.5e-8 of E_tot (a trend of 1.2e-12 per step), so the difference is the
+    mixed trajectory's own integration drift, lower than the FP64 run's; the
+    earlier degree-7 polynomial for erfcx had the same deficit (5.4e-11) and a
+    Ur bias trending by +5.4e-11 per step, which happened to cancel it to
+    2.8e-13.  FP64 runs started from positions perturbed by 1e-7 move the slope
+    by at most 8e-13."""
     p, d = load("traj_c0")
     p = p.replace(recip=RECIP_NONE)
     ref = d["out_trace"]  # Ur, Epot, mv2el, mv2ion
