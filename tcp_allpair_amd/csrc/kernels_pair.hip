@@ -245,13 +245,10 @@ __global__ __launch_bounds__(256) void adjoint_reduce_kernel(const double* part,
 // float(int(u_i - u_j)) s; for pairs closer than L 2^-8 the FP32 residuals
 // restore the FP64 positions to FP32 rounding, dx = float(int(u_i - u_j)) s +
 // (r_i - r_j), so there is no cancellation even for pairs 0.01 A apart in a
-// 50 A box.  The Ewald erfc(x) = exp(-x^2) erfcx(x), erfcx from the LDS table
-// of kErfcxTab (erfcx_row); the Kelbg erfc(r/lambda), whose argument can run
-// far past that table, = exp(-s^2) P(w), w = s/(2 + s), with P a degree-9
-// polynomial with P(0) = 1 fitted here (weighted least squares plus Lawson
-// iterations against exp(s^2) erfc(s); relative error 5.6e-9 in exact
-// arithmetic, 3e-8 near s = 0 and below 7.2e-7 up to s = 3.2 once evaluated in
-// FP32).  exp(-x^2) is shared with the 2x exp(-x^2)/sqrt(pi) force term.
+// 50 A box.  erfc(x) = exp(-x^2) erfcx(x), erfcx from the LDS table of
+// kErfcxTab (erfcx_row), for the Ewald erfc(alpha r) and the Kelbg
+// erfc(r/lambda) alike; exp(-x^2) is shared with the 2x exp(-x^2)/sqrt(pi)
+// force term.
 // Electron-ion pairs closer than 2 lambda take a rare, divergent branch
 // (close_ei) where the Deutsch and Ewald parts are evaluated without their
 // mutual cancellation.
@@ -278,20 +275,14 @@ __device__ inline float erfcx_row(const float4* xtab, float u)
     return fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
 }
 
-// e^{s^2} erfc(s) = P(w), w = s/(2 + s), for the Kelbg erfc(r/lambda).
-__device__ inline float erfcx_poly(float w)
+// The row coordinate of the Kelbg erfc(s), s = r/lambda, which can run far
+// past the table: clamped inside the last row.  Beyond s = 4.6 the factor
+// e^{-s^2} it is taken with is below 6e-10, so the clamped row's error there
+// is far below FP32 rounding of the pair's terms; a NaN s (coincident
+// particles) gives the last row too.
+__device__ inline float erfcx_u_clamped(float s)
 {
-    float p = 2.674967051e-02f;
-    p = fmaf(p, w, -1.345596015e-01f);
-    p = fmaf(p, w, 1.725731939e-01f);
-    p = fmaf(p, w, 4.970539361e-03f);
-    p = fmaf(p, w, 3.443385288e-02f);
-    p = fmaf(p, w, -3.166950941e-01f);
-    p = fmaf(p, w, -2.740879953e-01f);
-    p = fmaf(p, w, 1.743201137e+00f);
-    p = fmaf(p, w, -2.256757498e+00f);
-    p = fmaf(p, w, 1.0f);
-    return p;
+    return fminf(s * (float)kErfcxPerUnit, (float)kErfcxRows - 1e-3f);
 }
 
 // Close electron-ion pair (r < 2 lambda), where erfc + g and e^-s (1 + s) are
@@ -398,7 +389,6 @@ struct MixConst {
     float rc2;
     float half_alpha;          // alpha/2
     float xrow;                // kErfcxPerUnit alpha: erfcx table row coordinate of r
-    float two_rs;              // 2 lambda: the same for Kelbg's erfc(r/lambda)
     float na2l;                // -alpha^2 log2(e)
     float two_a_sqrtpi;        // 2 alpha/sqrt(pi)
     float inv_rs;              // 1/lambda
@@ -419,7 +409,6 @@ __device__ inline MixConst mix_const(const PairArgs& a)
     k.rc2 = (float)a.rc2;
     k.half_alpha = (float)(0.5 * a.alpha);
     k.xrow = (float)(kErfcxPerUnit * a.alpha);
-    k.two_rs = (float)(2.0 * a.rs);
     k.na2l = (float)(-a.alpha * a.alpha * 1.4426950408889634);
     k.two_a_sqrtpi = (float)(2.0 * a.alpha / kSqrtPi);
     k.inv_rs = (float)(1.0 / a.rs);
@@ -482,7 +471,7 @@ __device__ inline MixRaw mix_raw(const MixConst& k, const float4* xtab, const Fi
         if (kKelbg) {
             const float y2 = __builtin_amdgcn_exp2f(-1.4426950408889634f * sr * sr);
             // erfc(s) = exp(-s^2) P(s/(2 + s)), s/(2 + s) = 1/(1 + 2 lambda/r)
-            const float ecs = y2 * erfcx_poly(__builtin_amdgcn_rcpf(fmaf(rinv, k.two_rs, 1.0f)));
+            const float ecs = y2 * erfcx_row(xtab, erfcx_u_clamped(sr));
             num -= y2;
             en = fmaf(kSqrtPiF * sr, ecs, en - y2);
             if (o.in && r < k.rclose)
@@ -709,7 +698,6 @@ struct MixUnits {
     float rc2, rfix2;        // u^2
     float rnear2;            // u^2: max(rfix2, rclose^2)
     float half_alpha, two_a_sqrtpi, inv_rs, nl2rs;  // x u
-    float two_rs;            // 2 lambda: / u
     float na2l;              // x u^2
     float rclose;            // / u
     float idx_ee, idx_ei, idx_ii;  // RDF bin widths (reciprocal) x u
@@ -729,7 +717,6 @@ __device__ inline MixUnits mix_units(const PairArgs& a, bool cubic)
     const double lmax = fmax(fmax(a.box[0], a.box[1]), a.box[2]);
     w.rfix2 = (float)(lmax * lmax * 0x1p-16 / (u * u));
     w.half_alpha = (float)(0.5 * a.alpha * u);
-    w.two_rs = (float)(2.0 * a.rs / u);
     w.two_a_sqrtpi = (float)(2.0 * a.alpha / kSqrtPi * u);
     w.inv_rs = (float)(u / a.rs);
     w.nl2rs = (float)(-1.4426950408889634 * u / a.rs);
@@ -790,7 +777,7 @@ __device__ inline void mix_terms(const MixUnits& w, const float4* xtab, float r2
         if (kKelbg) {
             const float y2 = __builtin_amdgcn_exp2f(-1.4426950408889634f * sr * sr);
             // erfc(s) = exp(-s^2) P(s/(2 + s)), s/(2 + s) = 1/(1 + 2 lambda/r)
-            const float ecs = y2 * erfcx_poly(__builtin_amdgcn_rcpf(fmaf(rinv, w.two_rs, 1.0f)));
+            const float ecs = y2 * erfcx_row(xtab, erfcx_u_clamped(sr));
             er = fmaf(kSqrtPiF * sr, ecs, ec - y2) * rm;
             inner = fmaf(-kSqrtPiF * w.inv_rs, ecs, fmaf(w.two_a_sqrtpi, e2, er));
             if (kClose && o.in && r < w.rclose) {
